@@ -1,0 +1,590 @@
+"""CPU FP64 restatement of ALF's finite-temperature DQMC sweep.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker the GPU engine is
+compared against; nothing in the shipped path imports it.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use it.
+
+It follows the reference's call order and arithmetic (LAPACK via scipy):
+
+  sweep control ............ Prog/main.F90:589-631 (init), 714-887 (sequential sweep)
+  WRAPGRUP / WRAPGRDO ...... Prog/Wrapgr_mod.F90:81-156, 160-243
+  Upgrade2 ................. Prog/upgrade_mod.F90:105-302 (1x1 operators)
+  Op_Wrapup/Op_Wrapdo ...... Prog/Operator_mod.F90:743-835, 862-951 (diagonal branch)
+  Op_mmultR ................ Prog/Operator_mod.F90:648-717 (diagonal branch)
+  Op_phase ................. Prog/Operator_mod.F90:160-181
+  Hop_mod_* ................ Prog/Hop_mod.F90:143-250 with OpTTypes_mod.F90:150-304
+  WRAPUR / WRAPUL .......... Prog/wrapur_mod.F90:101-126, Prog/wrapul_mod.F90:108-132
+  decompose ................ Prog/udv_state_mod.F90:448-582 (default and STABLOG-style
+                             norm-presorted pivoting)
+  QDRP_decompose ........... Prog/QDRP_decompose_mod.F90:60-126
+  CGR (STAB3) .............. Prog/cgr1_mod.F90:174-447
+  DET_C .................... Libraries/Modules/mymats_mod.F90:1334-1366
+  Control_PrecisionG/P ..... Prog/control_mod.F90:207-320, COMPARE_C mymats_mod.F90:683-704
+  RNG consumption .......... 3 uniforms per type-2 field visit (nranf(3), T0 proposal,
+                             Metropolis), random_wrap_mod.F90:135-168
+
+Pinning (see tests/test_oracle_*.py): brute-force (1 + B...B)^-1 in mpmath for tiny
+lattices (CGR, WRAPUR/WRAPUL, decompose, phase), the Sherman-Morrison identity
+against a from-scratch Green's function with one field changed (Upgrade2, ratio),
+QR invariants (QDRP), closed-form hopping spectra (exp(-dtau T)), closed forms of
+the field tables.  A function without such a check says so in its docstring.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.linalg import lapack
+
+_S6 = math.sqrt(6.0)
+PHI2 = {-2: -math.sqrt(2.0 * (3.0 + _S6)), -1: -math.sqrt(2.0 * (3.0 - _S6)),
+        1: math.sqrt(2.0 * (3.0 - _S6)), 2: math.sqrt(2.0 * (3.0 + _S6))}
+GAMA2 = {-2: 1.0 - _S6 / 3.0, 2: 1.0 - _S6 / 3.0, -1: 1.0 + _S6 / 3.0, 1: 1.0 + _S6 / 3.0}
+FLIP2 = {-2: (-1, 1, 2), -1: (1, 2, -2), 1: (2, -2, -1), 2: (-2, -1, 1)}
+
+
+def phi(s, typ):
+    return float(s) if typ == 1 else PHI2[int(s)]
+
+
+def gama(s, typ):
+    return 1.0 if typ == 1 else GAMA2[int(s)]
+
+
+def nranf(u, n):
+    x = u * n + 0.5
+    k = int(math.floor(x + 0.5))
+    return min(max(k, 1), n)
+
+
+@dataclass
+class OracleModel:
+    """Everything the sweep reads from the plugin, as plain arrays.
+
+    ``mat``/``invmat`` are exp(-dtau T) and exp(+dtau T) (dense, symmetrised);
+    operators are 1x1 (``op_P`` site, ``op_E`` eigenvalue, ``op_g``, ``op_alpha``,
+    ``op_type`` in {1, 2}).
+    """
+    ndim: int
+    ltrot: int
+    nwrap: int
+    n_sun: int
+    mat: np.ndarray
+    invmat: np.ndarray
+    op_P: np.ndarray
+    op_E: np.ndarray
+    op_g: np.ndarray
+    op_alpha: np.ndarray
+    op_type: np.ndarray
+
+    @property
+    def n_op(self):
+        return len(self.op_P)
+
+    def e_exp(self, n, sp):
+        """E_exp(1, sp) of operator n (Operator_mod.F90:436-466)."""
+        s = abs(int(sp))
+        z = np.exp(self.op_g[n] * self.op_E[n] * phi(s, self.op_type[n]))
+        return z if sp > 0 else 1.0 / z
+
+    def stab_nt(self):
+        """Stab_nt(0:NSTM) (main.F90:446-457)."""
+        L, W = self.ltrot, self.nwrap
+        nstm = L // W if L % W == 0 else L // W + 1
+        s = [0] + [W * n for n in range(1, nstm)] + [L]
+        return nstm, s
+
+
+class Tape:
+    """Uniform random numbers consumed in order (stands in for ranf_wrap)."""
+
+    def __init__(self, u):
+        self.u = np.asarray(u, dtype=np.float64)
+        self.pos = 0
+
+    def __call__(self):
+        if self.pos >= self.u.size:
+            raise IndexError("random tape exhausted")
+        v = float(self.u[self.pos])
+        self.pos += 1
+        return v
+
+
+@dataclass
+class UDV:
+    """UDV_State (udv_state_mod.F90:85-110): U (n x n), D (n), V (n x n), side."""
+    U: np.ndarray
+    D: np.ndarray
+    V: np.ndarray
+    side: str
+    detU: complex = 1.0 + 0j   # det(U); carried for cross-checks only
+
+    @classmethod
+    def identity(cls, n, side):
+        return cls(np.eye(n, dtype=complex), np.ones(n, dtype=complex), np.eye(n, dtype=complex), side)
+
+    def copy(self):
+        return UDV(self.U.copy(), self.D.copy(), self.V.copy(), self.side, self.detU)
+
+
+@dataclass
+class Control:
+    """Counters and precision statistics (control_mod.F90:53-70, 164-176, 207-320)."""
+    NC_up: int = 0
+    ACC_up: int = 0
+    NC_eff_up: int = 0
+    ACC_eff_up: int = 0
+    NCG: int = 0
+    XMEANG: float = 0.0
+    XMAXG: float = 0.0
+    XMAXP: float = 0.0
+    history: list = field(default_factory=list)
+
+    def upgrade(self, toggle):
+        self.NC_up += 1
+        self.ACC_up += int(toggle)
+        self.NC_eff_up += 1
+        self.ACC_eff_up += int(toggle)
+
+    def precision_g(self, A, B):
+        if np.any(np.isnan(A)) or np.any(np.isnan(B)):
+            raise FloatingPointError("Green function contains NaN, calculation is being aborted!")
+        self.NCG += 1
+        diff = np.abs(A - B)
+        xmax = float(diff.max())
+        xmean = float(diff.sum() / diff.size)
+        if xmax > 10.0:
+            raise FloatingPointError(f"{xmax} is exceeding the threshold of 10 for G difference!")
+        self.XMAXG = max(self.XMAXG, xmax)
+        self.XMEANG += xmean
+        self.history.append((xmax, xmean))
+        return xmax, xmean
+
+    def precision_p(self, z, z1):
+        self.XMAXP = max(self.XMAXP, abs(z - z1))
+
+    @property
+    def precision_green_mean(self):
+        return self.XMEANG / self.NCG if self.NCG else 0.0
+
+
+def pivot_phase(ipvt):
+    """Pivot_phase (QDRP_decompose_mod.F90:103-126): sign of the permutation
+    given in LAPACK's 1-based 'column j of A*P is column IPVT(j) of A' form."""
+    n = len(ipvt)
+    visited = np.zeros(n, bool)
+    sign = 1.0
+    for i in range(n):
+        if not visited[i]:
+            nxt, L = i, 0
+            while not visited[nxt]:
+                L += 1
+                visited[nxt] = True
+                nxt = int(ipvt[nxt]) - 1
+            if L % 2 == 0:
+                sign = -sign
+    return sign
+
+
+def qdrp_decompose(mat, pivot=True):
+    """QDRP_decompose (QDRP_decompose_mod.F90:60-101).
+
+    ZGEQP3 of ``mat`` (column pivoting when ``pivot``; with all IPVT preset to 1
+    it degenerates to an unpivoted QR).  Returns (qr, D, ipvt, tau) where the upper
+    triangle of ``qr`` holds R with row i divided by D(i) = |R_ii| and the strict
+    lower triangle the Householder vectors; ipvt is 1-based.
+    """
+    a = np.array(mat, dtype=complex, order="F")
+    n = a.shape[1]
+    if pivot:
+        qr, jpvt, tau, work, info = lapack.zgeqp3(a, lwork=max(64 * (n + 1), 1))
+    else:
+        # ZGEQP3 with every JPVT preset non-zero performs no pivoting: plain ZGEQRF
+        qr, tau, work, info = lapack.zgeqrf(a, lwork=max(64 * n, 1))
+        jpvt = np.arange(1, n + 1)
+    if info != 0:
+        raise RuntimeError(f"QR info={info}")
+    D = np.zeros(n, dtype=complex)
+    for i in range(n):
+        x = abs(qr[i, i])
+        D[i] = x
+        qr[i, i:] = qr[i, i:] / x
+    return qr, D, np.asarray(jpvt, dtype=np.int64), tau
+
+
+def householder_det(tau):
+    """Determinant of Q = H_1...H_n for LAPACK reflectors (cgr1_mod.F90:331-349):
+    det(1 - tau v v^H) = 1 - 2 (tau/|tau|)(Re tau/|tau|)."""
+    z = 1.0 + 0j
+    for t in tau:
+        if t != 0:
+            x = abs(t)
+            z *= 1.0 - 2.0 * (t / x) * (t.real / x)
+    return z
+
+
+def form_q(qr, tau):
+    q, work, info = lapack.zungqr(np.array(qr, order="F"), tau)
+    if info != 0:
+        raise RuntimeError(f"zungqr info={info}")
+    return q
+
+
+def decompose(udv: UDV, presort: bool = False):
+    """decompose_UDV_state (udv_state_mod.F90:448-582), finite temperature.
+
+    ``presort=False``: the default build (ZGEQP3 pivoting, 494-500).
+    ``presort=True``: the column pivots are chosen beforehand by descending norm of
+    U*D and the QR itself is unpivoted -- the STABLOG branch's pivoting strategy
+    (udv_state_mod.F90:503-540) in linear scale; this is the strategy the GPU
+    engine uses.
+    """
+    n = udv.U.shape[0]
+    A = udv.U * udv.D[None, :]
+    if presort:
+        norms = np.linalg.norm(A, axis=0)
+        ipvt = np.zeros(n, dtype=np.int64)
+        perm = list(range(n))
+        work = A.copy()
+        sgn = 1.0
+        for i in range(n):
+            pvt = i
+            for j in range(i + 1, n):
+                if norms[j] > norms[pvt]:
+                    pvt = j
+            if pvt != i:
+                work[:, [i, pvt]] = work[:, [pvt, i]]
+                norms[[i, pvt]] = norms[[pvt, i]]
+                perm[i], perm[pvt] = perm[pvt], perm[i]
+                sgn = -sgn
+        qr, D, _, tau = qdrp_decompose(work, pivot=False)
+        ipvt = np.array(perm, dtype=np.int64) + 1   # column i of A*P is column ipvt(i) of A
+        phase = np.prod(np.diag(qr)) * sgn
+    else:
+        qr, D, ipvt, tau = qdrp_decompose(A, pivot=True)
+        phase = np.prod(np.diag(qr)) * pivot_phase(ipvt)
+    if udv.side.upper() == "L":
+        phase = np.conj(phase)
+    R = np.triu(qr)
+    R[0, :] = R[0, :] / phase
+    p0 = ipvt - 1
+    if udv.side.upper() == "R":
+        Vp = udv.V[p0, :]          # ZLAPMR forward: row i <- row ipvt(i)
+        V = R @ Vp
+    else:
+        Vp = udv.V[:, p0]          # ZLAPMT forward: col j <- col ipvt(j)
+        V = Vp @ R.conj().T
+    Q = form_q(qr, tau)
+    Q[:, 0] *= phase
+    detU = householder_det(tau) * phase
+    return UDV(Q, D, V, udv.side, detU)
+
+
+def det_c(mat):
+    """DET_C (mymats_mod.F90:1334-1366) via ZGETRF."""
+    lu, piv, info = lapack.zgetrf(np.array(mat, order="F"))
+    d = np.prod(np.diag(lu))
+    sgn = 1
+    for i, p in enumerate(piv):
+        if p != i:
+            sgn = -sgn
+    return -d if sgn == -1 else d
+
+
+def cgr(nvar, udvr: UDV, udvl: UDV, presort: bool = False):
+    """CGR with the STAB3 scale separation (cgr1_mod.F90:174-447).
+
+    Returns (phase, G) with G = (1 + UR DR VR VL DL UL^H)^-1 and phase the phase of
+    det(1 + B(tau,0) B(beta,tau)).  ``presort`` selects norm-presorted pivoting for
+    the QR of the middle matrix (the GPU engine's strategy) instead of ZGEQP3's.
+    """
+    n = udvl.U.shape[0]
+    RHS = udvr.U.conj().T @ udvl.U
+    TP = udvr.V @ udvl.V
+    DR = udvr.D.real
+    DL = udvl.D.real
+    rs = np.where(DR <= 1.0, DR, 1.0)       # DR_-
+    rb = np.where(DR <= 1.0, 1.0, DR)       # DR_+
+    ls = np.where(DL <= 1.0, DL, 1.0)
+    lb = np.where(DL <= 1.0, 1.0, DL)
+    TPUP = RHS / rb[:, None] / lb[None, :] + rs[:, None] * TP * ls[None, :]
+    phase = np.conj(det_c(RHS))
+    phase = phase / abs(phase)
+    if nvar != 1:
+        TPUP = TPUP.conj().T
+    if presort:
+        norms = np.linalg.norm(TPUP, axis=0)
+        perm = list(range(n))
+        work = TPUP.copy()
+        for i in range(n):
+            pvt = i
+            for j in range(i + 1, n):
+                if norms[j] > norms[pvt]:
+                    pvt = j
+            if pvt != i:
+                work[:, [i, pvt]] = work[:, [pvt, i]]
+                norms[[i, pvt]] = norms[[pvt, i]]
+                perm[i], perm[pvt] = perm[pvt], perm[i]
+        qr, DUP, _, tau = qdrp_decompose(work, pivot=False)
+        ipvt = np.array(perm, dtype=np.int64) + 1
+    else:
+        qr, DUP, ipvt, tau = qdrp_decompose(TPUP, pivot=True)
+    phase *= pivot_phase(ipvt)
+    for i in range(n):
+        z = tau[i]
+        if nvar == 1:
+            phase *= qr[i, i] / abs(qr[i, i])
+        else:
+            phase *= np.conj(qr[i, i]) / abs(qr[i, i])
+            z = np.conj(z)
+        if z != 0:
+            x = abs(z)
+            z = 1.0 - 2.0 * (z / x) * (z.real / x)
+            phase *= z / abs(z)
+    Q = form_q(qr, tau)
+    R = np.triu(qr)
+    p0 = ipvt - 1
+    if nvar == 1:
+        rhs = udvr.U.conj().T / rb[:, None]
+        rhs = Q.conj().T @ rhs
+        rhs = rhs / DUP[:, None]
+        rhs = sla.solve_triangular(R, rhs, lower=False)
+        out = np.empty_like(rhs)
+        out[p0, :] = rhs               # ZLAPMR backward: row ipvt(i) <- row i
+        out = out / lb[:, None]
+        G = udvl.U @ out
+    else:
+        rhs = udvl.U / lb[None, :]
+        rhs = rhs @ Q
+        rhs = rhs / DUP[None, :]
+        # y R^H = rhs  <=>  R y^H = rhs^H
+        rhs = sla.solve_triangular(R, rhs.conj().T, lower=False).conj().T
+        out = np.empty_like(rhs)
+        out[:, p0] = rhs               # ZLAPMT backward: col ipvt(j) <- col j
+        out = out / rb[None, :]
+        G = out @ udvr.U.conj().T
+    return phase, G
+
+
+class OracleSweep:
+    """The sequential finite-temperature sweep of main.F90 on one chain (N_FL = 1)."""
+
+    def __init__(self, model: OracleModel, fields, rng, presort=False):
+        self.m = model
+        self.f = np.array(fields, dtype=np.int64).copy()     # (n_op, ltrot)
+        self.rng = rng
+        self.presort = presort
+        self.ctrl = Control()
+        self.nstm, self.stab = model.stab_nt()
+        n = model.ndim
+        self.udvl = UDV.identity(n, "L")
+        self.udvr = UDV.identity(n, "R")
+        self.udvst = [None] + [UDV.identity(n, "L") for _ in range(self.nstm)]
+        self.G = None
+        self.phase = 1.0 + 0j
+        self.trace = []        # (event, ntau, G-copy or None, phase) at stabilisation points
+
+    # ---- per-slice pieces ------------------------------------------------
+    def op_phase(self, z):
+        m = self.m
+        for n in range(m.n_op):
+            for nt in range(m.ltrot):
+                ang = (m.op_g[n] * m.op_alpha[n] * phi(self.f[n, nt], m.op_type[n])).imag
+                z = z * complex(math.cos(ang), math.sin(ang))
+        return z
+
+    def flip(self, n, nt):
+        typ = self.m.op_type[n]
+        s = int(self.f[n, nt])
+        if typ == 1:
+            return -s
+        return FLIP2[s][nranf(self.rng(), 3) - 1]
+
+    def upgrade2(self, G, n, nt, hs_new):
+        """Upgrade2 for a 1x1 operator (upgrade_mod.F90:105-302), mode 'Final',
+        S0_ratio = T0_proposal_ratio = 1."""
+        m = self.m
+        P = int(m.op_P[n])
+        typ = m.op_type[n]
+        s_old = int(self.f[n, nt])
+        Z1 = m.op_g[n] * (phi(hs_new, typ) - phi(s_old, typ))
+        myexp = np.exp(Z1 * m.op_E[n])
+        Z = myexp - 1.0
+        D_mat = myexp + (-Z * G[P, P])
+        ratio = D_mat * np.exp(Z1 * m.op_alpha[n])
+        ratiotot = ratio ** float(m.n_sun) * gama(hs_new, typ) / gama(s_old, typ)
+        weight = abs((self.phase * ratiotot).real / self.phase.real)
+        toggle = weight > self.rng()
+        if toggle:
+            self.phase = self.phase * ratiotot / np.sqrt(ratiotot * np.conj(ratiotot))
+            v = -G[P, :].copy()
+            v[P] = 1.0 - G[P, P]
+            x = Z / (1.0 + v[P] * Z)
+            xp = G[:, P].copy()
+            G -= x * np.outer(xp, v)
+            self.f[n, nt] = hs_new
+        self.ctrl.upgrade(toggle)
+        return toggle
+
+    def wrapgrup(self, G, ntau):
+        """WRAPGRUP (Wrapgr_mod.F90:81-156); ntau in [0, L-1], updates slice ntau+1."""
+        m = self.m
+        nt = ntau          # 0-based column of slice ntau1 = ntau+1
+        G = m.mat @ G
+        G = G @ m.invmat
+        for n in range(m.n_op):
+            P = int(m.op_P[n])
+            sp = int(self.f[n, nt])
+            G[P, :] *= m.e_exp(n, sp)
+            G[:, P] *= m.e_exp(n, -sp)
+            hs_new = self.flip(n, nt)
+            self.rng()                       # T0_proposal = 1.5 > ranf_wrap()
+            self.upgrade2(G, n, nt, hs_new)
+        return G
+
+    def wrapgrdo(self, G, ntau):
+        """WRAPGRDO (Wrapgr_mod.F90:160-243); ntau in [L, 1], updates slice ntau."""
+        m = self.m
+        nt = ntau - 1
+        for n in range(m.n_op - 1, -1, -1):
+            hs_new = self.flip(n, nt)
+            self.rng()
+            self.upgrade2(G, n, nt, hs_new)
+            P = int(m.op_P[n])
+            sp = int(self.f[n, nt])
+            G[P, :] *= m.e_exp(n, -sp)
+            G[:, P] *= m.e_exp(n, sp)
+        G = G @ m.mat
+        G = m.invmat @ G
+        return G
+
+    def wrapur(self, ntau, ntau1, udv: UDV):
+        """WRAPUR default branch (wrapur_mod.F90:101-126)."""
+        m = self.m
+        U = udv.U.copy()
+        for nt in range(ntau + 1, ntau1 + 1):
+            U = m.mat @ U
+            for n in range(m.n_op):
+                U[int(m.op_P[n]), :] *= m.e_exp(n, int(self.f[n, nt - 1]))
+        return decompose(UDV(U, udv.D, udv.V, "R", udv.detU), self.presort)
+
+    def wrapul(self, ntau1, ntau, udv: UDV):
+        """WRAPUL default branch (wrapul_mod.F90:108-132)."""
+        m = self.m
+        U = udv.U.copy()
+        for nt in range(ntau1, ntau, -1):
+            for n in range(m.n_op - 1, -1, -1):
+                U[int(m.op_P[n]), :] *= np.conj(m.e_exp(n, int(self.f[n, nt - 1])))
+            U = m.mat @ U
+        return decompose(UDV(U, udv.D, udv.V, "L", udv.detU), self.presort)
+
+    def _stabilise(self, nvar, ntau1):
+        test = self.G
+        z1, Gf = cgr(nvar, self.udvr, self.udvl, self.presort)
+        xmax, xmean = self.ctrl.precision_g(Gf, test)
+        z1 = self.op_phase(z1)
+        z = z1 ** self.m.n_sun
+        self.ctrl.precision_p(z, self.phase)
+        self.trace.append((ntau1, nvar, Gf.copy(), test.copy(), z, self.phase, xmax, xmean))
+        self.phase = z
+        self.G = Gf
+
+    # ---- main.F90 ---------------------------------------------------------
+    def init(self):
+        """main.F90:589-631: fill the left storage, then G and Phase at tau = 0."""
+        n = self.m.ndim
+        s = self.stab
+        self.udvst[self.nstm] = UDV.identity(n, "L")
+        udvl = UDV.identity(n, "L")
+        for nst in range(self.nstm - 1, 0, -1):
+            udvl = self.wrapul(s[nst + 1], s[nst], udvl)
+            self.udvst[nst] = udvl.copy()
+        self.udvl = self.wrapul(s[1], 0, udvl)
+        self.udvr = UDV.identity(n, "R")
+        z, G = cgr(1, self.udvr, self.udvl, self.presort)
+        z = self.op_phase(z)
+        self.phase = z ** self.m.n_sun
+        self.G = G
+
+    def sweep(self):
+        """One NSW iteration of the sequential sweep (main.F90:714-887)."""
+        m = self.m
+        L = m.ltrot
+        s = self.stab
+        self.udvr = UDV.identity(m.ndim, "R")
+        nst = 1
+        for ntau in range(0, L):
+            ntau1 = ntau + 1
+            self.G = self.wrapgrup(self.G, ntau)
+            if ntau1 == s[nst]:
+                self.udvr = self.wrapur(s[nst - 1], ntau1, self.udvr)
+                self.udvl = self.udvst[nst]
+                self.udvst[nst] = self.udvr.copy()
+                nvar = 2 if ntau1 > L // 2 else 1
+                self._stabilise(nvar, ntau1)
+                nst += 1
+        self.udvl = UDV.identity(m.ndim, "L")
+        nst = self.nstm - 1
+        for ntau in range(L, 0, -1):
+            ntau1 = ntau - 1
+            self.G = self.wrapgrdo(self.G, ntau)
+            if nst >= 0 and s[nst] == ntau1 and ntau1 != 0:
+                self.udvl = self.wrapul(s[nst + 1], ntau1, self.udvl)
+                self.udvr = self.udvst[nst]
+                self.udvst[nst] = self.udvl.copy()
+                nvar = 2 if ntau1 > L // 2 else 1
+                self._stabilise(nvar, ntau1)
+                nst -= 1
+        self.udvl = self.wrapul(s[1], s[0], self.udvl)
+        self.udvr = UDV.identity(m.ndim, "R")
+        self._stabilise(1, 0)
+        self.udvst[self.nstm] = UDV.identity(m.ndim, "L")
+
+
+def model_from_ham(ham) -> OracleModel:
+    """Plain arrays from a host plugin object (alf_amd.host.hamiltonian.HamBase)."""
+    ops = ham.Op_V
+    return OracleModel(
+        ndim=ham.Ndim, ltrot=ham.Ltrot, nwrap=ham.Nwrap, n_sun=ham.N_SUN,
+        mat=np.array(ham.Op_T.mat), invmat=np.array(ham.Op_T.invmat),
+        op_P=np.array([int(o.P[0]) for o in ops]), op_E=np.array([float(o.E[0]) for o in ops]),
+        op_g=np.array([o.g for o in ops], dtype=complex), op_alpha=np.array([o.alpha for o in ops], dtype=complex),
+        op_type=np.array([o.type for o in ops]))
+
+
+def brute_force_green(model: OracleModel, fields, ntau, dps=40):
+    """G(tau) = (1 + B(tau,0) B(beta,tau))^-1 in mpmath at ``dps`` digits, with
+    B(nt) = diag(E_exp(sp)) exp(-dtau T) for slice nt (1-based).  Used to pin CGR,
+    WRAPUR/WRAPUL and decompose.  Also returns the phase of det(1 + ...)."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    n = model.ndim
+    mat = mp.matrix(n, n)
+    for i in range(n):
+        for j in range(n):
+            mat[i, j] = mp.mpc(complex(model.mat[i, j]))
+
+    def Bslice(nt):
+        d = [mp.mpc(1)] * n
+        for k in range(model.n_op):
+            P = int(model.op_P[k])
+            s = abs(int(fields[k, nt - 1]))
+            z = mp.exp(mp.mpc(complex(model.op_g[k])) * model.op_E[k] * mp.mpf(phi(s, model.op_type[k])))
+            d[P] = d[P] * (z if fields[k, nt - 1] > 0 else 1 / z)
+        out = mp.matrix(n, n)
+        for i in range(n):
+            for j in range(n):
+                out[i, j] = d[i] * mat[i, j]
+        return out
+
+    prod = mp.eye(n)
+    for nt in list(range(ntau, 0, -1)) + list(range(model.ltrot, ntau, -1)):
+        prod = prod * Bslice(nt)
+    M = mp.eye(n) + prod
+    G = M ** -1
+    det = mp.det(M)
+    Gn = np.array([[complex(G[i, j]) for j in range(n)] for i in range(n)])
+    ph = complex(det / abs(det))
+    return Gn, ph
