@@ -1,0 +1,1056 @@
+// Context, sweep orchestration and the C ABI of the gfx950 DQMC sweep engine.
+//
+// The control flow restates Prog/main.F90:589-631 (initialisation) and
+// 714-887 (sequential sweep); the per-slice work maps onto the kernels as
+//   WRAPGRUP:  GEMM exp(-dtau T) G, GEMM (.) exp(dtau T) with the slice's
+//              diagonal exponentials in the epilogue, then Metropolis rounds
+//              (chain kernel + rank-K flush GEMM)
+//   WRAPGRDO:  Metropolis rounds, then GEMM with the new diagonal exponentials
+//              and exp(-dtau T) fused, then GEMM exp(dtau T) (.)
+//   WRAPUR/UL: Nwrap GEMMs on U, then the presorted blocked QR (qr.hip)
+//   CGR:       two GEMMs for the scale-separated middle matrix, its QR,
+//              triangular inverse and three GEMMs for G
+// Everything stays on the device; the host only enqueues work.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/alfq.h"
+#include "alfq_chain.h"
+#include "alfq_qr.h"
+
+static thread_local std::string g_last_error;
+void alfq_set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+struct DevScalars {
+  cplx z_cgr;             // phase returned by CGR
+  cplx opph;              // Op_phase product
+  double xmaxg, xmeang;   // Control_PrecisionG accumulators
+  long long ncg;
+  double xmaxp;           // Control_PrecisionP
+  int status;             // 1: NaN in G, 3: XMAX > 10
+  int pad;
+  long long nlog;         // records written to the precision log
+};
+
+struct UDVSlot {
+  cplx* U = nullptr;
+  cplx* V = nullptr;
+  cplx* D = nullptr;
+  cplx* sc = nullptr;     // [0] = decompose phase, [1] = det(U)
+  int refs = 0;
+  char side = 'R';
+};
+
+constexpr int LOG_CAP = 1 << 16;
+
+__global__ void cgr_scales_kernel(const cplx* __restrict__ DR, const cplx* __restrict__ DL, int n,
+                                  cplx* __restrict__ out) {
+  // out: [0] 1/DR+  [1] 1/DL+  [2] DR-  [3] DL-   (cgr1_mod.F90:238-268 split at 1.0)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dr = DR[i].x, dl = DL[i].x;
+  out[i] = make_double2(dr <= 1.0 ? 1.0 : 1.0 / dr, 0.0);
+  out[n + i] = make_double2(dl <= 1.0 ? 1.0 : 1.0 / dl, 0.0);
+  out[2 * n + i] = make_double2(dr <= 1.0 ? dr : 1.0, 0.0);
+  out[3 * n + i] = make_double2(dl <= 1.0 ? dl : 1.0, 0.0);
+}
+
+__global__ void recip_real_kernel(const cplx* __restrict__ D, int n, cplx* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double2(1.0 / D[i].x, 0.0);
+}
+
+// Control_PrecisionG on (fresh, wrapped), then Op_phase/Z**N_SUN/Control_PrecisionP and
+// PHASE = Z (main.F90:744-753).  With check == 0 only PHASE = Z.
+__global__ __launch_bounds__(1024) void stabilise_kernel(const cplx* __restrict__ Gf, const cplx* __restrict__ Gw,
+                                                         int n, int check, int n_sun, DevScalars* sc,
+                                                         ChainState* st, double* __restrict__ log) {
+  __shared__ double smax[1024];
+  __shared__ double ssum[1024];
+  __shared__ int snan;
+  const int tid = threadIdx.x;
+  if (tid == 0) snan = 0;
+  __syncthreads();
+  double mx = 0.0, sm = 0.0;
+  int nan = 0;
+  if (check) {
+    const long long nn = (long long)n * n;
+    for (long long i = tid; i < nn; i += blockDim.x) {
+      const cplx a = Gf[i], b = Gw[i];
+      if (a.x != a.x || a.y != a.y || b.x != b.x || b.y != b.y) nan = 1;
+      const double d = hypot(a.x - b.x, a.y - b.y);
+      mx = fmax(mx, d);
+      sm += d;
+    }
+  }
+  smax[tid] = mx;
+  ssum[tid] = sm;
+  if (nan) atomicOr(&snan, 1);
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if (tid < s) {
+      smax[tid] = fmax(smax[tid], smax[tid + s]);
+      ssum[tid] += ssum[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cplx z1 = cmul(sc->z_cgr, sc->opph);
+    cplx z = z1;
+    for (int p = 1; p < n_sun; ++p) z = cmul(z, z1);
+    if (check) {
+      if (snan) {
+        sc->status = 1;
+      } else {
+        const double xmax = smax[0];
+        const double xmean = ssum[0] / ((double)n * (double)n);
+        sc->ncg += 1;
+        if (xmax > 10.0) sc->status = 3;
+        sc->xmaxg = fmax(sc->xmaxg, xmax);
+        sc->xmeang += xmean;
+        const double dp = hypot(z.x - st->phase.x, z.y - st->phase.y);
+        sc->xmaxp = fmax(sc->xmaxp, dp);
+        if (sc->nlog < LOG_CAP) {
+          double* r = log + 3 * sc->nlog;
+          r[0] = xmax;
+          r[1] = xmean;
+          r[2] = dp;
+        }
+        sc->nlog += 1;
+      }
+    }
+    st->phase = z;
+  }
+}
+
+__global__ void chain_reset_kernel(ChainState* st) {
+  st->pos = 0;
+  st->m = 0;
+}
+
+inline unsigned nblk(long long count, int per) { return (unsigned)((count + per - 1) / per); }
+
+}  // namespace
+
+struct alfq_ctx {
+  int n = 0, L = 0, W = 0, nstm = 0, n_op = 0, n_sun = 0;
+  bool t_real = true;
+  int K = 32;
+  hipStream_t s = nullptr;
+  std::vector<int> stab;                 // Stab_nt(0:NSTM)
+  // model
+  void* expT = nullptr;
+  void* expTinv = nullptr;
+  int* op_site = nullptr;
+  int* op_type = nullptr;
+  int* site_ptr = nullptr;
+  int* site_ops = nullptr;
+  cplx* eexp = nullptr;
+  cplx* myexp = nullptr;
+  cplx* expalpha = nullptr;
+  cplx* phasefac = nullptr;
+  double* gama_ratio = nullptr;
+  std::vector<int> h_op_type;
+  // state
+  int8_t* f = nullptr;
+  cplx* G = nullptr;
+  cplx* Gtmp = nullptr;
+  cplx* H = nullptr;
+  cplx* X = nullptr;
+  cplx* Y = nullptr;
+  ChainState* st = nullptr;
+  DevScalars* sc = nullptr;
+  double* log = nullptr;
+  // UDV pool
+  std::vector<UDVSlot> pool;
+  int var_r = -1, var_l = -1;
+  std::vector<int> var_st;               // udvst(1..NSTM) -> slot
+  // scratch
+  cplx* dv_d = nullptr;
+  cplx* dv_dinv = nullptr;
+  cplx* dv_dconj = nullptr;
+  cplx* U0 = nullptr;
+  cplx* U1 = nullptr;
+  cplx* TP = nullptr;
+  cplx* Qc = nullptr;
+  cplx* DUP = nullptr;
+  cplx* DUPinv = nullptr;
+  cplx* Xc = nullptr;
+  cplx* Yc = nullptr;
+  cplx* cscal = nullptr;
+  QRWork qr;
+  // random numbers
+  int rng_mode = 0;                      // 0: none, 1: tape, 2: philox
+  double* u = nullptr;
+  long long u_cap = 0;
+  long long u_len = 0;
+  unsigned long long u_base = 0;
+  unsigned long long rng_next = 0;       // host mirror of st->rng_pos
+  unsigned long long seed = 0;
+  unsigned stream_id = 0;
+  long long per_visit_sum = 0;           // uniforms per pass over all ops
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int slot_alloc(alfq_ctx* c) {
+  for (size_t i = 0; i < c->pool.size(); ++i)
+    if (c->pool[i].refs == 0) { c->pool[i].refs = 1; return (int)i; }
+  UDVSlot sl;
+  const size_t nn = (size_t)c->n * c->n;
+  if (hipMalloc(&sl.U, nn * sizeof(cplx)) != hipSuccess || hipMalloc(&sl.V, nn * sizeof(cplx)) != hipSuccess ||
+      hipMalloc(&sl.D, c->n * sizeof(cplx)) != hipSuccess || hipMalloc(&sl.sc, 2 * sizeof(cplx)) != hipSuccess) {
+    alfq_set_error("slot_alloc: hipMalloc failed");
+    return -1;
+  }
+  sl.refs = 1;
+  c->pool.push_back(sl);
+  return (int)c->pool.size() - 1;
+}
+
+void slot_release(alfq_ctx* c, int id) {
+  if (id >= 0 && c->pool[id].refs > 0) c->pool[id].refs -= 1;
+}
+
+int* var_ref(alfq_ctx* c, int var) {
+  if (var == ALFQ_UDVR) return &c->var_r;
+  if (var == ALFQ_UDVL) return &c->var_l;
+  if (var >= 1 && var <= c->nstm) return &c->var_st[var];
+  return nullptr;
+}
+
+void var_bind(alfq_ctx* c, int* v, int slot) {
+  c->pool[slot].refs += 1;
+  slot_release(c, *v);
+  *v = slot;
+}
+
+__global__ void udv_identity_kernel(cplx* U, cplx* V, cplx* D, cplx* sc, int n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < (long long)n * n) {
+    const int r = (int)(idx % n), cc = (int)(idx / n);
+    const cplx v = make_double2(r == cc ? 1.0 : 0.0, 0.0);
+    U[idx] = v;
+    V[idx] = v;
+  }
+  if (idx < n) D[idx] = make_double2(1.0, 0.0);
+  if (idx < 2) sc[idx] = make_double2(1.0, 0.0);
+}
+
+// reset: bind var to a fresh identity slot (U = V = 1, D = 1, det U = 1)
+int udv_reset_dev(alfq_ctx* c, int* v, char side) {
+  const int id = slot_alloc(c);
+  if (id < 0) return ST_GENERIC;
+  UDVSlot& sl = c->pool[id];
+  sl.side = side;
+  hipLaunchKernelGGL(udv_identity_kernel, dim3(nblk((long long)c->n * c->n, 256)), dim3(256), 0, c->s, sl.U, sl.V,
+                     sl.D, sl.sc, c->n);
+  HIP_OK(hipGetLastError());
+  var_bind(c, v, id);
+  slot_release(c, id);
+  return ST_NONE;
+}
+
+const cplx ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
+
+int fill_uniforms(alfq_ctx* c, long long count) {
+  // make uniforms [rng_next, rng_next + count) available on the device
+  if (c->rng_mode == 1) return ST_NONE;   // the tape is resident
+  if (c->rng_mode != 2) { alfq_set_error("no random stream: call alfq_rng_set_tape or alfq_rng_set_philox"); return ST_GENERIC; }
+  if (count > c->u_cap) {
+    if (c->u) hipFree(c->u);
+    c->u = nullptr;
+    HIP_OK(hipMalloc(&c->u, count * sizeof(double)));
+    c->u_cap = count;
+  }
+  c->u_base = c->rng_next;
+  c->u_len = count;
+  return launch_philox_fill(c->u, c->u_base, count, c->seed, c->stream_id, c->s);
+}
+
+int metropolis_slice(alfq_ctx* c, int slice, int dir) {
+  int st;
+  ChainArgs a{};
+  a.G = c->G; a.n = c->n; a.X = c->X; a.Y = c->Y; a.Kmax = c->K;
+  a.f = c->f; a.n_op = c->n_op; a.slice = slice; a.dir = dir;
+  a.op_site = c->op_site; a.op_type = c->op_type;
+  a.op_myexp = c->myexp; a.op_expalpha = c->expalpha; a.gama_ratio = c->gama_ratio;
+  a.n_sun = c->n_sun;
+  a.u = c->u; a.u_base = c->u_base; a.u_len = c->u_len; a.st = c->st;
+  hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(1), 0, c->s, c->st);
+  HIP_OK(hipGetLastError());
+  const int rounds = c->n_op / c->K + 1;
+  for (int r = 0; r < rounds; ++r) {
+    if ((st = launch_chain(a, c->s))) return st;
+    // G += X Y^T with the rank read from the chain state
+    GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
+    g.kdev = &c->st->m;
+    if ((st = gemm_launch(g, c->s))) return st;
+  }
+  c->rng_next += c->per_visit_sum;
+  return ST_NONE;
+}
+
+int wrapgrup_dev(alfq_ctx* c, int ntau) {
+  int st;
+  const int slice = ntau;   // fields column of time slice NTAU1 = NTAU + 1
+  if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
+                        c->dv_dconj, c->s)))
+    return st;
+  // H = exp(-dtau T) G
+  GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
+                          ONE, ZERO);
+  if ((st = gemm_launch(g1, c->s))) return st;
+  // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
+  GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
+                          c->n, ONE, ZERO);
+  g2.rs = c->dv_d;
+  g2.cs = c->dv_dinv;
+  if ((st = gemm_launch(g2, c->s))) return st;
+  std::swap(c->G, c->Gtmp);
+  if ((st = fill_uniforms(c, c->per_visit_sum))) return st;
+  return metropolis_slice(c, slice, +1);
+}
+
+int wrapgrdo_dev(alfq_ctx* c, int ntau) {
+  int st;
+  const int slice = ntau - 1;
+  if ((st = fill_uniforms(c, c->per_visit_sum))) return st;
+  if ((st = metropolis_slice(c, slice, -1))) return st;
+  if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
+                        c->dv_dconj, c->s)))
+    return st;
+  // H = D^-1 G D exp(-dtau T)   (Op_Wrapdo type 1 with the new fields, then Hop_mod_mmthl)
+  GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->G, c->n, OP_N, true, c->expT, c->n, OP_N, !c->t_real, c->H, c->n,
+                          ONE, ZERO);
+  g1.rs = c->dv_dinv;
+  g1.bs = c->dv_d;
+  if ((st = gemm_launch(g1, c->s))) return st;
+  // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
+  GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
+                          c->n, ONE, ZERO);
+  if ((st = gemm_launch(g2, c->s))) return st;
+  std::swap(c->G, c->Gtmp);
+  return ST_NONE;
+}
+
+// B-products of slices and the decomposition into a new slot bound to *var.
+int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
+  int st;
+  const int n = c->n;
+  const int ns = nt_hi - nt_lo;          // slices nt_lo+1 .. nt_hi (1-based)
+  if (ns < 0) { alfq_set_error("wrap: NTAU1 < NTAU"); return ST_GENERIC; }
+  if (ns > c->W) { alfq_set_error("wrap: more than Nwrap slices"); return ST_GENERIC; }
+  UDVSlot src = c->pool[*var];
+  const cplx* cur = src.U;
+  if (ns > 0) {
+    if ((st = launch_dvec(c->f, c->n_op, nt_lo, ns, n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
+                          c->dv_dconj, c->s)))
+      return st;
+  }
+  cplx* bufs[2] = {c->U0, c->U1};
+  int which = 0;
+  for (int q = 0; q < ns; ++q) {
+    // right chain: slices ascending, U <- D exp(-dtau T) U (Hop_mod_mmthR, Op_mmultR 'n')
+    // left chain:  slices descending, U <- exp(-dtau T) conj(D) U (Op_mmultR 'c', Hop_mod_mmthlc)
+    const int sl = left ? (ns - 1 - q) : q;
+    GemmArgs g = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, cur, n, OP_N, true, bufs[which], n, ONE, ZERO);
+    if (left) g.bs = c->dv_dconj + (long long)sl * n;
+    else g.rs = c->dv_d + (long long)sl * n;
+    if ((st = gemm_launch(g, c->s))) return st;
+    cur = bufs[which];
+    which ^= 1;
+  }
+  const int id = slot_alloc(c);
+  if (id < 0) return ST_GENERIC;
+  UDVSlot& out = c->pool[id];
+  out.side = left ? 'L' : 'R';
+  if ((st = udv_decompose_dev(c->qr, cur, src.D, src.V, left ? 1 : 0, out.U, out.D, out.V, out.sc, out.sc + 1,
+                              c->s)))
+    return st;
+  var_bind(c, var, id);
+  slot_release(c, id);
+  return ST_NONE;
+}
+
+int cgr_dev(alfq_ctx* c, int nvar, int slot_r, int slot_l, int check) {
+  int st;
+  const int n = c->n;
+  const UDVSlot& R = c->pool[slot_r];
+  const UDVSlot& Lf = c->pool[slot_l];
+  cplx* rbinv = c->cscal;
+  cplx* lbinv = c->cscal + n;
+  cplx* rsm = c->cscal + 2 * n;
+  cplx* lsm = c->cscal + 3 * n;
+  hipLaunchKernelGGL(cgr_scales_kernel, dim3(nblk(n, 256)), dim3(256), 0, c->s, R.D, Lf.D, n, c->cscal);
+  HIP_OK(hipGetLastError());
+  // TP = DR+^-1 (UR^H UL) DL+^-1
+  GemmArgs g1 = gemm_args(n, n, n, R.U, n, OP_C, true, Lf.U, n, OP_N, true, c->TP, n, ONE, ZERO);
+  g1.rs = rbinv;
+  g1.cs = lbinv;
+  if ((st = gemm_launch(g1, c->s))) return st;
+  // TP += DR- (VR VL) DL-
+  GemmArgs g2 = gemm_args(n, n, n, R.V, n, OP_N, true, Lf.V, n, OP_N, true, c->TP, n, ONE, ONE);
+  g2.rs = rsm;
+  g2.cs = lsm;
+  if ((st = gemm_launch(g2, c->s))) return st;
+  if ((st = qr_factor_cgr(c->qr, c->TP, nvar, R.sc + 1, Lf.sc + 1, c->Qc, c->DUP, &c->sc->z_cgr, c->s))) return st;
+  hipLaunchKernelGGL(recip_real_kernel, dim3(nblk(n, 256)), dim3(256), 0, c->s, c->DUP, n, c->DUPinv);
+  HIP_OK(hipGetLastError());
+  if ((st = tri_inverse_dev(c->qr, c->qr.Rt, c->qr.Rinv, c->s))) return st;
+  if (nvar == 1) {
+    // Xc = DUP^-1 Q^H DR+^-1 UR^H
+    GemmArgs a = gemm_args(n, n, n, c->Qc, n, OP_C, true, R.U, n, OP_C, true, c->Xc, n, ONE, ZERO);
+    a.bs = rbinv;
+    a.rs = c->DUPinv;
+    if ((st = gemm_launch(a, c->s))) return st;
+    // Yc = Rt^-1 Xc
+    GemmArgs b = gemm_args(n, n, n, c->qr.Rinv, n, OP_N, true, c->Xc, n, OP_N, true, c->Yc, n, ONE, ZERO);
+    if ((st = gemm_launch(b, c->s))) return st;
+    // G = UL DL+^-1 P Yc
+    GemmArgs d = gemm_args(n, n, n, Lf.U, n, OP_N, true, c->Yc, n, OP_N, true, c->Gtmp, n, ONE, ZERO);
+    d.bmap = c->qr.pinv;
+    d.bs = lbinv;
+    if ((st = gemm_launch(d, c->s))) return st;
+  } else {
+    // Xc = UL DL+^-1 Q DUP^-1
+    GemmArgs a = gemm_args(n, n, n, Lf.U, n, OP_N, true, c->Qc, n, OP_N, true, c->Xc, n, ONE, ZERO);
+    a.bs = lbinv;
+    a.cs = c->DUPinv;
+    if ((st = gemm_launch(a, c->s))) return st;
+    // Yc = Xc Rt^-H
+    GemmArgs b = gemm_args(n, n, n, c->Xc, n, OP_N, true, c->qr.Rinv, n, OP_C, true, c->Yc, n, ONE, ZERO);
+    if ((st = gemm_launch(b, c->s))) return st;
+    // G = Yc P^T DR+^-1 UR^H
+    GemmArgs d = gemm_args(n, n, n, c->Yc, n, OP_N, true, R.U, n, OP_C, true, c->Gtmp, n, ONE, ZERO);
+    d.amap = c->qr.pinv;
+    d.bs = rbinv;
+    if ((st = gemm_launch(d, c->s))) return st;
+  }
+  if ((st = launch_opphase(c->f, (long long)c->n_op * c->L, c->n_op, c->phasefac, &c->sc->opph, c->s))) return st;
+  hipLaunchKernelGGL(stabilise_kernel, dim3(1), dim3(1024), 0, c->s, c->Gtmp, c->G, n, check, c->n_sun, c->sc,
+                     c->st, c->log);
+  HIP_OK(hipGetLastError());
+  std::swap(c->G, c->Gtmp);
+  return ST_NONE;
+}
+
+int init_dev(alfq_ctx* c) {
+  int st;
+  const auto& s = c->stab;
+  if ((st = udv_reset_dev(c, &c->var_st[c->nstm], 'L'))) return st;
+  if ((st = udv_reset_dev(c, &c->var_l, 'L'))) return st;
+  if ((st = udv_reset_dev(c, &c->var_r, 'R'))) return st;
+  for (int nst = c->nstm - 1; nst >= 1; --nst) {
+    if ((st = wrap_udv_dev(c, &c->var_l, s[nst], s[nst + 1], true))) return st;
+    var_bind(c, &c->var_st[nst], c->var_l);
+  }
+  if ((st = wrap_udv_dev(c, &c->var_l, 0, s[1], true))) return st;
+  return cgr_dev(c, 1, c->var_r, c->var_l, 0);
+}
+
+int sweep_dev(alfq_ctx* c) {
+  int st;
+  const auto& s = c->stab;
+  const int L = c->L;
+  if ((st = udv_reset_dev(c, &c->var_r, 'R'))) return st;
+  int nst = 1;
+  for (int ntau = 0; ntau < L; ++ntau) {
+    const int ntau1 = ntau + 1;
+    if ((st = wrapgrup_dev(c, ntau))) return st;
+    if (ntau1 == s[nst]) {
+      if ((st = wrap_udv_dev(c, &c->var_r, s[nst - 1], ntau1, false))) return st;
+      var_bind(c, &c->var_l, c->var_st[nst]);
+      var_bind(c, &c->var_st[nst], c->var_r);
+      const int nvar = (ntau1 > L / 2) ? 2 : 1;
+      if ((st = cgr_dev(c, nvar, c->var_r, c->var_l, 1))) return st;
+      ++nst;
+    }
+  }
+  if ((st = udv_reset_dev(c, &c->var_l, 'L'))) return st;
+  nst = c->nstm - 1;
+  for (int ntau = L; ntau >= 1; --ntau) {
+    const int ntau1 = ntau - 1;
+    if ((st = wrapgrdo_dev(c, ntau))) return st;
+    if (nst >= 0 && s[nst] == ntau1 && ntau1 != 0) {
+      if ((st = wrap_udv_dev(c, &c->var_l, ntau1, s[nst + 1], true))) return st;
+      var_bind(c, &c->var_r, c->var_st[nst]);
+      var_bind(c, &c->var_st[nst], c->var_l);
+      const int nvar = (ntau1 > L / 2) ? 2 : 1;
+      if ((st = cgr_dev(c, nvar, c->var_r, c->var_l, 1))) return st;
+      --nst;
+    }
+  }
+  if ((st = wrap_udv_dev(c, &c->var_l, s[0], s[1], true))) return st;
+  if ((st = udv_reset_dev(c, &c->var_r, 'R'))) return st;
+  if ((st = cgr_dev(c, 1, c->var_r, c->var_l, 1))) return st;
+  if ((st = udv_reset_dev(c, &c->var_st[c->nstm], 'L'))) return st;
+  return ST_NONE;
+}
+
+int check_status(alfq_ctx* c) {
+  HIP_OK(hipStreamSynchronize(c->s));
+  ChainState cs;
+  DevScalars ds;
+  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  if (cs.status == 1) { alfq_set_error("random stream exhausted"); return ST_GENERIC; }
+  if (ds.status == 1) { alfq_set_error("Green function contains NaN, calculation is being aborted!"); return ST_GENERIC; }
+  if (ds.status == 3) {
+    alfq_set_error("XMAX is exceeding the threshold of 10 for G difference! Try with smaller Nwrap or dtau.");
+    return ST_UNSTABLE_MATRIX;
+  }
+  return ST_NONE;
+}
+
+double phi_st(int s, int typ) {
+  if (typ == 1) return (double)s;
+  const double s6 = std::sqrt(6.0);
+  const double a = std::sqrt(2.0 * (3.0 - s6)), b = std::sqrt(2.0 * (3.0 + s6));
+  switch (s) {
+    case -2: return -b;
+    case -1: return -a;
+    case 1: return a;
+    default: return b;
+  }
+}
+double gama_st(int s, int typ) {
+  if (typ == 1) return 1.0;
+  const double s6 = std::sqrt(6.0);
+  return (s == 1 || s == -1) ? 1.0 + s6 / 3.0 : 1.0 - s6 / 3.0;
+}
+
+}  // namespace
+
+// ============================================================================= C ABI
+
+extern "C" {
+
+const char* alfq_last_error(void) { return g_last_error.c_str(); }
+const char* alfq_version(void) { return "alfq 0.1 gfx950 fp64"; }
+
+int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
+  if (!d || !out) { alfq_set_error("null argument"); return ST_GENERIC; }
+  *out = nullptr;
+  if (d->n_fl != 1) { alfq_set_error("N_FL must be 1"); return ST_HAMILTONIAN; }
+  if (d->ndim < 1 || d->ltrot < 1 || d->nwrap < 1 || d->n_op < 0 || d->n_sun < 1) {
+    alfq_set_error("invalid model sizes");
+    return ST_HAMILTONIAN;
+  }
+  for (int i = 0; i < d->n_op; ++i) {
+    if (d->op_P[i] < 1 || d->op_P[i] > d->ndim) { alfq_set_error("operator site out of range"); return ST_HAMILTONIAN; }
+    if (d->op_type[i] != 1 && d->op_type[i] != 2) {
+      alfq_set_error("only type-1 and type-2 (discrete) fields are supported");
+      return ST_HAMILTONIAN;
+    }
+  }
+  alfq_ctx* c = new alfq_ctx();
+  c->n = d->ndim; c->L = d->ltrot; c->W = d->nwrap; c->n_op = d->n_op; c->n_sun = d->n_sun;
+  c->t_real = d->t_is_real != 0;
+  c->nstm = (c->L % c->W == 0) ? c->L / c->W : c->L / c->W + 1;
+  c->stab.assign(c->nstm + 1, 0);
+  for (int k = 1; k < c->nstm; ++k) c->stab[k] = c->W * k;
+  c->stab[c->nstm] = c->L;
+  c->K = std::min(32, std::max(1, c->n_op));
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  auto fail = [&](const char* what) {
+    alfq_set_error(std::string("alfq_ctx_create: ") + what);
+    alfq_ctx_destroy(c);
+    return ST_GENERIC;
+  };
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+  const size_t tb = nn * (c->t_real ? sizeof(double) : sizeof(cplx));
+  if (hipMalloc(&c->expT, tb) || hipMalloc(&c->expTinv, tb)) return fail("hipMalloc");
+  if (hipMemcpy(c->expT, d->expT, tb, hipMemcpyHostToDevice) ||
+      hipMemcpy(c->expTinv, d->expT_inv, tb, hipMemcpyHostToDevice))
+    return fail("copy exp(T)");
+  // operator tables
+  const int no = c->n_op;
+  std::vector<int> site(no), typ(no), sptr(n + 1, 0), sops(no);
+  std::vector<cplx> eexp(5 * no), myexp(25 * no), expal(25 * no), phf(5 * no);
+  std::vector<double> gr(25 * no);
+  for (int i = 0; i < no; ++i) {
+    site[i] = d->op_P[i] - 1;
+    typ[i] = d->op_type[i];
+    sptr[site[i] + 1] += 1;
+    const std::complex<double> g(d->op_g[2 * i], d->op_g[2 * i + 1]);
+    const std::complex<double> al(d->op_alpha[2 * i], d->op_alpha[2 * i + 1]);
+    const double E = d->op_E[i];
+    const int T = typ[i];
+    for (int sp = -2; sp <= 2; ++sp) {
+      std::complex<double> e(1.0, 0.0);
+      if (sp != 0 && !(T == 1 && (sp == 2 || sp == -2))) {
+        // E_exp(I) = exp(g E phi(I)), E_exp(-I) = 1/E_exp(I)   (Operator_mod.F90:436-466)
+        const std::complex<double> ep = std::exp(g * E * phi_st(std::abs(sp), T));
+        e = sp > 0 ? ep : 1.0 / ep;
+      }
+      eexp[5 * i + sp + 2] = make_double2(e.real(), e.imag());
+      std::complex<double> pf(1.0, 0.0);
+      if (sp != 0 && !(T == 1 && (sp == 2 || sp == -2))) {
+        const double ang = std::imag(g * al * phi_st(sp, T));
+        pf = std::complex<double>(std::cos(ang), std::sin(ang));
+      }
+      phf[5 * i + sp + 2] = make_double2(pf.real(), pf.imag());
+    }
+    for (int so = -2; so <= 2; ++so) {
+      for (int sn = -2; sn <= 2; ++sn) {
+        const int ti = 25 * i + (so + 2) * 5 + (sn + 2);
+        std::complex<double> me(1.0, 0.0), ea(1.0, 0.0);
+        double gra = 1.0;
+        const bool valid = so != 0 && sn != 0 && !(T == 1 && (std::abs(so) == 2 || std::abs(sn) == 2));
+        if (valid) {
+          // Z1 = g (phi_new - phi_old); myexp = exp(Z1 E); exp(Z1 alpha)   (upgrade_mod.F90:160-180)
+          const std::complex<double> z1 = g * (phi_st(sn, T) - phi_st(so, T));
+          me = std::exp(z1 * E);
+          ea = std::exp(z1 * al);
+          gra = gama_st(sn, T) / gama_st(so, T);
+        }
+        myexp[ti] = make_double2(me.real(), me.imag());
+        expal[ti] = make_double2(ea.real(), ea.imag());
+        gr[ti] = gra;
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) sptr[i + 1] += sptr[i];
+  {
+    std::vector<int> fillp(sptr.begin(), sptr.end() - 1);
+    for (int i = 0; i < no; ++i) sops[fillp[site[i]]++] = i;
+  }
+  c->h_op_type = typ;
+  c->per_visit_sum = 0;
+  for (int i = 0; i < no; ++i) c->per_visit_sum += (typ[i] == 2) ? 3 : 2;
+  const int nop1 = std::max(no, 1);
+  if (hipMalloc(&c->op_site, nop1 * sizeof(int)) || hipMalloc(&c->op_type, nop1 * sizeof(int)) ||
+      hipMalloc(&c->site_ptr, (n + 1) * sizeof(int)) || hipMalloc(&c->site_ops, nop1 * sizeof(int)) ||
+      hipMalloc(&c->eexp, 5 * nop1 * sizeof(cplx)) || hipMalloc(&c->myexp, 25 * nop1 * sizeof(cplx)) ||
+      hipMalloc(&c->expalpha, 25 * nop1 * sizeof(cplx)) || hipMalloc(&c->phasefac, 5 * nop1 * sizeof(cplx)) ||
+      hipMalloc(&c->gama_ratio, 25 * nop1 * sizeof(double)))
+    return fail("hipMalloc tables");
+  if (no > 0) {
+    hipMemcpy(c->op_site, site.data(), no * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(c->op_type, typ.data(), no * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(c->site_ops, sops.data(), no * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(c->eexp, eexp.data(), 5 * no * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->myexp, myexp.data(), 25 * no * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->expalpha, expal.data(), 25 * no * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->phasefac, phf.data(), 5 * no * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->gama_ratio, gr.data(), 25 * no * sizeof(double), hipMemcpyHostToDevice);
+  }
+  hipMemcpy(c->site_ptr, sptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice);
+  // state
+  const size_t fb = (size_t)std::max(no, 1) * c->L;
+  if (hipMalloc(&c->f, fb) || hipMemset(c->f, 1, fb)) return fail("fields");
+  if (hipMalloc(&c->G, nn * sizeof(cplx)) || hipMalloc(&c->Gtmp, nn * sizeof(cplx)) ||
+      hipMalloc(&c->H, nn * sizeof(cplx)) || hipMalloc(&c->X, (size_t)n * CHAIN_KMAX * sizeof(cplx)) ||
+      hipMalloc(&c->Y, (size_t)n * CHAIN_KMAX * sizeof(cplx)))
+    return fail("G buffers");
+  if (hipMalloc(&c->st, sizeof(ChainState)) || hipMalloc(&c->sc, sizeof(DevScalars)) ||
+      hipMalloc(&c->log, 3 * LOG_CAP * sizeof(double)))
+    return fail("scalars");
+  ChainState cs{};
+  cs.phase = make_double2(1.0, 0.0);
+  DevScalars ds{};
+  ds.z_cgr = make_double2(1.0, 0.0);
+  ds.opph = make_double2(1.0, 0.0);
+  hipMemcpy(c->st, &cs, sizeof(cs), hipMemcpyHostToDevice);
+  hipMemcpy(c->sc, &ds, sizeof(ds), hipMemcpyHostToDevice);
+  hipMemset(c->G, 0, nn * sizeof(cplx));
+  if (hipMalloc(&c->dv_d, (size_t)n * c->W * sizeof(cplx)) || hipMalloc(&c->dv_dinv, (size_t)n * c->W * sizeof(cplx)) ||
+      hipMalloc(&c->dv_dconj, (size_t)n * c->W * sizeof(cplx)) || hipMalloc(&c->U0, nn * sizeof(cplx)) ||
+      hipMalloc(&c->U1, nn * sizeof(cplx)) || hipMalloc(&c->TP, nn * sizeof(cplx)) ||
+      hipMalloc(&c->Qc, nn * sizeof(cplx)) || hipMalloc(&c->DUP, n * sizeof(cplx)) ||
+      hipMalloc(&c->DUPinv, n * sizeof(cplx)) || hipMalloc(&c->Xc, nn * sizeof(cplx)) ||
+      hipMalloc(&c->Yc, nn * sizeof(cplx)) || hipMalloc(&c->cscal, 4 * (size_t)n * sizeof(cplx)))
+    return fail("scratch");
+  if (qr_work_alloc(c->qr, n)) return fail("qr workspace");
+  c->var_st.assign(c->nstm + 1, -1);
+  if (udv_reset_dev(c, &c->var_r, 'R') || udv_reset_dev(c, &c->var_l, 'L')) return fail("udv");
+  for (int k = 1; k <= c->nstm; ++k)
+    if (udv_reset_dev(c, &c->var_st[k], 'L')) return fail("udv");
+  if (hipEventCreate(&c->ev0) || hipEventCreate(&c->ev1)) return fail("events");
+  if (hipStreamSynchronize(c->s)) return fail("sync");
+  *out = c;
+  return ST_NONE;
+}
+
+int alfq_ctx_destroy(alfq_ctx* c) {
+  if (!c) return ST_NONE;
+  if (c->s) hipStreamSynchronize(c->s);
+  for (auto& sl : c->pool) { hipFree(sl.U); hipFree(sl.V); hipFree(sl.D); hipFree(sl.sc); }
+  void* ptrs[] = {c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
+                  c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->st, c->sc,
+                  c->log, c->dv_d, c->dv_dinv, c->dv_dconj, c->U0, c->U1, c->TP, c->Qc, c->DUP, c->DUPinv,
+                  c->Xc, c->Yc, c->cscal, c->u};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->qr.A) qr_work_free(c->qr);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return ST_NONE;
+}
+
+int alfq_set_delay(alfq_ctx* c, int32_t k) {
+  if (!c) return ST_GENERIC;
+  if (k < 1 || k > CHAIN_KMAX) { alfq_set_error("delay depth outside [1, 64]"); return ST_GENERIC; }
+  c->K = k;
+  return ST_NONE;
+}
+
+int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) {
+  if (!c || !f) return ST_GENERIC;
+  if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
+  std::vector<int8_t> h((size_t)n_op * ltrot);
+  for (size_t i = 0; i < h.size(); ++i) {
+    const int v = f[i];
+    const int t = c->h_op_type[i % n_op];
+    if (v == 0 || v < -2 || v > 2 || (t == 1 && (v == 2 || v == -2))) {
+      alfq_set_error("fields: value outside the field type's range");
+      return ST_FIELDS;
+    }
+    h[i] = (int8_t)v;
+  }
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy(c->f, h.data(), h.size(), hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
+  if (!c || !f) return ST_GENERIC;
+  if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
+  std::vector<int8_t> h((size_t)n_op * ltrot);
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy(h.data(), c->f, h.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h.size(); ++i) f[i] = h[i];
+  return ST_NONE;
+}
+
+int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
+  if (!c || !G || ld < c->n) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy2D(c->G, c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
+                     hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_green_get(alfq_ctx* c, double* G, int32_t ld) {
+  if (!c || !G || ld < c->n) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), c->G, c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
+                     hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
+int alfq_phase_set(alfq_ctx* c, const double* phase) {
+  if (!c || !phase) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  const cplx p = make_double2(phase[0], phase[1]);
+  HIP_OK(hipMemcpy(&c->st->phase, &p, sizeof(cplx), hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_phase_get(alfq_ctx* c, double* phase) {
+  if (!c || !phase) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  cplx p;
+  HIP_OK(hipMemcpy(&p, &c->st->phase, sizeof(cplx), hipMemcpyDeviceToHost));
+  phase[0] = p.x;
+  phase[1] = p.y;
+  return ST_NONE;
+}
+
+int alfq_udv_set(alfq_ctx* c, int32_t var, const double* U, const double* D, const double* V, char side) {
+  if (!c) return ST_GENERIC;
+  int* v = var_ref(c, var);
+  if (!v) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
+  const int id = slot_alloc(c);
+  if (id < 0) return ST_GENERIC;
+  UDVSlot& sl = c->pool[id];
+  sl.side = (side == 'L' || side == 'l') ? 'L' : 'R';
+  const size_t nn = (size_t)c->n * c->n;
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy(sl.U, U, nn * sizeof(cplx), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sl.V, V, nn * sizeof(cplx), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sl.D, D, c->n * sizeof(cplx), hipMemcpyHostToDevice));
+  // det(U) of a caller-supplied U: LU on the host (the device keeps it updated afterwards)
+  {
+    std::vector<std::complex<double>> a(nn);
+    std::memcpy(a.data(), U, nn * sizeof(cplx));
+    const int n = c->n;
+    std::complex<double> det(1.0, 0.0);
+    for (int k = 0; k < n; ++k) {
+      int p = k;
+      for (int i = k + 1; i < n; ++i)
+        if (std::abs(a[i + (size_t)k * n]) > std::abs(a[p + (size_t)k * n])) p = i;
+      if (p != k) {
+        for (int j = 0; j < n; ++j) std::swap(a[k + (size_t)j * n], a[p + (size_t)j * n]);
+        det = -det;
+      }
+      const std::complex<double> piv = a[k + (size_t)k * n];
+      det *= piv;
+      if (std::abs(piv) == 0.0) break;
+      for (int i = k + 1; i < n; ++i) {
+        const std::complex<double> l = a[i + (size_t)k * n] / piv;
+        for (int j = k + 1; j < n; ++j) a[i + (size_t)j * n] -= l * a[k + (size_t)j * n];
+      }
+    }
+    const cplx sc[2] = {make_double2(1.0, 0.0), make_double2(det.real(), det.imag())};
+    HIP_OK(hipMemcpy(sl.sc, sc, sizeof(sc), hipMemcpyHostToDevice));
+  }
+  var_bind(c, v, id);
+  slot_release(c, id);
+  return ST_NONE;
+}
+
+int alfq_udv_get(alfq_ctx* c, int32_t var, double* U, double* D, double* V) {
+  if (!c) return ST_GENERIC;
+  int* v = var_ref(c, var);
+  if (!v) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
+  const UDVSlot& sl = c->pool[*v];
+  const size_t nn = (size_t)c->n * c->n;
+  HIP_OK(hipStreamSynchronize(c->s));
+  if (U) HIP_OK(hipMemcpy(U, sl.U, nn * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (V) HIP_OK(hipMemcpy(V, sl.V, nn * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (D) HIP_OK(hipMemcpy(D, sl.D, c->n * sizeof(cplx), hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
+int alfq_udv_reset(alfq_ctx* c, int32_t var, char side) {
+  if (!c) return ST_GENERIC;
+  int* v = var_ref(c, var);
+  if (!v) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
+  return udv_reset_dev(c, v, (side == 'L' || side == 'l') ? 'L' : 'R');
+}
+
+int alfq_udv_assign(alfq_ctx* c, int32_t dst, int32_t src) {
+  if (!c) return ST_GENERIC;
+  int* d = var_ref(c, dst);
+  int* s = var_ref(c, src);
+  if (!d || !s) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
+  var_bind(c, d, *s);
+  return ST_NONE;
+}
+
+int alfq_rng_set_tape(alfq_ctx* c, const double* u, int64_t n) {
+  if (!c || (!u && n > 0) || n < 0) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  if (c->u) hipFree(c->u);
+  c->u = nullptr;
+  HIP_OK(hipMalloc(&c->u, std::max<int64_t>(n, 1) * sizeof(double)));
+  if (n > 0) HIP_OK(hipMemcpy(c->u, u, n * sizeof(double), hipMemcpyHostToDevice));
+  c->u_cap = n;
+  c->u_len = n;
+  c->u_base = 0;
+  c->rng_mode = 1;
+  c->rng_next = 0;
+  const unsigned long long zero = 0;
+  HIP_OK(hipMemcpy(&c->st->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
+  if (!c) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  c->rng_mode = 2;
+  c->seed = seed;
+  c->stream_id = stream;
+  c->rng_next = 0;
+  c->u_len = 0;
+  c->u_base = 0;
+  const unsigned long long zero = 0;
+  HIP_OK(hipMemcpy(&c->st->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_rng_position(alfq_ctx* c, int64_t* pos) {
+  if (!c || !pos) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  unsigned long long p;
+  HIP_OK(hipMemcpy(&p, &c->st->rng_pos, sizeof(p), hipMemcpyDeviceToHost));
+  *pos = (int64_t)p;
+  return ST_NONE;
+}
+
+static int phase_in(alfq_ctx* c, const double* phase) {
+  if (!phase) return ST_NONE;
+  return alfq_phase_set(c, phase);
+}
+static int phase_out(alfq_ctx* c, double* phase) {
+  if (!phase) return ST_NONE;
+  return alfq_phase_get(c, phase);
+}
+
+int alfq_wrapgrup(alfq_ctx* c, int32_t ntau, double* phase) {
+  if (!c) return ST_GENERIC;
+  if (ntau < 0 || ntau >= c->L) { alfq_set_error("WRAPGRUP: NTAU outside [0, Ltrot-1]"); return ST_GENERIC; }
+  int st;
+  if ((st = phase_in(c, phase))) return st;
+  if ((st = wrapgrup_dev(c, ntau))) return st;
+  if ((st = check_status(c))) return st;
+  return phase_out(c, phase);
+}
+
+int alfq_wrapgrdo(alfq_ctx* c, int32_t ntau, double* phase) {
+  if (!c) return ST_GENERIC;
+  if (ntau < 1 || ntau > c->L) { alfq_set_error("WRAPGRDO: NTAU outside [1, Ltrot]"); return ST_GENERIC; }
+  int st;
+  if ((st = phase_in(c, phase))) return st;
+  if ((st = wrapgrdo_dev(c, ntau))) return st;
+  if ((st = check_status(c))) return st;
+  return phase_out(c, phase);
+}
+
+int alfq_wrapur(alfq_ctx* c, int32_t ntau, int32_t ntau1) {
+  if (!c) return ST_GENERIC;
+  if (ntau < 0 || ntau1 > c->L) { alfq_set_error("WRAPUR: slice range"); return ST_GENERIC; }
+  int st;
+  if ((st = wrap_udv_dev(c, &c->var_r, ntau, ntau1, false))) return st;
+  return check_status(c);
+}
+
+int alfq_wrapul(alfq_ctx* c, int32_t ntau1, int32_t ntau) {
+  if (!c) return ST_GENERIC;
+  if (ntau < 0 || ntau1 > c->L) { alfq_set_error("WRAPUL: slice range"); return ST_GENERIC; }
+  int st;
+  if ((st = wrap_udv_dev(c, &c->var_l, ntau, ntau1, true))) return st;
+  return check_status(c);
+}
+
+int alfq_cgr(alfq_ctx* c, int32_t nvar, int32_t var_r, int32_t var_l, int32_t check, double* z) {
+  if (!c) return ST_GENERIC;
+  if (nvar != 1 && nvar != 2) { alfq_set_error("CGR: NVAR must be 1 or 2"); return ST_GENERIC; }
+  int* r = var_ref(c, var_r);
+  int* l = var_ref(c, var_l);
+  if (!r || !l) { alfq_set_error("CGR: no such UDV variable"); return ST_GENERIC; }
+  int st;
+  if ((st = cgr_dev(c, nvar, *r, *l, check))) return st;
+  if ((st = check_status(c))) return st;
+  if (z) {
+    cplx zz;
+    HIP_OK(hipMemcpy(&zz, &c->sc->z_cgr, sizeof(cplx), hipMemcpyDeviceToHost));
+    z[0] = zz.x;
+    z[1] = zz.y;
+  }
+  return ST_NONE;
+}
+
+int alfq_init(alfq_ctx* c) {
+  if (!c) return ST_GENERIC;
+  int st;
+  if ((st = init_dev(c))) return st;
+  return check_status(c);
+}
+
+int alfq_sweep(alfq_ctx* c, int32_t nsweep) {
+  if (!c) return ST_GENERIC;
+  int st;
+  for (int i = 0; i < nsweep; ++i)
+    if ((st = sweep_dev(c))) return st;
+  return check_status(c);
+}
+
+int alfq_sweep_timed(alfq_ctx* c, int32_t nsweep, double* ms) {
+  if (!c || !ms) return ST_GENERIC;
+  int st;
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipEventRecord(c->ev0, c->s));
+  for (int i = 0; i < nsweep; ++i)
+    if ((st = sweep_dev(c))) return st;
+  HIP_OK(hipEventRecord(c->ev1, c->s));
+  HIP_OK(hipEventSynchronize(c->ev1));
+  float t = 0.f;
+  HIP_OK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t;
+  return check_status(c);
+}
+
+int alfq_sync(alfq_ctx* c) {
+  if (!c) return ST_GENERIC;
+  return check_status(c);
+}
+
+int alfq_control_get(alfq_ctx* c, alfq_control* out) {
+  if (!c || !out) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  ChainState cs;
+  DevScalars ds;
+  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  out->nc_up = cs.nc;
+  out->acc_up = cs.acc;
+  out->ncg = ds.ncg;
+  out->xmean_g_sum = ds.xmeang;
+  out->xmax_g = ds.xmaxg;
+  out->xmax_p = ds.xmaxp;
+  return ST_NONE;
+}
+
+int alfq_control_reset(alfq_ctx* c) {
+  if (!c) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  ChainState cs;
+  DevScalars ds;
+  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  cs.nc = cs.acc = 0;
+  ds.ncg = 0;
+  ds.xmeang = ds.xmaxg = ds.xmaxp = 0.0;
+  ds.nlog = 0;
+  ds.status = 0;
+  HIP_OK(hipMemcpy(c->st, &cs, sizeof(cs), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->sc, &ds, sizeof(ds), hipMemcpyHostToDevice));
+  return ST_NONE;
+}
+
+int alfq_precision_log(alfq_ctx* c, double* out3, int64_t cap, int64_t* count) {
+  if (!c || !count) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  DevScalars ds;
+  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  const int64_t avail = std::min<int64_t>(ds.nlog, LOG_CAP);
+  *count = ds.nlog;
+  const int64_t k = std::min<int64_t>(avail, cap);
+  if (out3 && k > 0) HIP_OK(hipMemcpy(out3, c->log, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
+int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
+              char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha, const double* beta,
+              double* Cm, int32_t ldc) {
+  if (M < 0 || N < 0 || K < 0 || !A || !B || !Cm || !alpha || !beta) { alfq_set_error("gemm: bad arguments"); return ST_GENERIC; }
+  auto op = [](char o) { return (o == 'T' || o == 't') ? OP_T : ((o == 'C' || o == 'c') ? OP_C : OP_N); };
+  const int oa = op(opa), ob = op(opb);
+  const long long acols = (oa == OP_N) ? K : M, bcols = (ob == OP_N) ? N : K;
+  const size_t asz = (size_t)lda * acols * (a_is_real ? sizeof(double) : sizeof(cplx));
+  const size_t bsz = (size_t)ldb * bcols * (b_is_real ? sizeof(double) : sizeof(cplx));
+  const size_t csz = (size_t)ldc * N * sizeof(cplx);
+  void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  HIP_OK(hipMalloc(&dA, asz ? asz : 16));
+  HIP_OK(hipMalloc(&dB, bsz ? bsz : 16));
+  HIP_OK(hipMalloc(&dC, csz ? csz : 16));
+  HIP_OK(hipMemcpy(dA, A, asz, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dB, B, bsz, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dC, Cm, csz, hipMemcpyHostToDevice));
+  GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
+                         make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
+  int st = gemm_launch(g, 0);
+  if (!st) {
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(Cm, dC, csz, hipMemcpyDeviceToHost));
+  }
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dC);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Metropolis chain kernel interface (chain.hip).
+#pragma once
+#include "alfq_common.h"
+
+#define CHAIN_KMAX 64   // largest delayed-update depth
+
+// Per-chain device state touched by the sequential Metropolis kernel.
+struct ChainState {
+  cplx phase;                     // Phase of the weight (upgrade_mod.F90: PHASE)
+  long long nc, acc;              // Control_upgrade counters (control_mod.F90:164-169)
+  unsigned long long rng_pos;     // global index of the next uniform
+  int pos;                        // next visit within the current slice
+  int m;                          // pending delayed updates in X/Y
+  int status;                     // 1: random stream exhausted
+  int pad;
+};
+
+struct ChainArgs {
+  const cplx* G; int n;
+  cplx* X; cplx* Y; int Kmax;
+  int8_t* f; int n_op; int slice; int dir;
+  const int* op_site; const int* op_type;
+  const cplx* op_myexp;           // n_op x 25: exp(Z1 E) for (old, new) field values
+  const cplx* op_expalpha;        // n_op x 25: exp(Z1 alpha)
+  const double* gama_ratio;       // n_op x 25: Gama(new)/Gama(old)
+  int n_sun;
+  const double* u; unsigned long long u_base; long long u_len;
+  ChainState* st;
+};
+
+int launch_chain(const ChainArgs& a, hipStream_t s);
+int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,
+                       unsigned stream, hipStream_t s);
+int launch_dvec(const int8_t* f, int n_op, int s0, int ns, int n, const int* site_ptr, const int* site_ops,
+                const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, hipStream_t s);
+int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, hipStream_t s);

@@ -1,0 +1,94 @@
+// Common types and helpers for the gfx950 DQMC sweep engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+typedef double2 cplx;   // interleaved (re, im), the layout of Fortran complex(8)
+
+__host__ __device__ inline cplx cmake(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline cplx cadd(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cplx csub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cplx cmul(cplx a, cplx b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline cplx cconj(cplx a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline cplx cscale(cplx a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ inline double cabs_(cplx a) { return hypot(a.x, a.y); }
+__host__ __device__ inline cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm keeps the intermediate products in range
+  if (fabs(b.x) >= fabs(b.y)) {
+    double r = b.y / b.x, d = b.x + r * b.y;
+    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  } else {
+    double r = b.x / b.y, d = b.y + r * b.x;
+    return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+  }
+}
+__host__ __device__ inline cplx cinv(cplx b) { return cdiv(make_double2(1.0, 0.0), b); }
+__device__ inline cplx cexp_(cplx z) {
+  double e = exp(z.x);
+  double s, c;
+  sincos(z.y, &s, &c);
+  return make_double2(e * c, e * s);
+}
+__host__ __device__ inline cplx cfma(cplx a, cplx b, cplx c) {  // a*b + c
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+
+// Status codes: the values of the reference's runtime_error_mod enum
+// (Libraries/Modules/runtime_error_mod.F90:56-68), returned by every C entry point.
+enum alfq_status_internal {
+  ST_NONE = 0,
+  ST_GENERIC = 1,
+  ST_FILE_NOT_FOUND = 2,
+  ST_UNSTABLE_MATRIX = 3,
+  ST_MISSING_OBS = 4,
+  ST_FIELDS = 5,
+  ST_HAMILTONIAN = 6,
+};
+
+void alfq_set_error(const std::string& msg);
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      alfq_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+      return ST_GENERIC;                                                          \
+    }                                                                             \
+  } while (0)
+
+// ---- GEMM (gemm.hip) ---------------------------------------------------------
+// C(M x N) = alpha * opA(A) * opB(B) + beta * C, all column-major FP64.
+// A/B are real (double) or complex (double2).  Optional per-row scale of the result
+// (rs, complex), per-column scale (cs, complex), per-row scale of opB (bs, complex),
+// row gather of opB (bmap: opB(k, j) = B'(bmap[k], j)) and column gather of opA
+// (amap: opA(i, k) = A'(i, amap[k])).  K may be read from device memory (kdev).
+enum GemmOp { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+struct GemmArgs {
+  int M, N, K;
+  const int* kdev;          // if non-null, K = *kdev (clamped to K)
+  const void* A; int lda; int opA; bool a_cplx;
+  const void* B; int ldb; int opB; bool b_cplx;
+  void* C; int ldc;
+  cplx alpha, beta;
+  const cplx* rs;           // rows of the product
+  const cplx* cs;           // columns of the product
+  const cplx* bs;           // rows of opB
+  const double* rs_real;    // real per-row scale (alternative to rs)
+  const int* amap;          // column gather of opA
+  const int* bmap;          // row gather of opB
+  int batch;                // number of independent problems (grid z)
+  long long strideA, strideB, strideC;  // element strides between problems
+};
+
+GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,
+                   const void* B, int ldb, int opB, bool b_cplx, void* C, int ldc,
+                   cplx alpha, cplx beta);
+int gemm_launch(const GemmArgs& g, hipStream_t s);

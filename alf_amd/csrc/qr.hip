@@ -1,0 +1,528 @@
+// Stabilisation kernels: norm-presorted blocked Householder QR (the UDV
+// decomposition and the QR of CGR's middle matrix), Q formation, the
+// scale/phase bookkeeping of QDRP_decompose, and a blocked upper-triangular
+// inverse for CGR's solve.
+//
+// Pivoting: columns are ordered once by descending norm and the QR itself is
+// unpivoted -- the pivoting strategy of the reference's STABLOG decompose
+// (Prog/udv_state_mod.F90:503-540) and of UDV_Wrap_Pivot
+// (Prog/UDV_WRAP_mod.F90:125-208), applied in linear scale.  The Green's
+// function and its phase do not depend on the pivot sequence.
+//
+// Blocked QR: a single-workgroup panel kernel factors nb columns (zgeqr2 /
+// zlarfg conventions: v(1) = 1, tau = (beta - alpha)/beta, H = I - tau v v^H)
+// and forms the compact-WY factor T (zlarft, forward columnwise); the trailing
+// matrix and the explicit Q are updated with the MFMA GEMM.
+#include "alfq_qr.h"
+
+namespace {
+
+constexpr int PT = 256;   // threads of the single-workgroup kernels
+constexpr int NBMAX = 32; // largest panel width
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block-wide sum of a double; all threads get the result.  red: >= PT/64 doubles of LDS.
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;
+}
+
+// ---- column norms of the matrix to be factorised -------------------------------
+// src(i, j) = S(i, j) * D(j)   or, with ctrans, conj(S(j, i)).
+__global__ void colnorm_kernel(const cplx* __restrict__ S, int n, int lds, const cplx* __restrict__ D,
+                               int ctrans, double* __restrict__ norms) {
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    cplx v = ctrans ? S[j + (long long)i * lds] : S[i + (long long)j * lds];
+    s += cabs2(v);
+  }
+  s = wave_sum(s);
+  if (lane == 0) {
+    double nr = sqrt(s);
+    if (D) nr *= cabs_(D[j]);
+    norms[j] = nr;
+  }
+}
+
+// rank sort, descending, ties broken by index: perm[rank] = j, pinv[j] = rank
+__global__ void sortperm_kernel(const double* __restrict__ norms, int n, int* __restrict__ perm,
+                                int* __restrict__ pinv) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double x = norms[j];
+    int r = 0;
+    for (int i = 0; i < n; ++i) {
+      const double y = norms[i];
+      r += (y > x) || (y == x && i < j);
+    }
+    perm[r] = j;
+    pinv[j] = r;
+  }
+}
+
+// A(:, i) = src(:, perm[i])
+__global__ void gather_kernel(const cplx* __restrict__ S, int n, int lds, const cplx* __restrict__ D,
+                              int ctrans, const int* __restrict__ perm, cplx* __restrict__ A) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int r = (int)(idx % n), i = (int)(idx / n);
+  const int j = perm[i];
+  cplx v;
+  if (ctrans) {
+    v = cconj(S[j + (long long)r * lds]);
+  } else {
+    v = S[r + (long long)j * lds];
+    if (D) v = cmul(v, D[j]);
+  }
+  A[idx] = v;
+}
+
+__global__ void fill_kernel(cplx* __restrict__ X, long long count, cplx val) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < count) X[idx] = val;
+}
+
+__global__ void identity_kernel(cplx* __restrict__ X, int n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  X[idx] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+}
+
+// ---- panel factorisation (one workgroup) ----------------------------------------
+// Factors A(k:n-1, k:k+nb-1) in place.  Writes tau(k..), the explicit V panel into
+// Vw(k:n-1, k:k+nb-1) (unit diagonal, zeros above), T (nb x nb, upper), Y1 = V T^H and
+// Y2 = V T into the same positions of Y1w/Y2w, and det(H_k...H_{k+nb-1}) into pdet[b].
+__global__ __launch_bounds__(PT) void panel_kernel(cplx* __restrict__ A, int n, int k, int nb,
+                                                   cplx* __restrict__ tau, cplx* __restrict__ Vw,
+                                                   cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
+                                                   cplx* __restrict__ Tw, cplx* __restrict__ pdet) {
+  __shared__ double red[PT / 64];
+  __shared__ cplx s_tau[NBMAX];
+  __shared__ cplx s_T[NBMAX * NBMAX];
+  __shared__ cplx s_w[NBMAX];
+  __shared__ double s_vn2[NBMAX];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int nw = blockDim.x >> 6;
+  const int ncol = (k + nb <= n) ? nb : n - k;
+
+  for (int j = 0; j < ncol; ++j) {
+    const int c = k + j;
+    cplx* col = A + (long long)c * n;
+    // reflector for col(c..n-1)
+    double xs = 0.0;
+    for (int r = c + 1 + tid; r < n; r += blockDim.x) xs += cabs2(col[r]);
+    const double xnorm2 = block_sum(xs, red);
+    const cplx alpha = col[c];
+    cplx t = make_double2(0.0, 0.0);
+    cplx scal = make_double2(1.0, 0.0);
+    double beta = alpha.x;
+    if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
+      t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      scal = cinv(make_double2(alpha.x - beta, alpha.y));
+    }
+    __syncthreads();
+    for (int r = c + 1 + tid; r < n; r += blockDim.x) col[r] = cmul(col[r], scal);
+    if (tid == 0) {
+      col[c] = make_double2(beta, 0.0);
+      s_tau[j] = t;
+      tau[c] = t;
+    }
+    __syncthreads();
+    // apply H^H = I - conj(tau) v v^H to the remaining panel columns, one wave per column
+    const cplx tc = cconj(t);
+    for (int q = c + 1 + wave; q < k + ncol; q += nw) {
+      cplx* aq = A + (long long)q * n;
+      double wr = 0.0, wi = 0.0;
+      for (int r = c + lane; r < n; r += 64) {
+        const cplx v = (r == c) ? make_double2(1.0, 0.0) : col[r];
+        const cplx a = aq[r];
+        wr += v.x * a.x + v.y * a.y;     // conj(v) * a
+        wi += v.x * a.y - v.y * a.x;
+      }
+      wr = wave_sum(wr);
+      wi = wave_sum(wi);
+      const cplx f = cmul(tc, make_double2(wr, wi));
+      for (int r = c + lane; r < n; r += 64) {
+        const cplx v = (r == c) ? make_double2(1.0, 0.0) : col[r];
+        aq[r] = csub(aq[r], cmul(v, f));
+      }
+    }
+    __syncthreads();
+  }
+
+  // explicit V panel and |v_j|^2
+  for (int j = 0; j < ncol; ++j) {
+    const int c = k + j;
+    double s = 0.0;
+    for (int r = k + tid; r < n; r += blockDim.x) {
+      cplx v;
+      if (r < c) v = make_double2(0.0, 0.0);
+      else if (r == c) v = make_double2(1.0, 0.0);
+      else v = A[r + (long long)c * n];
+      Vw[r + (long long)c * n] = v;
+      s += cabs2(v);
+    }
+    s = block_sum(s, red);
+    if (tid == 0) s_vn2[j] = s;
+  }
+  __syncthreads();
+  // T (zlarft forward columnwise): T(j,j) = tau_j, T(0:j, j) = -tau_j T(0:j,0:j) V(:,0:j)^H v_j
+  for (int idx = tid; idx < NBMAX * NBMAX; idx += blockDim.x) s_T[idx] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < ncol; ++j) {
+    const int c = k + j;
+    // s_w[i] = V(:, i)^H v_j for i < j, one wave per i
+    for (int i = wave; i < j; i += nw) {
+      const int ci = k + i;
+      double wr = 0.0, wi = 0.0;
+      for (int r = c + lane; r < n; r += 64) {   // v_j is zero above row c
+        const cplx a = Vw[r + (long long)ci * n];
+        const cplx b = Vw[r + (long long)c * n];
+        wr += a.x * b.x + a.y * b.y;
+        wi += a.x * b.y - a.y * b.x;
+      }
+      wr = wave_sum(wr);
+      wi = wave_sum(wi);
+      if (lane == 0) s_w[i] = make_double2(wr, wi);
+    }
+    __syncthreads();
+    if (tid < j) {
+      // T(tid, j) = -tau_j * sum_{l=tid..j-1} T(tid, l) w_l   (T(0:j,0:j) is upper)
+      cplx acc = make_double2(0.0, 0.0);
+      for (int l = tid; l < j; ++l) acc = cfma(s_T[tid + NBMAX * l], s_w[l], acc);
+      s_T[tid + NBMAX * j] = cmul(make_double2(-s_tau[j].x, -s_tau[j].y), acc);
+    }
+    if (tid == 0) s_T[j + NBMAX * j] = s_tau[j];
+    __syncthreads();
+  }
+  // store T, Y1 = V T^H, Y2 = V T  (rows k..n-1, columns k..k+ncol-1)
+  const int b = k / nb;
+  for (int idx = tid; idx < ncol * ncol; idx += blockDim.x) {
+    const int i = idx % ncol, j = idx / ncol;
+    Tw[(long long)b * nb * nb + i + j * nb] = s_T[i + NBMAX * j];
+  }
+  for (long long idx = tid; idx < (long long)(n - k) * ncol; idx += blockDim.x) {
+    const int r = k + (int)(idx % (n - k));
+    const int j = (int)(idx / (n - k));
+    cplx y1 = make_double2(0.0, 0.0), y2 = make_double2(0.0, 0.0);
+    for (int l = 0; l < ncol; ++l) {
+      const cplx v = Vw[r + (long long)(k + l) * n];
+      // Y1(r, j) = sum_l V(r, l) conj(T(j, l));  Y2(r, j) = sum_l V(r, l) T(l, j)
+      y1 = cfma(v, cconj(s_T[j + NBMAX * l]), y1);
+      y2 = cfma(v, s_T[l + NBMAX * j], y2);
+    }
+    Y1w[r + (long long)(k + j) * n] = y1;
+    Y2w[r + (long long)(k + j) * n] = y2;
+  }
+  if (tid == 0) {
+    cplx d = make_double2(1.0, 0.0);
+    for (int j = 0; j < ncol; ++j) {
+      const cplx h = csub(make_double2(1.0, 0.0), cscale(s_tau[j], s_vn2[j]));
+      d = cmul(d, h);
+    }
+    pdet[b] = d;
+  }
+}
+
+__device__ cplx block_cprod(cplx v, cplx* sh) {
+  // product over the block of one complex per thread; result valid in all threads
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = cmul(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  cplx r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ---- QDRP finishing --------------------------------------------------------------
+// D(i) = |R_ii|, Rt = R with row i divided by D(i) (zeros below the diagonal).
+// mode 0/1: UDV decompose, side R/L: phase = prod Rt_ii * sign(perm) (conj for L),
+//           Rt row 0 /= phase, phase_out = phase, detU_out = det(Q) * phase.
+// mode 2/3: CGR with NVAR = 1/2: z_out = conj(detRHS)/|detRHS| * sign(perm) *
+//           prod phase(Rt_ii) * prod phase(det H_j)   (conj of the last two for NVAR = 2).
+__global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ A, int n, int nblk,
+                                                       const int* __restrict__ perm,
+                                                       const cplx* __restrict__ pdet, int mode,
+                                                       const cplx* __restrict__ detUr,
+                                                       const cplx* __restrict__ detUl,
+                                                       cplx* __restrict__ Dout, cplx* __restrict__ Rt,
+                                                       cplx* __restrict__ phase_out,
+                                                       cplx* __restrict__ detU_out) {
+  __shared__ cplx sh[PT];
+  __shared__ int s_parity;
+  __shared__ cplx s_phase;
+  const int tid = threadIdx.x;
+  // product of normalised diagonal entries
+  cplx pd = make_double2(1.0, 0.0);
+  for (int i = tid; i < n; i += blockDim.x) {
+    const cplx a = A[i + (long long)i * n];
+    const double x = cabs_(a);
+    Dout[i] = make_double2(x, 0.0);
+    cplx u = cscale(a, 1.0 / x);
+    if (mode == 3) u = cconj(u);
+    pd = cmul(pd, u);
+  }
+  pd = block_cprod(pd, sh);
+  // reflector determinants
+  cplx qd = make_double2(1.0, 0.0);
+  for (int b = tid; b < nblk; b += blockDim.x) qd = cmul(qd, pdet[b]);
+  qd = block_cprod(qd, sh);
+  if (tid == 0) {
+    // permutation parity from the cycle decomposition
+    int sgn = 1;
+    // visited marks in LDS would cost n bytes; reuse sh as a bitmap for n <= 16384
+    unsigned* vis = (unsigned*)sh;
+    const int words = (n + 31) / 32;
+    for (int w = 0; w < words; ++w) vis[w] = 0u;
+    for (int i = 0; i < n; ++i) {
+      if (vis[i >> 5] & (1u << (i & 31))) continue;
+      int L = 0, nx = i;
+      while (!(vis[nx >> 5] & (1u << (nx & 31)))) {
+        vis[nx >> 5] |= 1u << (nx & 31);
+        ++L;
+        nx = perm[nx];
+      }
+      if ((L & 1) == 0) sgn = -sgn;
+    }
+    s_parity = sgn;
+    cplx ph;
+    if (mode <= 1) {
+      ph = cscale(pd, (double)sgn);
+      if (mode == 1) ph = cconj(ph);
+      *phase_out = ph;
+      *detU_out = cmul(qd, ph);
+    } else {
+      const cplx dr = *detUr, dl = *detUl;
+      cplx drhs = cmul(cconj(dr), dl);              // det(UR^H UL)
+      drhs = cscale(cconj(drhs), 1.0 / cabs_(drhs));
+      cplx qph = cscale(qd, 1.0 / cabs_(qd));
+      if (mode == 3) qph = cconj(qph);
+      ph = cmul(cmul(drhs, cscale(pd, (double)sgn)), qph);
+      *phase_out = ph;
+    }
+    s_phase = ph;
+  }
+  __syncthreads();
+  const cplx rowscale0 = (mode <= 1) ? cinv(s_phase) : make_double2(1.0, 0.0);
+  for (long long idx = tid; idx < (long long)n * n; idx += blockDim.x) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    cplx v = make_double2(0.0, 0.0);
+    if (i <= j) {
+      v = cscale(A[idx], 1.0 / cabs_(A[i + (long long)i * n]));
+      if (i == 0) v = cmul(v, rowscale0);
+    }
+    Rt[idx] = v;
+  }
+}
+
+// X(:, 0) *= s
+__global__ void scale_col0_kernel(cplx* __restrict__ X, int n, const cplx* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) X[i] = cmul(X[i], *s);
+}
+
+// Invert the nb x nb diagonal blocks of the upper-triangular Rt into Rinv (one block per WG).
+__global__ __launch_bounds__(PT) void trinv_diag_kernel(const cplx* __restrict__ Rt, int n, int nb,
+                                                        cplx* __restrict__ Rinv) {
+  __shared__ cplx s[NBMAX * NBMAX];
+  __shared__ cplx x[NBMAX * NBMAX];
+  const int b = blockIdx.x;
+  const int k = b * nb;
+  const int m = (k + nb <= n) ? nb : n - k;
+  for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+    const int i = idx % m, j = idx / m;
+    s[i + NBMAX * j] = Rt[(k + i) + (long long)(k + j) * n];
+    x[i + NBMAX * j] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  // column j of the inverse by back substitution, one thread per column
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    for (int i = j; i >= 0; --i) {
+      cplx acc = make_double2(i == j ? 1.0 : 0.0, 0.0);
+      for (int l = i + 1; l <= j; ++l) acc = csub(acc, cmul(s[i + NBMAX * l], x[l + NBMAX * j]));
+      x[i + NBMAX * j] = cdiv(acc, s[i + NBMAX * i]);
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+    const int i = idx % m, j = idx / m;
+    Rinv[(k + i) + (long long)(k + j) * n] = x[i + NBMAX * j];
+  }
+}
+
+inline unsigned nblocks(long long count, int per) { return (unsigned)((count + per - 1) / per); }
+
+}  // namespace
+
+int qr_work_alloc(QRWork& w, int n) {
+  w.n = n;
+  w.nb = n >= 512 ? 16 : 32;
+  w.nblk = (n + w.nb - 1) / w.nb;
+  const size_t nn = (size_t)n * n;
+  HIP_OK(hipMalloc(&w.A, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Vw, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Y1, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Y2, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Rt, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Rinv, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.Tmp, nn * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.T, (size_t)w.nblk * w.nb * w.nb * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.W, (size_t)w.nb * n * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.tau, n * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.pdet, w.nblk * sizeof(cplx)));
+  HIP_OK(hipMalloc(&w.norms, n * sizeof(double)));
+  HIP_OK(hipMalloc(&w.perm, n * sizeof(int)));
+  HIP_OK(hipMalloc(&w.pinv, n * sizeof(int)));
+  HIP_OK(hipMalloc(&w.scal, 8 * n * sizeof(cplx)));
+  return ST_NONE;
+}
+
+void qr_work_free(QRWork& w) {
+  hipFree(w.A); hipFree(w.Vw); hipFree(w.Y1); hipFree(w.Y2); hipFree(w.Rt); hipFree(w.Rinv);
+  hipFree(w.Tmp); hipFree(w.T); hipFree(w.W); hipFree(w.tau); hipFree(w.pdet); hipFree(w.norms);
+  hipFree(w.perm); hipFree(w.pinv); hipFree(w.scal);
+  w = QRWork{};
+}
+
+int launch_fill(cplx* X, long long count, cplx val, hipStream_t s) {
+  if (count <= 0) return ST_NONE;
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(count, 256)), dim3(256), 0, s, X, count, val);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
+
+int launch_identity(cplx* X, int n, hipStream_t s) {
+  hipLaunchKernelGGL(identity_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, X, n);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
+
+// Factor the (presorted) source into w.A = Q R; leaves V/T/Y panels in w.
+static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctrans, hipStream_t s) {
+  const int n = w.n, nb = w.nb;
+  int st;
+  hipLaunchKernelGGL(colnorm_kernel, dim3(nblocks(n, 4)), dim3(256), 0, s, S, n, lds, D, ctrans, w.norms);
+  hipLaunchKernelGGL(sortperm_kernel, dim3(1), dim3(256), 0, s, w.norms, n, w.perm, w.pinv);
+  hipLaunchKernelGGL(gather_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, S, n, lds, D,
+                     ctrans, w.perm, w.A);
+  HIP_OK(hipGetLastError());
+  const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+  for (int k = 0; k < n; k += nb) {
+    hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
+                       w.pdet);
+    HIP_OK(hipGetLastError());
+    const int kc = k + nb;
+    if (kc >= n) break;
+    const int rows = n - k, ntr = n - kc;
+    // W = V^H A(k:, kc:)
+    GemmArgs g1 = gemm_args(nb, ntr, rows, w.Vw + k + (long long)k * n, n, OP_C, true,
+                            w.A + k + (long long)kc * n, n, OP_N, true, w.W, nb, one, zero);
+    if ((st = gemm_launch(g1, s))) return st;
+    // A(k:, kc:) -= Y1 W
+    GemmArgs g2 = gemm_args(rows, ntr, nb, w.Y1 + k + (long long)k * n, n, OP_N, true, w.W, nb, OP_N, true,
+                            w.A + k + (long long)kc * n, n, mone, one);
+    if ((st = gemm_launch(g2, s))) return st;
+  }
+  return ST_NONE;
+}
+
+// Q = H_0 H_1 ... H_{n-1} formed explicitly into Q (n x n).
+static int qr_form_q(QRWork& w, cplx* Q, hipStream_t s) {
+  const int n = w.n, nb = w.nb;
+  int st;
+  if ((st = launch_identity(Q, n, s))) return st;
+  const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+  for (int b = w.nblk - 1; b >= 0; --b) {
+    const int k = b * nb;
+    const int ncol = (k + nb <= n) ? nb : n - k;
+    const int rows = n - k;
+    // W = V^H Q(k:, k:)
+    GemmArgs g1 = gemm_args(ncol, rows, rows, w.Vw + k + (long long)k * n, n, OP_C, true,
+                            Q + k + (long long)k * n, n, OP_N, true, w.W, nb, one, zero);
+    if ((st = gemm_launch(g1, s))) return st;
+    // Q(k:, k:) -= Y2 W
+    GemmArgs g2 = gemm_args(rows, rows, ncol, w.Y2 + k + (long long)k * n, n, OP_N, true, w.W, nb, OP_N, true,
+                            Q + k + (long long)k * n, n, mone, one);
+    if ((st = gemm_launch(g2, s))) return st;
+  }
+  return ST_NONE;
+}
+
+int udv_decompose_dev(QRWork& w, const cplx* Usrc, const cplx* Dsrc, const cplx* Vsrc, int side_l,
+                      cplx* Uout, cplx* Dout, cplx* Vout, cplx* phase_out, cplx* detU_out, hipStream_t s) {
+  const int n = w.n;
+  int st;
+  if ((st = qr_factor(w, Usrc, n, Dsrc, 0, s))) return st;
+  hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet, side_l ? 1 : 0,
+                     (const cplx*)nullptr, (const cplx*)nullptr, Dout, w.Rt, phase_out, detU_out);
+  HIP_OK(hipGetLastError());
+  const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  if (!side_l) {
+    // V = Rt * P^T * V   (row i of P^T V is row perm[i] of V)
+    GemmArgs g = gemm_args(n, n, n, w.Rt, n, OP_N, true, Vsrc, n, OP_N, true, Vout, n, one, zero);
+    g.bmap = w.perm;
+    if ((st = gemm_launch(g, s))) return st;
+  } else {
+    // V = V P * Rt^H    (column i of V P is column perm[i] of V)
+    GemmArgs g = gemm_args(n, n, n, Vsrc, n, OP_N, true, w.Rt, n, OP_C, true, Vout, n, one, zero);
+    g.amap = w.perm;
+    if ((st = gemm_launch(g, s))) return st;
+  }
+  if ((st = qr_form_q(w, Uout, s))) return st;
+  hipLaunchKernelGGL(scale_col0_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, Uout, n, (const cplx*)phase_out);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
+
+int qr_factor_cgr(QRWork& w, const cplx* TPUP, int nvar, const cplx* detUr, const cplx* detUl, cplx* Qout,
+                  cplx* DUP, cplx* z_out, hipStream_t s) {
+  const int n = w.n;
+  int st;
+  if ((st = qr_factor(w, TPUP, n, nullptr, nvar == 1 ? 0 : 1, s))) return st;
+  hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet,
+                     nvar == 1 ? 2 : 3, detUr, detUl, DUP, w.Rt, z_out, (cplx*)nullptr);
+  HIP_OK(hipGetLastError());
+  if ((st = qr_form_q(w, Qout, s))) return st;
+  return ST_NONE;
+}
+
+int tri_inverse_dev(QRWork& w, const cplx* Rt, cplx* Rinv, hipStream_t s) {
+  const int n = w.n, nb = w.nb;
+  int st;
+  if ((st = launch_fill(Rinv, (long long)n * n, make_double2(0.0, 0.0), s))) return st;
+  hipLaunchKernelGGL(trinv_diag_kernel, dim3(w.nblk), dim3(PT), 0, s, Rt, n, nb, Rinv);
+  HIP_OK(hipGetLastError());
+  const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+  for (int b = 1; b < w.nblk; ++b) {
+    const int k = b * nb;
+    const int m = (k + nb <= n) ? nb : n - k;
+    // T1 = Rt(0:k, k:k+m) Rinv(k:k+m, k:k+m)
+    GemmArgs g1 = gemm_args(k, m, m, Rt + (long long)k * n, n, OP_N, true, Rinv + k + (long long)k * n, n, OP_N,
+                            true, w.Tmp, n, one, zero);
+    if ((st = gemm_launch(g1, s))) return st;
+    // Rinv(0:k, k:k+m) = -Rinv(0:k, 0:k) T1
+    GemmArgs g2 = gemm_args(k, m, k, Rinv, n, OP_N, true, w.Tmp, n, OP_N, true, Rinv + (long long)k * n, n, mone,
+                            zero);
+    if ((st = gemm_launch(g2, s))) return st;
+  }
+  return ST_NONE;
+}

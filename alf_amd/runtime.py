@@ -1,0 +1,315 @@
+"""ctypes binding of the C ABI declared in include/alfq.h.
+
+This is the Python side of the drop-in boundary: the same entry points a
+Fortran host binds with ISO_C_BINDING (see alf_amd/fortran/alfq_binding.F90).
+The library is the in-tree ``alf_amd/libalfq.so`` built for gfx950; there is
+no fallback -- loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libalfq.so")
+
+ALFQ_UDVR = -1
+ALFQ_UDVL = -2
+
+ERROR_NAMES = {0: "ERROR_NONE", 1: "ERROR_GENERIC", 2: "ERROR_FILE_NOT_FOUND", 3: "ERROR_UNSTABLE_MATRIX",
+               4: "ERROR_MISSING_OBS", 5: "ERROR_FIELDS", 6: "ERROR_HAMILTONIAN"}
+
+# every symbol include/alfq.h declares
+EXPORTS = [
+    "alfq_ctx_create", "alfq_ctx_destroy", "alfq_set_delay",
+    "alfq_fields_set", "alfq_fields_get", "alfq_green_set", "alfq_green_get",
+    "alfq_phase_set", "alfq_phase_get", "alfq_udv_set", "alfq_udv_get", "alfq_udv_reset", "alfq_udv_assign",
+    "alfq_rng_set_tape", "alfq_rng_set_philox", "alfq_rng_position",
+    "alfq_wrapgrup", "alfq_wrapgrdo", "alfq_wrapur", "alfq_wrapul", "alfq_cgr",
+    "alfq_init", "alfq_sweep", "alfq_sweep_timed",
+    "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
+    "alfq_last_error", "alfq_version", "alfq_gemm",
+]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32), ("ltrot", C.c_int32), ("nwrap", C.c_int32), ("n_sun", C.c_int32),
+        ("n_fl", C.c_int32), ("n_op", C.c_int32), ("t_is_real", C.c_int32),
+        ("expT", C.POINTER(C.c_double)), ("expT_inv", C.POINTER(C.c_double)),
+        ("op_P", C.POINTER(C.c_int32)), ("op_type", C.POINTER(C.c_int32)),
+        ("op_E", C.POINTER(C.c_double)), ("op_g", C.POINTER(C.c_double)), ("op_alpha", C.POINTER(C.c_double)),
+    ]
+
+
+class Control(C.Structure):
+    _fields_ = [("nc_up", C.c_int64), ("acc_up", C.c_int64), ("ncg", C.c_int64),
+                ("xmean_g_sum", C.c_double), ("xmax_g", C.c_double), ("xmax_p", C.c_double)]
+
+
+class AlfqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libalfq.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    sig = {
+        "alfq_ctx_create": [C.POINTER(ModelDesc), C.POINTER(P)],
+        "alfq_ctx_destroy": [P],
+        "alfq_set_delay": [P, C.c_int32],
+        "alfq_fields_set": [P, ip, C.c_int32, C.c_int32],
+        "alfq_fields_get": [P, ip, C.c_int32, C.c_int32],
+        "alfq_green_set": [P, dp, C.c_int32],
+        "alfq_green_get": [P, dp, C.c_int32],
+        "alfq_phase_set": [P, dp],
+        "alfq_phase_get": [P, dp],
+        "alfq_udv_set": [P, C.c_int32, dp, dp, dp, C.c_char],
+        "alfq_udv_get": [P, C.c_int32, dp, dp, dp],
+        "alfq_udv_reset": [P, C.c_int32, C.c_char],
+        "alfq_udv_assign": [P, C.c_int32, C.c_int32],
+        "alfq_rng_set_tape": [P, dp, C.c_int64],
+        "alfq_rng_set_philox": [P, C.c_uint64, C.c_uint32],
+        "alfq_rng_position": [P, C.POINTER(C.c_int64)],
+        "alfq_wrapgrup": [P, C.c_int32, dp],
+        "alfq_wrapgrdo": [P, C.c_int32, dp],
+        "alfq_wrapur": [P, C.c_int32, C.c_int32],
+        "alfq_wrapul": [P, C.c_int32, C.c_int32],
+        "alfq_cgr": [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp],
+        "alfq_init": [P],
+        "alfq_sweep": [P, C.c_int32],
+        "alfq_sweep_timed": [P, C.c_int32, dp],
+        "alfq_sync": [P],
+        "alfq_control_get": [P, C.POINTER(Control)],
+        "alfq_control_reset": [P],
+        "alfq_precision_log": [P, dp, C.c_int64, C.POINTER(C.c_int64)],
+        "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
+                      C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.alfq_last_error.restype = C.c_char_p
+    lib.alfq_last_error.argtypes = []
+    lib.alfq_version.restype = C.c_char_p
+    lib.alfq_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def check(lib, code):
+    if code != 0:
+        raise AlfqError(code, lib.alfq_last_error().decode())
+
+
+class Engine:
+    """One Markov chain on the GPU behind the C ABI.
+
+    ``model`` is the host plugin object (alf_amd.host.hamiltonian.HamBase).
+    """
+
+    def __init__(self, model, delay: int | None = None):
+        self.lib = load_library()
+        self.model = model
+        ops = model.Op_V
+        n = model.Ndim
+        t_real = bool(model.Op_T.is_real)
+        if t_real:
+            self._expT = np.asfortranarray(np.real(model.Op_T.mat), dtype=np.float64)
+            self._expTi = np.asfortranarray(np.real(model.Op_T.invmat), dtype=np.float64)
+        else:
+            self._expT = np.asfortranarray(model.Op_T.mat, dtype=np.complex128)
+            self._expTi = np.asfortranarray(model.Op_T.invmat, dtype=np.complex128)
+        self._P = np.array([int(o.P[0]) + 1 for o in ops], dtype=np.int32)
+        self._type = np.array([o.type for o in ops], dtype=np.int32)
+        self._E = np.array([float(o.E[0]) for o in ops], dtype=np.float64)
+        self._g = np.array([o.g for o in ops], dtype=np.complex128)
+        self._alpha = np.array([o.alpha for o in ops], dtype=np.complex128)
+        if any(o.N != 1 for o in ops):
+            raise NotImplementedError("the device sweep handles 1x1 interaction operators")
+        d = ModelDesc(n, model.Ltrot, model.Nwrap, model.N_SUN, model.N_FL, len(ops), int(t_real),
+                      _dptr(self._expT), _dptr(self._expTi), _iptr(self._P), _iptr(self._type),
+                      _dptr(self._E), _dptr(self._g), _dptr(self._alpha))
+        self._desc = d
+        h = C.c_void_p()
+        check(self.lib, self.lib.alfq_ctx_create(C.byref(d), C.byref(h)))
+        self.ctx = h
+        self.n = n
+        if delay is not None:
+            self.set_delay(delay)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.alfq_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, code):
+        check(self.lib, code)
+
+    # ---- state ----
+    def set_delay(self, k):
+        self._c(self.lib.alfq_set_delay(self.ctx, int(k)))
+
+    def set_fields(self, f):
+        a = np.asfortranarray(f, dtype=np.int32)
+        self._c(self.lib.alfq_fields_set(self.ctx, _iptr(a), a.shape[0], a.shape[1]))
+
+    def get_fields(self):
+        m = self.model
+        a = np.zeros((len(m.Op_V), m.Ltrot), dtype=np.int32, order="F")
+        self._c(self.lib.alfq_fields_get(self.ctx, _iptr(a), a.shape[0], a.shape[1]))
+        return a
+
+    def set_green(self, G):
+        a = np.asfortranarray(G, dtype=np.complex128)
+        self._c(self.lib.alfq_green_set(self.ctx, _dptr(a), self.n))
+
+    def get_green(self):
+        a = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        self._c(self.lib.alfq_green_get(self.ctx, _dptr(a), self.n))
+        return a
+
+    def set_phase(self, z):
+        a = np.array([complex(z).real, complex(z).imag])
+        self._c(self.lib.alfq_phase_set(self.ctx, _dptr(a)))
+
+    def get_phase(self):
+        a = np.zeros(2)
+        self._c(self.lib.alfq_phase_get(self.ctx, _dptr(a)))
+        return complex(a[0], a[1])
+
+    def set_udv(self, var, U, D, V, side):
+        Ua = np.asfortranarray(U, dtype=np.complex128)
+        Va = np.asfortranarray(V, dtype=np.complex128)
+        Da = np.ascontiguousarray(D, dtype=np.complex128)
+        self._c(self.lib.alfq_udv_set(self.ctx, var, _dptr(Ua), _dptr(Da),
+                                      _dptr(Va), side.encode()))
+
+    def get_udv(self, var):
+        n = self.n
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        V = np.zeros((n, n), dtype=np.complex128, order="F")
+        D = np.zeros(n, dtype=np.complex128)
+        self._c(self.lib.alfq_udv_get(self.ctx, var, _dptr(U), _dptr(D),
+                                      _dptr(V)))
+        return U, D, V
+
+    def udv_reset(self, var, side):
+        self._c(self.lib.alfq_udv_reset(self.ctx, var, side.encode()))
+
+    def udv_assign(self, dst, src):
+        self._c(self.lib.alfq_udv_assign(self.ctx, dst, src))
+
+    def set_tape(self, u):
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        self._c(self.lib.alfq_rng_set_tape(self.ctx, _dptr(a), a.size))
+
+    def set_philox(self, seed, stream=0):
+        self._c(self.lib.alfq_rng_set_philox(self.ctx, C.c_uint64(seed), C.c_uint32(stream)))
+
+    def rng_position(self):
+        p = C.c_int64()
+        self._c(self.lib.alfq_rng_position(self.ctx, C.byref(p)))
+        return p.value
+
+    # ---- reference routines ----
+    def wrapgrup(self, ntau, phase=None):
+        a = None if phase is None else np.array([complex(phase).real, complex(phase).imag])
+        self._c(self.lib.alfq_wrapgrup(self.ctx, ntau, _dptr(a) if a is not None else None))
+        return None if a is None else complex(a[0], a[1])
+
+    def wrapgrdo(self, ntau, phase=None):
+        a = None if phase is None else np.array([complex(phase).real, complex(phase).imag])
+        self._c(self.lib.alfq_wrapgrdo(self.ctx, ntau, _dptr(a) if a is not None else None))
+        return None if a is None else complex(a[0], a[1])
+
+    def wrapur(self, ntau, ntau1):
+        self._c(self.lib.alfq_wrapur(self.ctx, ntau, ntau1))
+
+    def wrapul(self, ntau1, ntau):
+        self._c(self.lib.alfq_wrapul(self.ctx, ntau1, ntau))
+
+    def cgr(self, nvar, var_r=ALFQ_UDVR, var_l=ALFQ_UDVL, check_precision=False):
+        z = np.zeros(2)
+        self._c(self.lib.alfq_cgr(self.ctx, nvar, var_r, var_l, int(check_precision), _dptr(z)))
+        return complex(z[0], z[1])
+
+    # ---- coarse ----
+    def init(self):
+        self._c(self.lib.alfq_init(self.ctx))
+
+    def sweep(self, nsweep=1):
+        self._c(self.lib.alfq_sweep(self.ctx, nsweep))
+
+    def sweep_timed(self, nsweep=1):
+        ms = C.c_double()
+        self._c(self.lib.alfq_sweep_timed(self.ctx, nsweep, C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._c(self.lib.alfq_sync(self.ctx))
+
+    def control(self):
+        c = Control()
+        self._c(self.lib.alfq_control_get(self.ctx, C.byref(c)))
+        return {"NC_up": c.nc_up, "ACC_up": c.acc_up, "NCG": c.ncg, "XMEANG": c.xmean_g_sum,
+                "XMAXG": c.xmax_g, "XMAXP": c.xmax_p,
+                "precision_green_mean": (c.xmean_g_sum / c.ncg) if c.ncg else 0.0}
+
+    def control_reset(self):
+        self._c(self.lib.alfq_control_reset(self.ctx))
+
+    def precision_log(self, cap=1 << 16):
+        out = np.zeros(3 * cap)
+        cnt = C.c_int64()
+        self._c(self.lib.alfq_precision_log(self.ctx, _dptr(out), cap, C.byref(cnt)))
+        k = min(cnt.value, cap)
+        return out[:3 * k].reshape(k, 3)
+
+
+def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None):
+    """C = alpha op(A) op(B) + beta C through the engine's MFMA kernel (alfq_gemm)."""
+    lib = load_library()
+    a_real = not np.iscomplexobj(A)
+    b_real = not np.iscomplexobj(B)
+    Aa = np.asfortranarray(A, dtype=np.float64 if a_real else np.complex128)
+    Ba = np.asfortranarray(B, dtype=np.float64 if b_real else np.complex128)
+    M = Aa.shape[0] if opa == "N" else Aa.shape[1]
+    K = Aa.shape[1] if opa == "N" else Aa.shape[0]
+    N = Ba.shape[1] if opb == "N" else Ba.shape[0]
+    Cc = np.zeros((M, N), dtype=np.complex128, order="F") if Cin is None else np.array(Cin, dtype=np.complex128, order="F")
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    check(lib, lib.alfq_gemm(M, N, K, opa.encode(), int(a_real), _dptr(Aa), max(Aa.shape[0], 1), opb.encode(),
+                             int(b_real), _dptr(Ba), max(Ba.shape[0], 1), _dptr(al), _dptr(be), _dptr(Cc), max(M, 1)))
+    return Cc

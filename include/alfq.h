@@ -1,0 +1,143 @@
+/*
+ * alfq.h -- C ABI of the gfx950 finite-temperature DQMC sweep engine.
+ *
+ * Drop-in boundary for the sequential sweep of ALF (Prog/main.F90:714-887).
+ * Every entry point returns an int32 status whose values are the reference's
+ * error enum (Libraries/Modules/runtime_error_mod.F90:56-68): 0 = ERROR_NONE,
+ * 1 = ERROR_GENERIC, 3 = ERROR_UNSTABLE_MATRIX, 5 = ERROR_FIELDS,
+ * 6 = ERROR_HAMILTONIAN.  alfq_last_error() gives the message.  No C++
+ * exception crosses this boundary.
+ *
+ * Conventions: matrices are column-major; complex numbers are interleaved
+ * (re, im) doubles, i.e. the memory of Fortran COMPLEX(KIND=8); indices that
+ * the reference passes as Fortran integers keep their meaning (NTAU, NVAR,
+ * sites are 1-based where the reference's are).  One context = one Markov
+ * chain with N_FL = 1 on one HIP stream; contexts are independent and a
+ * context must be driven by one host thread at a time.
+ */
+#ifndef ALFQ_H
+#define ALFQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct alfq_ctx alfq_ctx;
+
+/* What the model plugin sets up (Hamiltonian_main_mod.F90:181-197):
+ * the dense hopping exponential of Op_T (OpTTypes_mod.F90:92-133, 203-234:
+ * exp(g T) with g = -dtau, Hermitian-symmetrised) and the 1x1 interaction
+ * operators Op_V(1:n_op, 1) (Operator_mod.F90:56-90). */
+typedef struct alfq_model_desc {
+  int32_t ndim;            /* Ndim */
+  int32_t ltrot;           /* Ltrot */
+  int32_t nwrap;           /* Nwrap */
+  int32_t n_sun;           /* N_SUN */
+  int32_t n_fl;            /* N_FL; this engine supports 1 */
+  int32_t n_op;            /* size(Op_V, 1) */
+  int32_t t_is_real;       /* 1: RealExpOpT (real symmetric), 0: CmplxExpOpT */
+  const double* expT;      /* exp(g T): ndim*ndim reals, or 2*ndim*ndim (complex) */
+  const double* expT_inv;  /* exp(-g T), same layout */
+  const int32_t* op_P;     /* Op_V(n)%P(1), 1-based site, n_op entries */
+  const int32_t* op_type;  /* Op_V(n)%type: 1 (Ising) or 2 (discrete HS) */
+  const double* op_E;      /* Op_V(n)%E(1) */
+  const double* op_g;      /* Op_V(n)%g, complex, 2*n_op doubles */
+  const double* op_alpha;  /* Op_V(n)%alpha, complex, 2*n_op doubles */
+} alfq_model_desc;
+
+/* Statistics of Control_upgrade / Control_PrecisionG / Control_PrecisionP
+ * (Prog/control_mod.F90:164-176, 207-320). */
+typedef struct alfq_control {
+  int64_t nc_up, acc_up;
+  int64_t ncg;
+  double xmean_g_sum;      /* XMEANG: sum of per-call mean deviations */
+  double xmax_g;           /* XMAXG */
+  double xmax_p;           /* XMAXP */
+} alfq_control;
+
+/* UDV variables addressed by the fine-grained calls */
+#define ALFQ_UDVR (-1)     /* udvr */
+#define ALFQ_UDVL (-2)     /* udvl */
+/* k in 1..NSTM addresses udvst(k) */
+
+/* ---- lifecycle ----------------------------------------------------------- */
+/* Replaces Ham_Set's output consumption + Hop_mod_init (Prog/Hop_mod.F90:118-139),
+ * Op_set's E_exp tables (Prog/Operator_mod.F90:436-466) and the allocations of
+ * main.F90:588-605. */
+int alfq_ctx_create(const alfq_model_desc* desc, alfq_ctx** out);
+int alfq_ctx_destroy(alfq_ctx* ctx);
+/* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */
+int alfq_set_delay(alfq_ctx* ctx, int32_t k);
+
+/* ---- state movement ------------------------------------------------------ */
+/* nsigma%f (Fields_mod.F90:79-99): n_op x ltrot integers, column-major. */
+int alfq_fields_set(alfq_ctx* ctx, const int32_t* f, int32_t n_op, int32_t ltrot);
+int alfq_fields_get(alfq_ctx* ctx, int32_t* f, int32_t n_op, int32_t ltrot);
+/* GR(:,:,1) */
+int alfq_green_set(alfq_ctx* ctx, const double* G, int32_t ld);
+int alfq_green_get(alfq_ctx* ctx, double* G, int32_t ld);
+/* PHASE */
+int alfq_phase_set(alfq_ctx* ctx, const double* phase);
+int alfq_phase_get(alfq_ctx* ctx, double* phase);
+/* UDV_State contents (udv_state_mod.F90:85-110); side is 'R' or 'L'. */
+int alfq_udv_set(alfq_ctx* ctx, int32_t var, const double* U, const double* D, const double* V, char side);
+int alfq_udv_get(alfq_ctx* ctx, int32_t var, double* U, double* D, double* V);
+/* udv%reset(side) (udv_state_mod.F90:320-345) and dst = src (assign_UDV_state). */
+int alfq_udv_reset(alfq_ctx* ctx, int32_t var, char side);
+int alfq_udv_assign(alfq_ctx* ctx, int32_t dst, int32_t src);
+
+/* ---- random numbers (replaces ranf_wrap/nranf, random_wrap_mod.F90:135-168) */
+/* A tape of uniforms consumed in the reference's order: per type-2 field visit
+ * nranf(3), the T0 proposal, the Metropolis test; type 1 skips the first. */
+int alfq_rng_set_tape(alfq_ctx* ctx, const double* u, int64_t n);
+/* Counter-based Philox4x32-10 stream (seed, stream id). */
+int alfq_rng_set_philox(alfq_ctx* ctx, uint64_t seed, uint32_t stream);
+int alfq_rng_position(alfq_ctx* ctx, int64_t* pos);
+
+/* ---- the reference routines, same semantics ------------------------------ */
+/* WRAPGRUP(GR, NTAU, PHASE, ...) (Prog/Wrapgr_mod.F90:81-156), NTAU in [0, Ltrot-1].
+ * phase: in/out complex (2 doubles); NULL keeps the device-resident phase. */
+int alfq_wrapgrup(alfq_ctx* ctx, int32_t ntau, double* phase);
+/* WRAPGRDO(GR, NTAU, PHASE, ...) (Prog/Wrapgr_mod.F90:160-243), NTAU in [Ltrot, 1]. */
+int alfq_wrapgrdo(alfq_ctx* ctx, int32_t ntau, double* phase);
+/* WRAPUR(NTAU, NTAU1, UDVR) (Prog/wrapur_mod.F90:37-128) on udvr. */
+int alfq_wrapur(alfq_ctx* ctx, int32_t ntau, int32_t ntau1);
+/* WRAPUL(NTAU1, NTAU, UDVL) (Prog/wrapul_mod.F90:36-133) on udvl. */
+int alfq_wrapul(alfq_ctx* ctx, int32_t ntau1, int32_t ntau);
+/* CGR(Z, NVAR, GR, udvr, udvl) (Prog/cgr1_mod.F90:36-449, STAB3): the fresh
+ * Green's function replaces GR; the phase Z is returned in z (2 doubles).
+ * With check != 0 the replaced GR is compared as in Control_PrecisionG and the
+ * stabilisation step of main.F90:744-753 follows: Op_phase, Z**N_SUN,
+ * Control_PrecisionP, PHASE = Z. */
+int alfq_cgr(alfq_ctx* ctx, int32_t nvar, int32_t var_r, int32_t var_l, int32_t check, double* z);
+
+/* ---- coarse entries ------------------------------------------------------- */
+/* main.F90:589-631: fill udvst by WRAPUL, G and PHASE at tau = 0. */
+int alfq_init(alfq_ctx* ctx);
+/* nsweep iterations of main.F90:714-887 (up pass, down pass, slice-0 refresh). */
+int alfq_sweep(alfq_ctx* ctx, int32_t nsweep);
+/* Like alfq_sweep, timed with HIP events on the context's stream; *ms = elapsed. */
+int alfq_sweep_timed(alfq_ctx* ctx, int32_t nsweep, double* ms);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+int alfq_sync(alfq_ctx* ctx);
+int alfq_control_get(alfq_ctx* ctx, alfq_control* out);
+int alfq_control_reset(alfq_ctx* ctx);
+/* Per-stabilisation record: (XMAX, XMEAN, |Z - PHASE|) of each checked CGR call. */
+int alfq_precision_log(alfq_ctx* ctx, double* out3, int64_t cap, int64_t* count);
+const char* alfq_last_error(void);
+/* The engine's FP64 MFMA product on host arrays (copied in and out), for
+ * checking the kernel in isolation: C = alpha op(A) op(B) + beta C with op in
+ * 'N', 'T', 'C'; A and/or B may be real (a_is_real/b_is_real), C is complex. */
+int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
+              char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
+              const double* beta, double* C, int32_t ldc);
+/* Library build identification. */
+const char* alfq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALFQ_H */
