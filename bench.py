@@ -1,0 +1,123 @@
+"""Throughput of the gfx950 DQMC sweep engine on one MI355X.
+
+Workload (the single-chain, real-size case): Hubbard model on a 16x16 square
+lattice (Ndim = 256), t = 1, U = 4, beta = 10, dtau = 0.1 (Ltrot = 100),
+Nwrap = 10, N_SUN = 2, N_FL = 1, dense exp(-dtau T), complex128 Green's
+function, discrete four-state HS field drawn uniformly from seed 0, Philox
+random stream.  One step = one sweep of ALF's sequential finite-temperature
+update (up pass, down pass, slice-0 refresh), all on the device.
+
+Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle (numpy /
+LAPACK restatement of the same sweep) timed on one host core for one sweep.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+WORKLOAD = dict(Lattice_type="Square", L1=16, L2=16, ham_T=1.0, Ham_U=4.0, Beta=10.0, Dtau=0.1, Nwrap=10)
+
+
+def algorithmic_flops_per_sweep(n, L, S, acc, complex_t=False):
+    """F = n^3 (6 L c_T + 117.3 S) + 16 acc L n^3 with c_T = 4 (real T) or 8."""
+    cT = 8 if complex_t else 4
+    return n ** 3 * (6 * L * cT + 117.3 * S) + 16 * acc * L * n ** 3
+
+
+def cpu_baseline(ham, fields, seconds_budget=30.0):
+    """Time the oracle on one core over one sweep of the same workload."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle.dqmc_oracle import OracleSweep, Tape, model_from_ham
+    m = model_from_ham(ham)
+    rng = Tape(np.random.default_rng(12345).random(4 * m.n_op * m.ltrot * 2 + 16))
+    with threadpool_limits(1):
+        s = OracleSweep(m, fields, rng, presort=False)
+        s.init()
+        t0 = time.perf_counter()
+        s.sweep()
+        dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
+            "sample": "1 sweep of the 16x16 U=4 beta=10 workload by the numpy/LAPACK FP64 oracle "
+                      f"(ZGEQP3 pivoting), one thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--delay", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
+              "independent chains shard across GPUs without communication, so use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+    from alf_amd.runtime import Engine
+    from alf_amd import kernelbench
+
+    ham = hubbard_ham_set(HubbardParams(**WORKLOAD))
+    ham.nsigma.set_uniform_four_state(0)
+    fields = ham.nsigma.f.copy()
+    eng = Engine(ham, delay=args.delay)
+    eng.set_fields(fields)
+    eng.set_philox(seed=0, stream=0)
+    eng.init()
+    for _ in range(args.warmup):
+        eng.sweep(1)
+    eng.sync()
+    eng.control_reset()
+    t0 = time.perf_counter()
+    ms = eng.sweep_timed(args.steps)
+    wall = time.perf_counter() - t0
+    ctl = eng.control()
+    n, L = ham.Ndim, ham.Ltrot
+    S = (L + ham.Nwrap - 1) // ham.Nwrap
+    acc = ctl["ACC_up"] / max(ctl["NC_up"], 1)
+    sweeps_per_s = args.steps / (ms / 1e3)
+
+    # dominant kernel: its average launch duration measured with HIP events on the engine's stream
+    kb = kernelbench.measure_dominant(eng)
+    result = {
+        "metric": "MC sweeps/s per GPU (FP64 drop-in path)",
+        "value": sweeps_per_s,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Hubbard 16x16 square, Ndim=256, U=4, beta=10, dtau=0.1 (Ltrot=100), Nwrap=10, "
+                               "complex128 G, 4-state HS field from seed 0, Philox stream",
+                   "path": "FP64 drop-in", "delay_k": args.delay,
+                   "acceptance": acc,
+                   "precision_green_mean": ctl["precision_green_mean"], "precision_green_max": ctl["XMAXG"],
+                   "precision_phase_max": ctl["XMAXP"],
+                   "wall_s": wall,
+                   "algorithmic_gflop_per_sweep": algorithmic_flops_per_sweep(n, L, S, acc) / 1e9,
+                   "sweep_fraction_of_fp64_peak": algorithmic_flops_per_sweep(n, L, S, acc) * sweeps_per_s / 78.6e12},
+        "roofline": kb,
+    }
+    eng.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(ham, fields)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

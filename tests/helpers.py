@@ -1,0 +1,28 @@
+"""Shared set-up for the tests: small Hubbard models, fields and tapes."""
+import numpy as np
+
+from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+from oracle.dqmc_oracle import OracleSweep, Tape, model_from_ham
+
+
+def hubbard(L1=4, L2=4, beta=1.0, U=4.0, nwrap=5, lattice="Square", n_phi=0, dtau=0.1, chem=0.0):
+    p = HubbardParams(Lattice_type=lattice, L1=L1, L2=L2, Beta=beta, Ham_U=U, Nwrap=nwrap, N_Phi=n_phi,
+                      Dtau=dtau, Ham_chem=chem)
+    return hubbard_ham_set(p)
+
+
+def fields_for(ham, seed=0):
+    ham.nsigma.set_uniform_four_state(seed)
+    return ham.nsigma.f.copy()
+
+
+def tape(n, seed=1):
+    return np.random.default_rng(seed).random(n)
+
+
+def oracle(ham, f, u, presort=False):
+    return OracleSweep(model_from_ham(ham), f, Tape(u), presort=presort)
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))

@@ -1,0 +1,234 @@
+"""GPU parity of the gfx950 sweep engine against the CPU oracle, through the C ABI.
+
+Tolerances: G and PHASE within 1e-10 relative (FP64 path, different but
+equally stable pivoting and summation orders); field configurations,
+acceptance counters and random-stream positions bit-exact (the same random
+tape drives both sides).
+"""
+import numpy as np
+import pytest
+
+from helpers import fields_for, hubbard, oracle, rel, tape
+from oracle import dqmc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from alf_amd import runtime
+    runtime.load_library()
+    return runtime
+
+
+def engine(rt, ham, f, u, delay=8):
+    e = rt.Engine(ham, delay=delay)
+    e.set_fields(f)
+    e.set_tape(u)
+    return e
+
+
+# ---------------------------------------------------------------- K1: the MFMA GEMM
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (37, 50, 29), (100, 36, 64), (256, 256, 256)])
+def test_gemm_against_numpy(rt, shape):
+    M, N, K = shape
+    r = np.random.default_rng(M * 7 + N)
+    A = r.standard_normal((M, K)) + 1j * r.standard_normal((M, K))
+    B = r.standard_normal((K, N)) + 1j * r.standard_normal((K, N))
+    Ar = r.standard_normal((M, K))
+    C0 = r.standard_normal((M, N)) + 1j * r.standard_normal((M, N))
+    tol = 1e-13 * K
+    assert np.abs(rt.gemm(A, B) - A @ B).max() < tol
+    assert np.abs(rt.gemm(Ar, B) - Ar @ B).max() < tol
+    assert np.abs(rt.gemm(A.conj().T.copy(), B, opa="C") - A @ B).max() < tol
+    assert np.abs(rt.gemm(A.T.copy(), B.T.copy(), opa="T", opb="T") - A @ B).max() < tol
+    got = rt.gemm(A, B.conj().T.copy(), opb="C", alpha=0.5 - 1j, beta=2.0, Cin=C0)
+    assert np.abs(got - ((0.5 - 1j) * (A @ B) + 2.0 * C0)).max() < tol
+
+
+# ---------------------------------------------------------------- K3/K4: decompose and CGR
+@pytest.mark.parametrize("L1,L2,beta,nwrap", [(4, 4, 1.0, 5), (6, 6, 4.0, 10), (8, 8, 2.0, 10)])
+def test_cgr_from_oracle_udv(rt, L1, L2, beta, nwrap):
+    ham = hubbard(L1=L1, L2=L2, beta=beta, nwrap=nwrap)
+    f = fields_for(ham, 1)
+    s = oracle(ham, f, tape(10))
+    s.init()
+    e = engine(rt, ham, f, tape(10))
+    e.set_udv(rt.ALFQ_UDVR, s.udvr.U, s.udvr.D, s.udvr.V, "R")
+    e.set_udv(rt.ALFQ_UDVL, s.udvl.U, s.udvl.D, s.udvl.V, "L")
+    for nvar in (1, 2):
+        z = e.cgr(nvar)
+        zo, Go = O.cgr(nvar, s.udvr, s.udvl, presort=False)     # reference pivoting (ZGEQP3)
+        assert rel(e.get_green(), Go) < TOL
+        assert abs(z - zo) < TOL
+    e.close()
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_wrap_and_decompose(rt, left):
+    ham = hubbard(L1=6, L2=6, beta=3.0, nwrap=10)
+    f = fields_for(ham, 2)
+    s = oracle(ham, f, tape(10), presort=True)    # same pivot strategy, so D is comparable too
+    e = engine(rt, ham, f, tape(10))
+    n = ham.Ndim
+    if left:
+        e.udv_reset(rt.ALFQ_UDVL, "L")
+        e.wrapul(30, 20)
+        e.wrapul(20, 10)
+        U, D, V = e.get_udv(rt.ALFQ_UDVL)
+        uo = s.wrapul(20, 10, s.wrapul(30, 20, O.UDV.identity(n, "L")))
+        got, want = (U * D) @ V.conj().T, (uo.U * uo.D) @ uo.V.conj().T
+    else:
+        e.udv_reset(rt.ALFQ_UDVR, "R")
+        e.wrapur(0, 10)
+        e.wrapur(10, 20)
+        U, D, V = e.get_udv(rt.ALFQ_UDVR)
+        uo = s.wrapur(10, 20, s.wrapur(0, 10, O.UDV.identity(n, "R")))
+        got, want = (U * D) @ V, (uo.U * uo.D) @ uo.V
+    assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-13
+    assert rel(np.sort(D.real), np.sort(uo.D.real)) < 1e-12
+    assert rel(got, want) < 1e-12
+    e.close()
+
+
+# ---------------------------------------------------------------- K2 + K1 per slice
+def test_wrapgrup_wrapgrdo_single_slices(rt):
+    ham = hubbard(L1=4, L2=4, beta=2.0, nwrap=10)
+    f = fields_for(ham, 3)
+    u = tape(100000)
+    s = oracle(ham, f, u)
+    s.init()
+    e = engine(rt, ham, f, u, delay=5)
+    e.set_green(s.G)
+    e.set_phase(s.phase)
+    for ntau in range(0, 4):
+        s.G = s.wrapgrup(s.G, ntau)
+        ph = e.wrapgrup(ntau, s.phase if ntau == 0 else e.get_phase())
+        assert rel(e.get_green(), s.G) < TOL
+        assert abs(ph - s.phase) < TOL
+    for ntau in range(4, 1, -1):
+        s.G = s.wrapgrdo(s.G, ntau)
+        ph = e.wrapgrdo(ntau, e.get_phase())
+        assert rel(e.get_green(), s.G) < TOL
+        assert abs(ph - s.phase) < TOL
+    assert np.array_equal(e.get_fields(), s.f)
+    assert e.rng_position() == s.rng.pos
+    c = e.control()
+    assert c["NC_up"] == s.ctrl.NC_up and c["ACC_up"] == s.ctrl.ACC_up
+    e.close()
+
+
+# ---------------------------------------------------------------- full sweeps
+@pytest.mark.parametrize("L1,L2,beta,nwrap,delay", [(4, 4, 5.0, 10, 32), (6, 6, 4.0, 10, 7), (4, 4, 2.0, 4, 1)])
+def test_sweeps_match_oracle(rt, L1, L2, beta, nwrap, delay):
+    ham = hubbard(L1=L1, L2=L2, beta=beta, nwrap=nwrap)
+    f = fields_for(ham, 0)
+    u = tape(2_000_000)
+    s = oracle(ham, f, u)
+    e = engine(rt, ham, f, u, delay=delay)
+    s.init()
+    e.init()
+    assert rel(e.get_green(), s.G) < TOL
+    for _ in range(2):
+        s.sweep()
+        e.sweep(1)
+        assert np.array_equal(e.get_fields(), s.f)
+        assert rel(e.get_green(), s.G) < TOL
+        assert abs(e.get_phase() - s.phase) < TOL
+        assert e.rng_position() == s.rng.pos
+    c = e.control()
+    assert c["NC_up"] == s.ctrl.NC_up and c["ACC_up"] == s.ctrl.ACC_up
+    assert c["NCG"] == s.ctrl.NCG
+    # Precision Green / Phase: roundoff statistics of the same trajectory
+    assert c["XMAXG"] < 1e-9 and s.ctrl.XMAXG < 1e-9
+    assert c["XMAXP"] < 1e-9
+    log = e.precision_log()
+    assert log.shape[0] == s.ctrl.NCG
+    # the final G is the Green's function of the final fields
+    Gb, _ = O.brute_force_green(s.m, e.get_fields(), 0, dps=30) if ham.Ndim <= 16 else (s.G, None)
+    assert rel(e.get_green(), Gb) < TOL
+    e.close()
+
+
+def test_complex_hopping_flux_sweep(rt):
+    """Honeycomb lattice with a Peierls flux: complex exp(-dtau T) (CmplxExpOpT path)."""
+    ham = hubbard(L1=3, L2=3, beta=2.0, U=3.0, nwrap=8, lattice="Honeycomb", n_phi=1)
+    assert not ham.Op_T.is_real
+    f = fields_for(ham, 4)
+    u = tape(500_000)
+    s = oracle(ham, f, u)
+    e = engine(rt, ham, f, u, delay=6)
+    s.init()
+    e.init()
+    s.sweep()
+    e.sweep(1)
+    assert np.array_equal(e.get_fields(), s.f)
+    assert rel(e.get_green(), s.G) < TOL
+    assert abs(e.get_phase() - s.phase) < TOL
+    e.close()
+
+
+def test_delay_depth_does_not_change_the_trajectory(rt):
+    ham = hubbard(L1=6, L2=6, beta=2.0, nwrap=10)
+    f = fields_for(ham, 9)
+    u = tape(1_000_000)
+    res = []
+    for k in (1, 3, 16, 64):
+        e = engine(rt, ham, f, u, delay=k)
+        e.init()
+        e.sweep(1)
+        res.append((e.get_fields(), e.get_green(), e.control()["ACC_up"]))
+        e.close()
+    for fl, G, acc in res[1:]:
+        assert np.array_equal(fl, res[0][0]) and acc == res[0][2]
+        assert rel(G, res[0][1]) < 1e-12
+
+
+def test_philox_stream_is_deterministic(rt):
+    ham = hubbard(L1=4, L2=4, beta=2.0, nwrap=10)
+    f = fields_for(ham, 5)
+    out = []
+    for seed, stream in ((7, 0), (7, 0), (7, 1)):
+        e = rt.Engine(ham, delay=8)
+        e.set_fields(f)
+        e.set_philox(seed, stream)
+        e.init()
+        e.sweep(2)
+        out.append((e.get_fields(), e.get_green()))
+        assert e.control()["XMAXG"] < 1e-9
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert not np.array_equal(out[0][0], out[2][0])
+
+
+def test_exhausted_tape_is_an_error_not_a_fault(rt):
+    ham = hubbard(L1=4, L2=4, beta=1.0, nwrap=5)
+    f = fields_for(ham, 0)
+    e = engine(rt, ham, f, tape(100))
+    e.init()
+    with pytest.raises(rt.AlfqError) as ei:
+        e.sweep(1)
+    assert ei.value.code == 1
+    e.close()
+
+
+@pytest.mark.slow
+def test_real_size_single_sweep_matches_oracle(rt):
+    """The 16x16, U=4, beta=10 workload: one sweep on the same tape."""
+    ham = hubbard(L1=16, L2=16, beta=10.0, nwrap=10)
+    f = fields_for(ham, 0)
+    u = tape(2 * 3 * 256 * 100 + 10)
+    s = oracle(ham, f, u)
+    e = engine(rt, ham, f, u, delay=32)
+    s.init()
+    e.init()
+    assert rel(e.get_green(), s.G) < TOL
+    s.sweep()
+    e.sweep(1)
+    assert np.array_equal(e.get_fields(), s.f)
+    assert e.control()["ACC_up"] == s.ctrl.ACC_up
+    assert rel(e.get_green(), s.G) < TOL
+    assert abs(e.get_phase() - s.phase) < TOL
+    e.close()
