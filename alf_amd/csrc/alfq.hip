@@ -132,6 +132,7 @@ __global__ __launch_bounds__(1024) void stabilise_kernel(const cplx* __restrict_
 __global__ void chain_reset_kernel(ChainState* st) {
   st->pos = 0;
   st->m = 0;
+  st->rng_slice_base = st->rng_pos;
 }
 
 inline unsigned nblk(long long count, int per) { return (unsigned)((count + per - 1) / per); }
@@ -156,6 +157,8 @@ struct alfq_ctx {
   cplx* expalpha = nullptr;
   cplx* phasefac = nullptr;
   double* gama_ratio = nullptr;
+  int* uoff_up = nullptr;                // uniforms consumed before visit v (ops ascending)
+  int* uoff_dn = nullptr;                // the same for the descending visit order
   std::vector<int> h_op_type;
   // state
   int8_t* f = nullptr;
@@ -284,6 +287,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.op_myexp = c->myexp; a.op_expalpha = c->expalpha; a.gama_ratio = c->gama_ratio;
   a.n_sun = c->n_sun;
   a.u = c->u; a.u_base = c->u_base; a.u_len = c->u_len; a.st = c->st;
+  a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
   hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(1), 0, c->s, c->st);
   HIP_OK(hipGetLastError());
   const int rounds = c->n_op / c->K + 1;
@@ -558,7 +562,7 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
   c->stab.assign(c->nstm + 1, 0);
   for (int k = 1; k < c->nstm; ++k) c->stab[k] = c->W * k;
   c->stab[c->nstm] = c->L;
-  c->K = std::min(32, std::max(1, c->n_op));
+  c->K = std::max(1, std::min(chain_max_delay(d->ndim) > 0 ? chain_max_delay(d->ndim) : 32, std::max(1, d->n_op)));
   const int n = c->n;
   const size_t nn = (size_t)n * n;
   auto fail = [&](const char* what) {
@@ -627,6 +631,17 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
   c->h_op_type = typ;
   c->per_visit_sum = 0;
   for (int i = 0; i < no; ++i) c->per_visit_sum += (typ[i] == 2) ? 3 : 2;
+  {
+    std::vector<int> up(no + 1, 0), dn(no + 1, 0);
+    for (int v = 0; v < no; ++v) {
+      up[v + 1] = up[v] + ((typ[v] == 2) ? 3 : 2);
+      dn[v + 1] = dn[v] + ((typ[no - 1 - v] == 2) ? 3 : 2);
+    }
+    if (hipMalloc(&c->uoff_up, (no + 1) * sizeof(int)) || hipMalloc(&c->uoff_dn, (no + 1) * sizeof(int)) ||
+        hipMemcpy(c->uoff_up, up.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->uoff_dn, dn.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice))
+      return fail("uniform offsets");
+  }
   const int nop1 = std::max(no, 1);
   if (hipMalloc(&c->op_site, nop1 * sizeof(int)) || hipMalloc(&c->op_type, nop1 * sizeof(int)) ||
       hipMalloc(&c->site_ptr, (n + 1) * sizeof(int)) || hipMalloc(&c->site_ops, nop1 * sizeof(int)) ||
@@ -685,7 +700,7 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (!c) return ST_NONE;
   if (c->s) hipStreamSynchronize(c->s);
   for (auto& sl : c->pool) { hipFree(sl.U); hipFree(sl.V); hipFree(sl.D); hipFree(sl.sc); }
-  void* ptrs[] = {c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
+  void* ptrs[] = {c->uoff_up, c->uoff_dn, c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
                   c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->st, c->sc,
                   c->log, c->dv_d, c->dv_dinv, c->dv_dconj, c->U0, c->U1, c->TP, c->Qc, c->DUP, c->DUPinv,
                   c->Xc, c->Yc, c->cscal, c->u};

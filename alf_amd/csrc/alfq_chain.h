@@ -13,6 +13,7 @@ struct ChainState {
   int m;                          // pending delayed updates in X/Y
   int status;                     // 1: random stream exhausted
   int pad;
+  unsigned long long rng_slice_base;  // rng_pos at the start of the current slice
 };
 
 struct ChainArgs {
@@ -25,10 +26,13 @@ struct ChainArgs {
   const double* gama_ratio;       // n_op x 25: Gama(new)/Gama(old)
   int n_sun;
   const double* u; unsigned long long u_base; long long u_len;
+  const int* uoff;                // uniforms consumed before visit v of the slice (n_op + 1 entries)
   ChainState* st;
 };
 
 int launch_chain(const ChainArgs& a, hipStream_t s);
+// largest delay depth the register-resident chain kernel supports for n sites (0: generic kernel)
+int chain_max_delay(int n);
 int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,
                        unsigned stream, hipStream_t s);
 int launch_dvec(const int8_t* f, int n_op, int s0, int ns, int n, const int* site_ptr, const int* site_ops,

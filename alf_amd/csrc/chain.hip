@@ -186,6 +186,212 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Register-resident chain kernel.  Thread t owns rows t, t + 256, ... (RPT rows) and
+// keeps X(r, 0:KT), Y(r, 0:KT) of its rows in VGPRs.  Everything about a visit that
+// does not depend on earlier decisions -- the proposal (the field of an operator
+// changes only at its own visit), the table entries, the uniforms, the stale G_PP --
+// is precomputed into LDS in a parallel prologue.  The sequential part per visit:
+// the owner of row P forms G_cur(P,P) from its registers, decides, and publishes
+// (accept, x, X(P,:), Y(P,:)) in a double-buffered LDS slot; one barrier; on accept
+// every thread extends its rows of X and Y.  Column and row P of the stale G are
+// prefetched one visit ahead.
+struct Visit {
+  cplx myexp, expal, gpp;
+  double gr, u3;
+  int P, snew, nop, ok;
+};
+
+__device__ __forceinline__ void block_sync_lds() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+template <int RPT, int KT>
+__global__ __launch_bounds__(256) void chain2_kernel(ChainArgs a) {
+  extern __shared__ char smem_raw[];
+  Visit* vis = (Visit*)smem_raw;                        // n_op entries
+  __shared__ cplx sX[2][KT], sY[2][KT];
+  __shared__ cplx s_x[2];
+  __shared__ int s_acc[2];
+  __shared__ cplx s_phase;
+  __shared__ long long s_nc, s_accn;
+  __shared__ int s_stop_at;
+
+  const int tid = threadIdx.x;
+  const int n = a.n;
+  ChainState* st = a.st;
+  const int pos0 = st->pos;
+  const unsigned long long base = st->rng_slice_base;
+  // ---- prologue: per-visit proposal, tables, uniforms and stale G_PP ----
+  if (tid == 0) {
+    s_phase = st->phase;
+    s_nc = st->nc;
+    s_accn = st->acc;
+    s_stop_at = a.n_op;
+  }
+  for (int v = pos0 + tid; v < a.n_op; v += blockDim.x) {
+    Visit w;
+    const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
+    const int typ = a.op_type[nop];
+    const int s_old = a.f[(long long)a.slice * a.n_op + nop];
+    const unsigned long long q = base + (unsigned long long)a.uoff[v];
+    const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
+    w.ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
+    int s_new = -s_old;
+    double u3 = 2.0;
+    if (w.ok) {
+      const double* u = a.u + (q - a.u_base);
+      if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
+    }
+    const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
+    const int P = a.op_site[nop];
+    w.myexp = a.op_myexp[ti];
+    w.expal = a.op_expalpha[ti];
+    w.gr = a.gama_ratio[ti];
+    w.u3 = u3;
+    w.P = P;
+    w.snew = s_new;
+    w.nop = nop;
+    w.gpp = a.G[P + (long long)P * n];
+    vis[v] = w;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int v = pos0; v < a.n_op; ++v)
+      if (!vis[v].ok) { s_stop_at = v; break; }
+  }
+
+  cplx Xr[RPT][KT], Yr[RPT][KT];
+#pragma unroll
+  for (int rr = 0; rr < RPT; ++rr)
+#pragma unroll
+    for (int j = 0; j < KT; ++j) { Xr[rr][j] = make_double2(0.0, 0.0); Yr[rr][j] = make_double2(0.0, 0.0); }
+
+  cplx pc[RPT], pr[RPT], nc_[RPT], nr_[RPT];
+  __syncthreads();
+  const int stop_at = s_stop_at;
+  int pos = pos0, m = 0;
+  if (pos < stop_at) {
+    const int P = vis[pos].P;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      const int r = tid + 256 * rr;
+      pc[rr] = r < n ? a.G[r + (long long)P * n] : make_double2(0.0, 0.0);
+      pr[rr] = r < n ? a.G[P + (long long)r * n] : make_double2(0.0, 0.0);
+    }
+  }
+  const int kmax = a.Kmax < KT ? a.Kmax : KT;
+  int buf = 0;
+  while (pos < stop_at && m < kmax) {
+    const Visit w = vis[pos];
+    const int P = w.P;
+    // prefetch column/row of the next visit's site
+    if (pos + 1 < stop_at) {
+      const int P1 = vis[pos + 1].P;
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        const int r = tid + 256 * rr;
+        nc_[rr] = r < n ? a.G[r + (long long)P1 * n] : make_double2(0.0, 0.0);
+        nr_[rr] = r < n ? a.G[P1 + (long long)r * n] : make_double2(0.0, 0.0);
+      }
+    }
+    // owner of row P decides
+    const int owner = P & 255, orr = P >> 8;
+    if (tid == owner) {
+      cplx d = w.gpp;
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        if (rr == orr) {
+#pragma unroll
+          for (int j = 0; j < KT; ++j) {
+            if (j < m) {
+              d = cfma(Xr[rr][j], Yr[rr][j], d);
+              sX[buf][j] = Xr[rr][j];
+              sY[buf][j] = Yr[rr][j];
+            }
+          }
+        }
+      }
+      const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
+      const cplx dmat = csub(w.myexp, cmul(delta, d));
+      const cplx ratio = cmul(dmat, w.expal);
+      cplx rt = ratio;
+      for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);
+      rt = cscale(rt, w.gr);
+      const cplx ph = s_phase;
+      const double weight = fabs(cmul(ph, rt).x / ph.x);
+      s_nc += 1;
+      int acc = 0;
+      if (weight > w.u3) {
+        acc = 1;
+        s_accn += 1;
+        s_phase = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
+        a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
+        const cplx vp = make_double2(1.0 - d.x, -d.y);
+        s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
+      }
+      s_acc[buf] = acc;
+    }
+    block_sync_lds();
+    if (s_acc[buf]) {
+      const cplx x = s_x[buf];
+      const cplx mx = make_double2(-x.x, -x.y);
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        const int r = tid + 256 * rr;
+        cplx col = pc[rr], row = pr[rr];
+#pragma unroll
+        for (int j = 0; j < KT; ++j) {
+          if (j < m) {
+            col = cfma(Xr[rr][j], sY[buf][j], col);
+            row = cfma(sX[buf][j], Yr[rr][j], row);
+          }
+        }
+        const cplx xn = cmul(mx, col);
+        const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);
+#pragma unroll
+        for (int j = 0; j < KT; ++j) {
+          if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }
+        }
+      }
+      ++m;
+    }
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) { pc[rr] = nc_[rr]; pr[rr] = nr_[rr]; }
+    ++pos;
+    buf ^= 1;
+  }
+  // ---- hand the pending columns to the flush and save the chain state ----
+#pragma unroll
+  for (int rr = 0; rr < RPT; ++rr) {
+    const int r = tid + 256 * rr;
+    if (r < n) {
+#pragma unroll
+      for (int j = 0; j < KT; ++j) {
+        if (j < m) {
+          a.X[r + (long long)j * n] = Xr[rr][j];
+          a.Y[r + (long long)j * n] = Yr[rr][j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    st->phase = s_phase;
+    st->nc = s_nc;
+    st->acc = s_accn;
+    st->pos = pos;
+    st->m = m;
+    st->rng_pos = base + (unsigned long long)a.uoff[pos];
+    if (pos == stop_at && stop_at < a.n_op) {
+      st->status = 1;
+      st->pos = a.n_op;    // do not revisit
+    }
+  }
+}
+
 // Diagonal exponentials of slices [s0, s0 + ns): for each site i,
 //   d(i) = prod_{ops at i} E_exp(op, f),  dinv(i) = prod E_exp(op, -f),  dconj(i) = conj(d(i)).
 __global__ void dvec_kernel(const int8_t* __restrict__ f, int n_op, int s0, int ns, int n,
@@ -238,10 +444,40 @@ int launch_philox_fill(double* u, unsigned long long base, long long count, unsi
   return ST_NONE;
 }
 
+int chain_max_delay(int n) {
+  if (n <= 256) return 32;
+  if (n <= 512) return 16;
+  if (n <= 1024) return 8;
+  if (n <= 2048) return 4;
+  return 0;
+}
+
+template <int RPT, int KT>
+static int launch_chain2(const ChainArgs& a, hipStream_t s) {
+  const size_t lds = sizeof(Visit) * (size_t)(a.n_op > 0 ? a.n_op : 1);
+  static bool attr_set = false;
+  if (lds > 64 * 1024 && !attr_set) {
+    HIP_OK(hipFuncSetAttribute((const void*)chain2_kernel<RPT, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               96 * 1024));
+    attr_set = true;
+  }
+  if (lds > 96 * 1024) { alfq_set_error("launch_chain: too many operators per slice for the LDS tables"); return ST_GENERIC; }
+  hipLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, a);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
+
 int launch_chain(const ChainArgs& a, hipStream_t s) {
   if (a.Kmax < 1 || a.Kmax > CHAIN_KMAX) {
     alfq_set_error("launch_chain: K outside [1, CHAIN_KMAX]");
     return ST_GENERIC;
+  }
+  const int kr = chain_max_delay(a.n);
+  if (kr > 0 && a.Kmax <= kr && a.uoff) {
+    if (a.n <= 256) return launch_chain2<1, 32>(a, s);
+    if (a.n <= 512) return launch_chain2<2, 16>(a, s);
+    if (a.n <= 1024) return launch_chain2<4, 8>(a, s);
+    return launch_chain2<8, 4>(a, s);
   }
   hipLaunchKernelGGL(chain_kernel, dim3(1), dim3(256), 0, s, a);
   HIP_OK(hipGetLastError());
