@@ -199,6 +199,7 @@ struct alfq_ctx {
   unsigned stream_id = 0;
   long long per_visit_sum = 0;           // uniforms per pass over all ops
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  KTimer chain_timer;
 };
 
 namespace {
@@ -292,7 +293,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   HIP_OK(hipGetLastError());
   const int rounds = c->n_op / c->K + 1;
   for (int r = 0; r < rounds; ++r) {
-    if ((st = launch_chain(a, c->s))) return st;
+    if ((st = launch_chain(a, c->s, &c->chain_timer))) return st;
     // G += X Y^T with the rank read from the chain state
     GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
     g.kdev = &c->st->m;
@@ -707,6 +708,7 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->qr.A) qr_work_free(c->qr);
+  for (auto& pr : c->chain_timer.ev) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->s) hipStreamDestroy(c->s);
@@ -1035,6 +1037,28 @@ int alfq_precision_log(alfq_ctx* c, double* out3, int64_t cap, int64_t* count) {
   *count = ds.nlog;
   const int64_t k = std::min<int64_t>(avail, cap);
   if (out3 && k > 0) HIP_OK(hipMemcpy(out3, c->log, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
+int alfq_kernel_timing(alfq_ctx* c, int32_t enable) {
+  if (!c) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  c->chain_timer.on = enable != 0;
+  c->chain_timer.used = 0;
+  return ST_NONE;
+}
+
+int alfq_kernel_timing_get(alfq_ctx* c, double* total_ms, int64_t* launches) {
+  if (!c || !total_ms || !launches) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  double t = 0.0;
+  for (size_t i = 0; i < c->chain_timer.used; ++i) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, c->chain_timer.ev[i].first, c->chain_timer.ev[i].second));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = (int64_t)c->chain_timer.used;
   return ST_NONE;
 }
 

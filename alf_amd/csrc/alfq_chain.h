@@ -2,6 +2,9 @@
 #pragma once
 #include "alfq_common.h"
 
+#include <utility>
+#include <vector>
+
 #define CHAIN_KMAX 64   // largest delayed-update depth
 
 // Per-chain device state touched by the sequential Metropolis kernel.
@@ -30,7 +33,15 @@ struct ChainArgs {
   ChainState* st;
 };
 
-int launch_chain(const ChainArgs& a, hipStream_t s);
+// Optional per-launch timing of the chain kernel: start/stop events recorded by the
+// launch itself (hipExtLaunchKernelGGL) on the launch's stream.
+struct KTimer {
+  bool on = false;
+  size_t used = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* timer = nullptr);
 // largest delay depth the register-resident chain kernel supports for n sites (0: generic kernel)
 int chain_max_delay(int n);
 int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,

@@ -4,6 +4,7 @@
 
 struct QRWork {
   int n = 0, nb = 0, nblk = 0;
+  bool lds_panel = false; // panel factorisation in LDS (n <= 2048)
   cplx* A = nullptr;      // n x n factorisation workspace
   cplx* Vw = nullptr;     // explicit Householder panels (n x n)
   cplx* Y1 = nullptr;     // V T^H per panel

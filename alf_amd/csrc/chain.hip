@@ -16,6 +16,8 @@
 // operators they are a similarity transform G -> D G D^-1 that commutes with
 // the rank-1 update and leaves G_PP unchanged, so all of a slice's scalings are
 // fused into the epilogue of the adjacent hopping GEMM.
+#include <hip/hip_ext.h>
+
 #include "alfq_chain.h"
 
 namespace {
@@ -209,7 +211,7 @@ __device__ __forceinline__ void block_sync_lds() {
 }
 
 template <int RPT, int KT>
-__global__ __launch_bounds__(256) void chain2_kernel(ChainArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void chain2_kernel(ChainArgs a) {
   extern __shared__ char smem_raw[];
   Visit* vis = (Visit*)smem_raw;                        // n_op entries
   __shared__ cplx sX[2][KT], sY[2][KT];
@@ -269,99 +271,110 @@ __global__ __launch_bounds__(256) void chain2_kernel(ChainArgs a) {
 #pragma unroll
     for (int j = 0; j < KT; ++j) { Xr[rr][j] = make_double2(0.0, 0.0); Yr[rr][j] = make_double2(0.0, 0.0); }
 
-  cplx pc[RPT], pr[RPT], nc_[RPT], nr_[RPT];
+  // column and row P of the stale G for the next PD visits, one ring slot per visit
+  constexpr int PD = (RPT == 1) ? 4 : 2;
+  cplx rc[PD][RPT], rw[PD][RPT];
   __syncthreads();
   const int stop_at = s_stop_at;
   int pos = pos0, m = 0;
-  if (pos < stop_at) {
-    const int P = vis[pos].P;
+#pragma unroll
+  for (int S = 0; S < PD; ++S) {
+    const int v = pos0 + S;
+    const int P = v < stop_at ? vis[v].P : 0;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
       const int r = tid + 256 * rr;
-      pc[rr] = r < n ? a.G[r + (long long)P * n] : make_double2(0.0, 0.0);
-      pr[rr] = r < n ? a.G[P + (long long)r * n] : make_double2(0.0, 0.0);
+      const bool ok = v < stop_at && r < n;
+      rc[S][rr] = ok ? a.G[r + (long long)P * n] : make_double2(0.0, 0.0);
+      rw[S][rr] = ok ? a.G[P + (long long)r * n] : make_double2(0.0, 0.0);
     }
   }
   const int kmax = a.Kmax < KT ? a.Kmax : KT;
   int buf = 0;
   while (pos < stop_at && m < kmax) {
-    const Visit w = vis[pos];
-    const int P = w.P;
-    // prefetch column/row of the next visit's site
-    if (pos + 1 < stop_at) {
-      const int P1 = vis[pos + 1].P;
 #pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        const int r = tid + 256 * rr;
-        nc_[rr] = r < n ? a.G[r + (long long)P1 * n] : make_double2(0.0, 0.0);
-        nr_[rr] = r < n ? a.G[P1 + (long long)r * n] : make_double2(0.0, 0.0);
-      }
-    }
-    // owner of row P decides
-    const int owner = P & 255, orr = P >> 8;
-    if (tid == owner) {
-      cplx d = w.gpp;
+    for (int S = 0; S < PD; ++S) {
+      if (pos < stop_at && m < kmax) {
+        const Visit w = vis[pos];
+        const int P = w.P;
+        // owner of row P decides
+        const int owner = P & 255, orr = P >> 8;
+        if (tid == owner) {
+          cplx d = w.gpp;
 #pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        if (rr == orr) {
+          for (int rr = 0; rr < RPT; ++rr) {
+            if (rr == orr) {
 #pragma unroll
-          for (int j = 0; j < KT; ++j) {
-            if (j < m) {
-              d = cfma(Xr[rr][j], Yr[rr][j], d);
-              sX[buf][j] = Xr[rr][j];
-              sY[buf][j] = Yr[rr][j];
+              for (int j = 0; j < KT; ++j) {
+                if (j < m) {
+                  d = cfma(Xr[rr][j], Yr[rr][j], d);
+                  sX[buf][j] = Xr[rr][j];
+                  sY[buf][j] = Yr[rr][j];
+                }
+              }
+            }
+          }
+          const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
+          const cplx dmat = csub(w.myexp, cmul(delta, d));
+          const cplx ratio = cmul(dmat, w.expal);
+          cplx rt = ratio;
+          for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);
+          rt = cscale(rt, w.gr);
+          const cplx ph = s_phase;
+          const double weight = fabs(cmul(ph, rt).x / ph.x);
+          s_nc += 1;
+          int acc = 0;
+          if (weight > w.u3) {
+            acc = 1;
+            s_accn += 1;
+            s_phase = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
+            a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
+            const cplx vp = make_double2(1.0 - d.x, -d.y);
+            s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
+          }
+          s_acc[buf] = acc;
+        }
+        block_sync_lds();
+        if (s_acc[buf]) {
+          const cplx x = s_x[buf];
+          const cplx mx = make_double2(-x.x, -x.y);
+#pragma unroll
+          for (int rr = 0; rr < RPT; ++rr) {
+            const int r = tid + 256 * rr;
+            cplx col = rc[S][rr], row = rw[S][rr];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) {
+              if (j < m) {
+                col = cfma(Xr[rr][j], sY[buf][j], col);
+                row = cfma(sX[buf][j], Yr[rr][j], row);
+              }
+            }
+            const cplx xn = cmul(mx, col);
+            const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);
+#pragma unroll
+            for (int j = 0; j < KT; ++j) {
+              if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }
+            }
+          }
+          m = __builtin_amdgcn_readfirstlane(m + 1);
+        }
+        // refill this slot with the visit PD ahead
+        {
+          const int v = pos + PD;
+          const int Pn = v < stop_at ? vis[v].P : 0;
+#pragma unroll
+          for (int rr = 0; rr < RPT; ++rr) {
+            const int r = tid + 256 * rr;
+            if (v < stop_at && r < n) {
+              rc[S][rr] = a.G[r + (long long)Pn * n];
+              rw[S][rr] = a.G[Pn + (long long)r * n];
             }
           }
         }
+        pos = __builtin_amdgcn_readfirstlane(pos + 1);
+        buf ^= 1;
       }
-      const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
-      const cplx dmat = csub(w.myexp, cmul(delta, d));
-      const cplx ratio = cmul(dmat, w.expal);
-      cplx rt = ratio;
-      for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);
-      rt = cscale(rt, w.gr);
-      const cplx ph = s_phase;
-      const double weight = fabs(cmul(ph, rt).x / ph.x);
-      s_nc += 1;
-      int acc = 0;
-      if (weight > w.u3) {
-        acc = 1;
-        s_accn += 1;
-        s_phase = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
-        a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
-        const cplx vp = make_double2(1.0 - d.x, -d.y);
-        s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
-      }
-      s_acc[buf] = acc;
     }
-    block_sync_lds();
-    if (s_acc[buf]) {
-      const cplx x = s_x[buf];
-      const cplx mx = make_double2(-x.x, -x.y);
-#pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        const int r = tid + 256 * rr;
-        cplx col = pc[rr], row = pr[rr];
-#pragma unroll
-        for (int j = 0; j < KT; ++j) {
-          if (j < m) {
-            col = cfma(Xr[rr][j], sY[buf][j], col);
-            row = cfma(sX[buf][j], Yr[rr][j], row);
-          }
-        }
-        const cplx xn = cmul(mx, col);
-        const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);
-#pragma unroll
-        for (int j = 0; j < KT; ++j) {
-          if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }
-        }
-      }
-      ++m;
-    }
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) { pc[rr] = nc_[rr]; pr[rr] = nr_[rr]; }
-    ++pos;
-    buf ^= 1;
   }
   // ---- hand the pending columns to the flush and save the chain state ----
 #pragma unroll
@@ -453,7 +466,7 @@ int chain_max_delay(int n) {
 }
 
 template <int RPT, int KT>
-static int launch_chain2(const ChainArgs& a, hipStream_t s) {
+static int launch_chain2(const ChainArgs& a, hipStream_t s, KTimer* t) {
   const size_t lds = sizeof(Visit) * (size_t)(a.n_op > 0 ? a.n_op : 1);
   static bool attr_set = false;
   if (lds > 64 * 1024 && !attr_set) {
@@ -462,22 +475,33 @@ static int launch_chain2(const ChainArgs& a, hipStream_t s) {
     attr_set = true;
   }
   if (lds > 96 * 1024) { alfq_set_error("launch_chain: too many operators per slice for the LDS tables"); return ST_GENERIC; }
-  hipLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, a);
+  if (t && t->on) {
+    if (t->used == t->ev.size()) {
+      hipEvent_t e0, e1;
+      HIP_OK(hipEventCreate(&e0));
+      HIP_OK(hipEventCreate(&e1));
+      t->ev.emplace_back(e0, e1);
+    }
+    auto& pr = t->ev[t->used++];
+    hipExtLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, pr.first, pr.second, 0, a);
+  } else {
+    hipLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, a);
+  }
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
 
-int launch_chain(const ChainArgs& a, hipStream_t s) {
+int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
   if (a.Kmax < 1 || a.Kmax > CHAIN_KMAX) {
     alfq_set_error("launch_chain: K outside [1, CHAIN_KMAX]");
     return ST_GENERIC;
   }
   const int kr = chain_max_delay(a.n);
   if (kr > 0 && a.Kmax <= kr && a.uoff) {
-    if (a.n <= 256) return launch_chain2<1, 32>(a, s);
-    if (a.n <= 512) return launch_chain2<2, 16>(a, s);
-    if (a.n <= 1024) return launch_chain2<4, 8>(a, s);
-    return launch_chain2<8, 4>(a, s);
+    if (a.n <= 256) return launch_chain2<1, 32>(a, s, t);
+    if (a.n <= 512) return launch_chain2<2, 16>(a, s, t);
+    if (a.n <= 1024) return launch_chain2<4, 8>(a, s, t);
+    return launch_chain2<8, 4>(a, s, t);
   }
   hipLaunchKernelGGL(chain_kernel, dim3(1), dim3(256), 0, s, a);
   HIP_OK(hipGetLastError());

@@ -13,6 +13,8 @@
 // zlarfg conventions: v(1) = 1, tau = (beta - alpha)/beta, H = I - tau v v^H)
 // and forms the compact-WY factor T (zlarft, forward columnwise); the trailing
 // matrix and the explicit Q are updated with the MFMA GEMM.
+#include <algorithm>
+
 #include "alfq_qr.h"
 
 namespace {
@@ -240,6 +242,216 @@ __global__ __launch_bounds__(PT) void panel_kernel(cplx* __restrict__ A, int n, 
   }
 }
 
+// ---- panel factorisation with the panel resident in LDS -------------------------
+// Same contract as panel_kernel.  The rows x ncol panel sits in dynamic LDS; for each
+// column the reflector is formed from the norm left by the previous step, and the
+// waves apply H^H to the remaining columns (wave w owns columns j+1+w, j+5+w, ...)
+// with the reflector's scale folded in, so one barrier separates two columns.  The
+// Gram matrix V^H V, and Y1 = V T^H, Y2 = V T, are MFMA products out of LDS.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+template <class FA, class FB>
+__device__ __forceinline__ void cmfma16(int K, FA A, FB B, v4d_t& re, v4d_t& im) {
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const cplx a = A(lr, k0 + lk);
+    const cplx b = B(k0 + lk, lr);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, im, 0, 0, 0);
+  }
+}
+
+constexpr int PB = 32;          // widest LDS panel
+constexpr int QPW = PB / 4;     // panel columns per wave
+
+__global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int n, int k, int nb,
+                                                       cplx* __restrict__ tau, cplx* __restrict__ Vw,
+                                                       cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
+                                                       cplx* __restrict__ Tw, cplx* __restrict__ pdet) {
+  extern __shared__ cplx sp[];                 // rows x ncol, column-major, ld = rows
+  __shared__ cplx s_TG[PB * PB];               // upper incl. diagonal: T; strict lower: Gram
+  __shared__ cplx s_tau[PB], s_sc[PB];
+  __shared__ double s_beta[PB], s_norm[PB + 1], s_vn2[PB];
+  __shared__ double red[PT / 64];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int rows = n - k;
+  const int ncol = (k + nb <= n) ? nb : n - k;
+
+  for (long long idx = tid; idx < (long long)rows * ncol; idx += blockDim.x) {
+    const int r = (int)(idx % rows), c = (int)(idx / rows);
+    sp[idx] = A[(k + r) + (long long)(k + c) * n];
+  }
+  __syncthreads();
+  {
+    double xs = 0.0;
+    for (int r = 1 + tid; r < rows; r += blockDim.x) xs += cabs2(sp[r]);
+    xs = block_sum(xs, red);
+    if (tid == 0) s_norm[0] = xs;
+  }
+  __syncthreads();
+
+  for (int j = 0; j < ncol; ++j) {
+    const cplx* xj = sp + (long long)j * rows;
+    const cplx alpha = xj[j];
+    const double xn2 = s_norm[j];
+    cplx t = make_double2(0.0, 0.0), sc = make_double2(1.0, 0.0);
+    double beta = alpha.x;
+    if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+      t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      sc = cinv(make_double2(alpha.x - beta, alpha.y));
+    }
+    if (tid == 0) {
+      s_tau[j] = t;
+      s_sc[j] = sc;
+      s_beta[j] = beta;
+      tau[k + j] = t;
+    }
+    // dots d_q = sum_{r>j} conj(x_r) A(r, q) for this wave's columns
+    cplx d[QPW];
+#pragma unroll
+    for (int i = 0; i < QPW; ++i) d[i] = make_double2(0.0, 0.0);
+    for (int r = j + 1 + lane; r < rows; r += 64) {
+      const cplx xr = xj[r];
+#pragma unroll
+      for (int i = 0; i < QPW; ++i) {
+        const int q = j + 1 + wave + 4 * i;
+        if (q < ncol) {
+          const cplx a = sp[r + (long long)q * rows];
+          d[i].x += xr.x * a.x + xr.y * a.y;
+          d[i].y += xr.x * a.y - xr.y * a.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int i = 0; i < QPW; ++i) {
+        d[i].x += __shfl_xor(d[i].x, o, 64);
+        d[i].y += __shfl_xor(d[i].y, o, 64);
+      }
+    }
+    const cplx tc = cconj(t), scc = cconj(sc);
+#pragma unroll
+    for (int i = 0; i < QPW; ++i) {
+      const int q = j + 1 + wave + 4 * i;
+      if (q < ncol) {
+        cplx* aq = sp + (long long)q * rows;
+        const cplx w = cadd(aq[j], cmul(scc, d[i]));
+        const cplx f = cmul(tc, w);
+        const cplx scf = cmul(sc, f);
+        for (int r = j + lane; r < rows; r += 64) {
+          if (r == j) aq[j] = csub(aq[j], f);
+          else aq[r] = csub(aq[r], cmul(xj[r], scf));
+        }
+        if (q == j + 1) {
+          double xs = 0.0;
+          for (int r = j + 2 + lane; r < rows; r += 64) xs += cabs2(aq[r]);
+          xs = wave_sum(xs);
+          if (lane == 0) s_norm[j + 1] = xs;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // finalise the reflectors in place: R on and above the diagonal, scaled v below
+  for (long long idx = tid; idx < (long long)rows * ncol; idx += blockDim.x) {
+    const int r = (int)(idx % rows), c = (int)(idx / rows);
+    if (r == c) sp[idx] = make_double2(s_beta[c], 0.0);
+    else if (r > c) sp[idx] = cmul(sp[idx], s_sc[c]);
+  }
+  __syncthreads();
+  for (long long idx = tid; idx < (long long)rows * ncol; idx += blockDim.x) {
+    const int r = (int)(idx % rows), c = (int)(idx / rows);
+    const cplx a = sp[idx];
+    A[(k + r) + (long long)(k + c) * n] = a;
+    cplx v = make_double2(0.0, 0.0);
+    if (r == c) v = make_double2(1.0, 0.0);
+    else if (r > c) v = a;
+    Vw[(k + r) + (long long)(k + c) * n] = v;
+  }
+  auto Vget = [&](int r, int c) -> cplx {
+    if (r >= rows || c >= ncol || r < c) return make_double2(0.0, 0.0);
+    if (r == c) return make_double2(1.0, 0.0);
+    return sp[r + (long long)c * rows];
+  };
+  // Gram = V^H V, one 16 x 16 block per wave
+  {
+    const int bi = wave & 1, bj = wave >> 1;
+    v4d_t re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+    if (16 * bi < ncol && 16 * bj < ncol) {
+      cmfma16(
+          rows, [&](int i, int kk) { return cconj(Vget(kk, 16 * bi + i)); },
+          [&](int kk, int jj) { return Vget(kk, 16 * bj + jj); }, re, im);
+      const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int gi = 16 * bi + 4 * r4 + lk, gj = 16 * bj + lr;
+        if (gi < ncol && gj < ncol) {
+          if (gi > gj) s_TG[gi + PB * gj] = make_double2(re[r4], im[r4]);
+          if (gi == gj) s_vn2[gi] = re[r4];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // T recurrence (wave 0; lane i owns row i of T)
+  if (wave == 0 && lane < ncol) {
+    const int i = lane;
+    s_TG[i + PB * i] = s_tau[i];
+    for (int j = i + 1; j < ncol; ++j) {
+      cplx acc = make_double2(0.0, 0.0);
+      for (int l = i; l < j; ++l) {
+        const cplx g = cconj(s_TG[j + PB * l]);       // G(l, j) = conj(G(j, l))
+        acc = cfma(s_TG[i + PB * l], g, acc);
+      }
+      s_TG[i + PB * j] = cmul(make_double2(-s_tau[j].x, -s_tau[j].y), acc);
+    }
+  }
+  __syncthreads();
+  auto Tget = [&](int i, int j) -> cplx {
+    return (i < ncol && j < ncol && i <= j) ? s_TG[i + PB * j] : make_double2(0.0, 0.0);
+  };
+  const int b = k / nb;
+  for (int idx = tid; idx < ncol * ncol; idx += blockDim.x) {
+    const int i = idx % ncol, j = idx / ncol;
+    Tw[(long long)b * nb * nb + i + j * nb] = Tget(i, j);
+  }
+  // Y1 = V T^H, Y2 = V T
+  {
+    const int tr = (rows + 15) / 16, tc = (ncol + 15) / 16;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int tI = wave; tI < tr * tc; tI += PT / 64) {
+      const int r0 = 16 * (tI % tr), c0 = 16 * (tI / tr);
+      v4d_t re1 = {0, 0, 0, 0}, im1 = {0, 0, 0, 0}, re2 = {0, 0, 0, 0}, im2 = {0, 0, 0, 0};
+      const int Kc = (ncol + 3) & ~3;
+      cmfma16(
+          Kc, [&](int i, int kk) { return Vget(r0 + i, kk); },
+          [&](int kk, int jj) { return cconj(Tget(c0 + jj, kk)); }, re1, im1);
+      cmfma16(
+          Kc, [&](int i, int kk) { return Vget(r0 + i, kk); }, [&](int kk, int jj) { return Tget(kk, c0 + jj); },
+          re2, im2);
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int gi = r0 + 4 * r4 + lk, gj = c0 + lr;
+        if (gi < rows && gj < ncol) {
+          Y1w[(k + gi) + (long long)(k + gj) * n] = make_double2(re1[r4], im1[r4]);
+          Y2w[(k + gi) + (long long)(k + gj) * n] = make_double2(re2[r4], im2[r4]);
+        }
+      }
+    }
+  }
+  if (tid == 0) {
+    cplx dd = make_double2(1.0, 0.0);
+    for (int j = 0; j < ncol; ++j) dd = cmul(dd, csub(make_double2(1.0, 0.0), cscale(s_tau[j], s_vn2[j])));
+    pdet[b] = dd;
+  }
+}
+
 __device__ cplx block_cprod(cplx v, cplx* sh) {
   // product over the block of one complex per thread; result valid in all threads
   sh[threadIdx.x] = v;
@@ -375,7 +587,13 @@ inline unsigned nblocks(long long count, int per) { return (unsigned)((count + p
 
 int qr_work_alloc(QRWork& w, int n) {
   w.n = n;
-  w.nb = n >= 512 ? 16 : 32;
+  // panel width: the widest whose rows x nb panel fits the LDS kernel's 128 KiB
+  if (n <= 256) w.nb = 32;
+  else if (n <= 512) w.nb = 16;
+  else if (n <= 1024) w.nb = 8;
+  else if (n <= 2048) w.nb = 4;
+  else w.nb = 32;
+  w.lds_panel = n <= 2048;
   w.nblk = (n + w.nb - 1) / w.nb;
   const size_t nn = (size_t)n * n;
   HIP_OK(hipMalloc(&w.A, nn * sizeof(cplx)));
@@ -427,8 +645,20 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   for (int k = 0; k < n; k += nb) {
-    hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
-                       w.pdet);
+    if (w.lds_panel) {
+      const size_t lds = (size_t)(n - k) * std::min(nb, n - k) * sizeof(cplx);
+      static bool attr = false;
+      if (!attr) {
+        HIP_OK(hipFuncSetAttribute((const void*)panel_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   128 * 1024));
+        attr = true;
+      }
+      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
+                         w.pdet);
+    } else {
+      hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
+                         w.pdet);
+    }
     HIP_OK(hipGetLastError());
     const int kc = k + nb;
     if (kc >= n) break;

@@ -134,6 +134,11 @@ const char* alfq_last_error(void);
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
               const double* beta, double* C, int32_t ldc);
+/* Per-launch timing of the Metropolis chain kernel (start/stop events recorded
+ * with each launch on the context's stream).  Enabling resets the record;
+ * *total_ms is the summed kernel time of the recorded launches. */
+int alfq_kernel_timing(alfq_ctx* ctx, int32_t enable);
+int alfq_kernel_timing_get(alfq_ctx* ctx, double* total_ms, int64_t* launches);
 /* Library build identification. */
 const char* alfq_version(void);
 
