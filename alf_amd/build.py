@@ -25,6 +25,37 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_DEMO = os.path.join(FORTRAN_DIR, "alfq_ring_demo")
+
+
+def amdflang() -> str | None:
+    for p in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang"):
+        if p and os.path.exists(p):
+            return p
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the ISO_C_BINDING module and the Fortran host demo against libalfq.so."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran binding not built")
+        return None
+    cmd = [fc, "-O2", "-J", FORTRAN_DIR, os.path.join(FORTRAN_DIR, "alfq_mod.F90"),
+           os.path.join(FORTRAN_DIR, "alfq_ring_demo.F90"), "-o", FORTRAN_DEMO + ".tmp",
+           "-L" + HERE, "-lalfq", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_DIR)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed building the Fortran binding")
+    os.replace(FORTRAN_DEMO + ".tmp", FORTRAN_DEMO)
+    if verbose:
+        print("built", FORTRAN_DEMO)
+    return FORTRAN_DEMO
+
+
 def build_library(verbose: bool = False) -> str:
     cmd = [hipcc(), *HIPCC_FLAGS, "-o", OUT + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -39,3 +70,4 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_fortran(verbose=True)
