@@ -12,10 +12,12 @@
 //              triangular inverse and three GEMMs for G
 // Everything stays on the device; the host only enqueues work.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/alfq.h"
@@ -200,6 +202,8 @@ struct alfq_ctx {
   long long per_visit_sum = 0;           // uniforms per pass over all ops
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   KTimer chain_timer;
+  int rounds = 1 << 30;                  // chain rounds per slice (adapted per sweep)
+  bool uniforms_ready = false;           // a whole sweep's uniforms are already on the device
 };
 
 namespace {
@@ -289,10 +293,19 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.n_sun = c->n_sun;
   a.u = c->u; a.u_base = c->u_base; a.u_len = c->u_len; a.st = c->st;
   a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
-  hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(1), 0, c->s, c->st);
-  HIP_OK(hipGetLastError());
-  const int rounds = c->n_op / c->K + 1;
+  const int worst = c->n_op / c->K + 1;
+  const bool reg_kernel = chain_max_delay(c->n) >= c->K;
+  int rounds = worst;
+  if (reg_kernel) {
+    // the final round flushes in place if the slice needs more, so any count is correct
+    rounds = std::max(1, std::min(c->rounds, worst));
+  } else {
+    hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(1), 0, c->s, c->st);
+    HIP_OK(hipGetLastError());
+  }
   for (int r = 0; r < rounds; ++r) {
+    a.first_round = (r == 0);
+    a.final_round = (r == rounds - 1);
     if ((st = launch_chain(a, c->s, &c->chain_timer))) return st;
     // G += X Y^T with the rank read from the chain state
     GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
@@ -300,6 +313,21 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
     if ((st = gemm_launch(g, c->s))) return st;
   }
   c->rng_next += c->per_visit_sum;
+  return ST_NONE;
+}
+
+// Choose the number of chain rounds per slice from the largest accepted count seen in
+// the previous sweep (with a margin); reads one integer from the device.
+int adapt_rounds(alfq_ctx* c) {
+  int mx = 0;
+  HIP_OK(hipMemcpyAsync(&mx, &c->st->max_slice_acc, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  const int zero = 0;
+  HIP_OK(hipMemcpyAsync(&c->st->max_slice_acc, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
+  const int worst = c->n_op / c->K + 1;
+  const int want = (int)std::ceil(1.25 * mx / c->K) + 1;
+  c->rounds = std::max(2, std::min(worst, want));
+  HIP_OK(hipStreamSynchronize(c->s));
   return ST_NONE;
 }
 
@@ -320,14 +348,14 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
   g2.cs = c->dv_dinv;
   if ((st = gemm_launch(g2, c->s))) return st;
   std::swap(c->G, c->Gtmp);
-  if ((st = fill_uniforms(c, c->per_visit_sum))) return st;
+  if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   return metropolis_slice(c, slice, +1);
 }
 
 int wrapgrdo_dev(alfq_ctx* c, int ntau) {
   int st;
   const int slice = ntau - 1;
-  if ((st = fill_uniforms(c, c->per_visit_sum))) return st;
+  if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   if ((st = metropolis_slice(c, slice, -1))) return st;
   if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
                         c->dv_dconj, c->s)))
@@ -447,6 +475,8 @@ int cgr_dev(alfq_ctx* c, int nvar, int slot_r, int slot_l, int check) {
   return ST_NONE;
 }
 
+int sweep_body(alfq_ctx* c);
+
 int init_dev(alfq_ctx* c) {
   int st;
   const auto& s = c->stab;
@@ -462,6 +492,17 @@ int init_dev(alfq_ctx* c) {
 }
 
 int sweep_dev(alfq_ctx* c) {
+  int st;
+  const auto& s = c->stab;
+  const int L = c->L;
+  if ((st = fill_uniforms(c, 2LL * L * c->per_visit_sum))) return st;
+  c->uniforms_ready = true;
+  st = sweep_body(c);
+  c->uniforms_ready = false;
+  return st;
+}
+
+int sweep_body(alfq_ctx* c) {
   int st;
   const auto& s = c->stab;
   const int L = c->L;
@@ -970,9 +1011,41 @@ int alfq_init(alfq_ctx* c) {
 int alfq_sweep(alfq_ctx* c, int32_t nsweep) {
   if (!c) return ST_GENERIC;
   int st;
-  for (int i = 0; i < nsweep; ++i)
+  for (int i = 0; i < nsweep; ++i) {
     if ((st = sweep_dev(c))) return st;
+    if ((st = adapt_rounds(c))) return st;
+  }
   return check_status(c);
+}
+
+int alfq_sweep_many(alfq_ctx* const* ctxs, int32_t count, int32_t nsweep, double* ms) {
+  if (!ctxs || count < 1) { alfq_set_error("sweep_many: no contexts"); return ST_GENERIC; }
+  for (int i = 0; i < count; ++i)
+    if (!ctxs[i]) { alfq_set_error("sweep_many: null context"); return ST_GENERIC; }
+  for (int i = 0; i < count; ++i) HIP_OK(hipStreamSynchronize(ctxs[i]->s));
+  std::vector<int> status(count, ST_NONE);
+  std::vector<std::string> errors(count);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto work = [&](int i) {
+    alfq_ctx* c = ctxs[i];
+    int st = ST_NONE;
+    for (int k = 0; k < nsweep && st == ST_NONE; ++k) {
+      st = sweep_dev(c);
+      if (st == ST_NONE) st = adapt_rounds(c);
+    }
+    if (st == ST_NONE) st = check_status(c);
+    status[i] = st;
+    if (st) errors[i] = alfq_last_error();
+  };
+  std::vector<std::thread> th;
+  for (int i = 1; i < count; ++i) th.emplace_back(work, i);
+  work(0);
+  for (auto& t : th) t.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  for (int i = 0; i < count; ++i)
+    if (status[i]) { alfq_set_error("chain " + std::to_string(i) + ": " + errors[i]); return status[i]; }
+  return ST_NONE;
 }
 
 int alfq_sweep_timed(alfq_ctx* c, int32_t nsweep, double* ms) {
@@ -983,6 +1056,7 @@ int alfq_sweep_timed(alfq_ctx* c, int32_t nsweep, double* ms) {
   for (int i = 0; i < nsweep; ++i)
     if ((st = sweep_dev(c))) return st;
   HIP_OK(hipEventRecord(c->ev1, c->s));
+  if ((st = adapt_rounds(c))) return st;
   HIP_OK(hipEventSynchronize(c->ev1));
   float t = 0.f;
   HIP_OK(hipEventElapsedTime(&t, c->ev0, c->ev1));

@@ -17,10 +17,12 @@ struct ChainState {
   int status;                     // 1: random stream exhausted
   int pad;
   unsigned long long rng_slice_base;  // rng_pos at the start of the current slice
+  int slice_acc;                  // accepted flips in the current slice
+  int max_slice_acc;              // largest slice_acc seen since the host last cleared it
 };
 
 struct ChainArgs {
-  const cplx* G; int n;
+  cplx* G; int n;
   cplx* X; cplx* Y; int Kmax;
   int8_t* f; int n_op; int slice; int dir;
   const int* op_site; const int* op_type;
@@ -30,6 +32,8 @@ struct ChainArgs {
   int n_sun;
   const double* u; unsigned long long u_base; long long u_len;
   const int* uoff;                // uniforms consumed before visit v of the slice (n_op + 1 entries)
+  int first_round;                // start of the slice: visit 0, rng base = current position
+  int final_round;                // last scheduled round: flush in place instead of stopping at K
   ChainState* st;
 };
 

@@ -18,6 +18,8 @@
 // fused into the epilogue of the adjacent hopping GEMM.
 #include <hip/hip_ext.h>
 
+#include <mutex>
+
 #include "alfq_chain.h"
 
 namespace {
@@ -218,165 +220,223 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __shared__ cplx s_x[2];
   __shared__ int s_acc[2];
   __shared__ cplx s_phase;
+  __shared__ double s_iph;                              // 1 / Re(PHASE)
   __shared__ long long s_nc, s_accn;
-  __shared__ int s_stop_at;
+  __shared__ int s_stop_at, s_pos0;
+  __shared__ unsigned long long s_base;
 
   const int tid = threadIdx.x;
   const int n = a.n;
   ChainState* st = a.st;
-  const int pos0 = st->pos;
-  const unsigned long long base = st->rng_slice_base;
-  // ---- prologue: per-visit proposal, tables, uniforms and stale G_PP ----
+  long long acc_at_start = 0;
   if (tid == 0) {
+    s_pos0 = a.first_round ? 0 : st->pos;
+    s_base = a.first_round ? st->rng_pos : st->rng_slice_base;
     s_phase = st->phase;
+    s_iph = 1.0 / st->phase.x;
     s_nc = st->nc;
     s_accn = st->acc;
-    s_stop_at = a.n_op;
-  }
-  for (int v = pos0 + tid; v < a.n_op; v += blockDim.x) {
-    Visit w;
-    const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
-    const int typ = a.op_type[nop];
-    const int s_old = a.f[(long long)a.slice * a.n_op + nop];
-    const unsigned long long q = base + (unsigned long long)a.uoff[v];
-    const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
-    w.ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
-    int s_new = -s_old;
-    double u3 = 2.0;
-    if (w.ok) {
-      const double* u = a.u + (q - a.u_base);
-      if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
-    }
-    const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
-    const int P = a.op_site[nop];
-    w.myexp = a.op_myexp[ti];
-    w.expal = a.op_expalpha[ti];
-    w.gr = a.gama_ratio[ti];
-    w.u3 = u3;
-    w.P = P;
-    w.snew = s_new;
-    w.nop = nop;
-    w.gpp = a.G[P + (long long)P * n];
-    vis[v] = w;
+    acc_at_start = st->acc;
+    if (a.first_round) st->slice_acc = 0;
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int v = pos0; v < a.n_op; ++v)
-      if (!vis[v].ok) { s_stop_at = v; break; }
-  }
+  const unsigned long long base = s_base;
+  int pos = s_pos0, m = 0;
+  const int kmax = a.Kmax < KT ? a.Kmax : KT;
 
-  cplx Xr[RPT][KT], Yr[RPT][KT];
+  cplx Xr[RPT][KT], Yr[RPT][KT], dcorr[RPT];
 #pragma unroll
-  for (int rr = 0; rr < RPT; ++rr)
+  for (int rr = 0; rr < RPT; ++rr) {
+    dcorr[rr] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int j = 0; j < KT; ++j) { Xr[rr][j] = make_double2(0.0, 0.0); Yr[rr][j] = make_double2(0.0, 0.0); }
-
+  }
   // column and row P of the stale G for the next PD visits, one ring slot per visit
   constexpr int PD = (RPT == 1) ? 4 : 2;
   cplx rc[PD][RPT], rw[PD][RPT];
-  __syncthreads();
-  const int stop_at = s_stop_at;
-  int pos = pos0, m = 0;
+  int stop_at = a.n_op;
+  int buf = 0;
+  bool prologue = true;
+  while (true) {
+    if (prologue) {
+      // ---- per-visit proposal, tables, uniforms and stale G_PP for visits pos.. ----
+      for (int v = pos + tid; v < a.n_op; v += blockDim.x) {
+        Visit w;
+        const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
+        const int typ = a.op_type[nop];
+        const int s_old = a.f[(long long)a.slice * a.n_op + nop];
+        const unsigned long long q = base + (unsigned long long)a.uoff[v];
+        const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
+        w.ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
+        int s_new = -s_old;
+        double u3 = 2.0;
+        if (w.ok) {
+          const double* u = a.u + (q - a.u_base);
+          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
+        }
+        const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
+        const int P = a.op_site[nop];
+        w.myexp = a.op_myexp[ti];
+        w.expal = a.op_expalpha[ti];
+        w.gr = a.gama_ratio[ti];
+        w.u3 = u3;
+        w.P = P;
+        w.snew = s_new;
+        w.nop = nop;
+        w.gpp = a.G[P + (long long)P * n];
+        vis[v] = w;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_stop_at = a.n_op;
+        for (int v = pos; v < a.n_op; ++v)
+          if (!vis[v].ok) { s_stop_at = v; break; }
+      }
+      __syncthreads();
+      stop_at = s_stop_at;
+      if (pos < stop_at) {
 #pragma unroll
-  for (int S = 0; S < PD; ++S) {
-    const int v = pos0 + S;
-    const int P = v < stop_at ? vis[v].P : 0;
+        for (int S = 0; S < PD; ++S) {
+          const int v = pos + S < stop_at ? pos + S : stop_at - 1;
+          const int P = vis[v].P;
+#pragma unroll
+          for (int rr = 0; rr < RPT; ++rr) {
+            const int r = tid + 256 * rr;
+            const int rcl = r < n ? r : n - 1;
+            rc[S][rr] = a.G[rcl + (long long)P * n];
+            rw[S][rr] = a.G[P + (long long)rcl * n];
+          }
+        }
+      }
+      prologue = false;
+    }
+    // One visit: the owner of row P decides; on acceptance every thread extends its rows
+    // of X, Y (and the running diagonal correction) using column/row P of the stale G
+    // held in COLV/ROWV.
+#define ALFQ_CHAIN_VISIT(COLV, ROWV)                                                            \
+    {                                                                                         \
+      const Visit w = vis[pos];                                                               \
+      const int P = w.P;                                                                      \
+      const int owner = P & 255, orr = P >> 8;                                                \
+      if (tid == owner) {                                                                     \
+        cplx d = w.gpp;                                                                       \
+        _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) if (rr == orr) d = cadd(d, dcorr[rr]); \
+        const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);                          \
+        const cplx dmat = csub(w.myexp, cmul(delta, d));                                      \
+        const cplx ratio = cmul(dmat, w.expal);                                               \
+        cplx rt = ratio;                                                                      \
+        for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);                               \
+        rt = cscale(rt, w.gr);                                                                \
+        const cplx ph = s_phase;                                                              \
+        const double weight = fabs(cmul(ph, rt).x * s_iph);                                   \
+        s_nc += 1;                                                                            \
+        int acc = 0;                                                                          \
+        if (weight > w.u3) {                                                                  \
+          acc = 1;                                                                            \
+          s_accn += 1;                                                                        \
+          const cplx nph = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));                       \
+          s_phase = nph;                                                                      \
+          s_iph = 1.0 / nph.x;                                                                \
+          a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;                          \
+          const cplx vp = make_double2(1.0 - d.x, -d.y);                                      \
+          s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));              \
+          _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) {                                \
+            if (rr == orr) {                                                                  \
+              _Pragma("unroll") for (int j = 0; j < KT; ++j) {                                \
+                if (j < m) { sX[buf][j] = Xr[rr][j]; sY[buf][j] = Yr[rr][j]; }                \
+              }                                                                               \
+            }                                                                                 \
+          }                                                                                   \
+        }                                                                                     \
+        s_acc[buf] = acc;                                                                     \
+      }                                                                                       \
+      block_sync_lds();                                                                       \
+      if (s_acc[buf]) {                                                                       \
+        const cplx x = s_x[buf];                                                              \
+        const cplx mx = make_double2(-x.x, -x.y);                                             \
+        _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) {                                  \
+          const int r = tid + 256 * rr;                                                       \
+          cplx col = COLV[rr], row = ROWV[rr];                                                \
+          _Pragma("unroll") for (int j = 0; j < KT; ++j) {                                    \
+            if (j < m) {                                                                      \
+              col = cfma(Xr[rr][j], sY[buf][j], col);                                         \
+              row = cfma(sX[buf][j], Yr[rr][j], row);                                         \
+            }                                                                                 \
+          }                                                                                   \
+          const cplx xn = cmul(mx, col);                                                      \
+          const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);                 \
+          dcorr[rr] = cfma(xn, yn, dcorr[rr]);                                                \
+          _Pragma("unroll") for (int j = 0; j < KT; ++j)                                      \
+            if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }                                   \
+        }                                                                                     \
+        m = __builtin_amdgcn_readfirstlane(m + 1);                                            \
+      }                                                                                       \
+      pos = __builtin_amdgcn_readfirstlane(pos + 1);                                          \
+      buf ^= 1;                                                                               \
+    }
+
+    // fast path: PD visits per iteration with no per-visit guard, so the refills of the
+    // ring stay in flight across iterations; taken while PD more visits and accepts fit
+    while (pos + PD <= stop_at && m + PD <= kmax) {
+#pragma unroll
+      for (int S = 0; S < PD; ++S) {
+        ALFQ_CHAIN_VISIT(rc[S], rw[S])
+        {
+          const int v = pos - 1 + PD < stop_at ? pos - 1 + PD : stop_at - 1;
+          const int Pn = vis[v].P;
+#pragma unroll
+          for (int rr = 0; rr < RPT; ++rr) {
+            const int rcl = (tid + 256 * rr) < n ? tid + 256 * rr : n - 1;
+            rc[S][rr] = a.G[rcl + (long long)Pn * n];
+            rw[S][rr] = a.G[Pn + (long long)rcl * n];
+          }
+        }
+      }
+    }
+    // slow path for the last few visits or accepts of the round: direct loads
+    while (pos < stop_at && m < kmax) {
+      cplx dc[RPT], dr[RPT];
+      const int P = vis[pos].P;
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        const int rcl = (tid + 256 * rr) < n ? tid + 256 * rr : n - 1;
+        dc[rr] = a.G[rcl + (long long)P * n];
+        dr[rr] = a.G[P + (long long)rcl * n];
+      }
+      ALFQ_CHAIN_VISIT(dc, dr)
+    }
+#undef ALFQ_CHAIN_VISIT
+    if (pos >= stop_at || !a.final_round) break;
+    // ---- last round of the slice and K accepts pending: flush G += X Y^T in place ----
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
       const int r = tid + 256 * rr;
-      const bool ok = v < stop_at && r < n;
-      rc[S][rr] = ok ? a.G[r + (long long)P * n] : make_double2(0.0, 0.0);
-      rw[S][rr] = ok ? a.G[P + (long long)r * n] : make_double2(0.0, 0.0);
-    }
-  }
-  const int kmax = a.Kmax < KT ? a.Kmax : KT;
-  int buf = 0;
-  while (pos < stop_at && m < kmax) {
+      if (r < n) {
 #pragma unroll
-    for (int S = 0; S < PD; ++S) {
-      if (pos < stop_at && m < kmax) {
-        const Visit w = vis[pos];
-        const int P = w.P;
-        // owner of row P decides
-        const int owner = P & 255, orr = P >> 8;
-        if (tid == owner) {
-          cplx d = w.gpp;
-#pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            if (rr == orr) {
-#pragma unroll
-              for (int j = 0; j < KT; ++j) {
-                if (j < m) {
-                  d = cfma(Xr[rr][j], Yr[rr][j], d);
-                  sX[buf][j] = Xr[rr][j];
-                  sY[buf][j] = Yr[rr][j];
-                }
-              }
-            }
-          }
-          const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
-          const cplx dmat = csub(w.myexp, cmul(delta, d));
-          const cplx ratio = cmul(dmat, w.expal);
-          cplx rt = ratio;
-          for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);
-          rt = cscale(rt, w.gr);
-          const cplx ph = s_phase;
-          const double weight = fabs(cmul(ph, rt).x / ph.x);
-          s_nc += 1;
-          int acc = 0;
-          if (weight > w.u3) {
-            acc = 1;
-            s_accn += 1;
-            s_phase = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
-            a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
-            const cplx vp = make_double2(1.0 - d.x, -d.y);
-            s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
-          }
-          s_acc[buf] = acc;
-        }
-        block_sync_lds();
-        if (s_acc[buf]) {
-          const cplx x = s_x[buf];
-          const cplx mx = make_double2(-x.x, -x.y);
-#pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            const int r = tid + 256 * rr;
-            cplx col = rc[S][rr], row = rw[S][rr];
-#pragma unroll
-            for (int j = 0; j < KT; ++j) {
-              if (j < m) {
-                col = cfma(Xr[rr][j], sY[buf][j], col);
-                row = cfma(sX[buf][j], Yr[rr][j], row);
-              }
-            }
-            const cplx xn = cmul(mx, col);
-            const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);
-#pragma unroll
-            for (int j = 0; j < KT; ++j) {
-              if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }
-            }
-          }
-          m = __builtin_amdgcn_readfirstlane(m + 1);
-        }
-        // refill this slot with the visit PD ahead
-        {
-          const int v = pos + PD;
-          const int Pn = v < stop_at ? vis[v].P : 0;
-#pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            const int r = tid + 256 * rr;
-            if (v < stop_at && r < n) {
-              rc[S][rr] = a.G[r + (long long)Pn * n];
-              rw[S][rr] = a.G[Pn + (long long)r * n];
-            }
-          }
-        }
-        pos = __builtin_amdgcn_readfirstlane(pos + 1);
-        buf ^= 1;
+        for (int j = 0; j < KT; ++j)
+          if (j < m) a.Y[r + (long long)j * n] = Yr[rr][j];
       }
     }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      const int r = tid + 256 * rr;
+      if (r < n) {
+        for (int c = 0; c < n; ++c) {
+          cplx acc = a.G[r + (long long)c * n];
+#pragma unroll
+          for (int j = 0; j < KT; ++j)
+            if (j < m) acc = cfma(Xr[rr][j], a.Y[c + (long long)j * n], acc);
+          a.G[r + (long long)c * n] = acc;
+        }
+        dcorr[rr] = make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    m = 0;
+    prologue = true;
   }
-  // ---- hand the pending columns to the flush and save the chain state ----
+  // ---- hand the pending columns to the flush GEMM and save the chain state ----
 #pragma unroll
   for (int rr = 0; rr < RPT; ++rr) {
     const int r = tid + 256 * rr;
@@ -397,11 +457,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     st->acc = s_accn;
     st->pos = pos;
     st->m = m;
+    st->rng_slice_base = base;
     st->rng_pos = base + (unsigned long long)a.uoff[pos];
+    st->slice_acc += (int)(s_accn - acc_at_start);
     if (pos == stop_at && stop_at < a.n_op) {
       st->status = 1;
       st->pos = a.n_op;    // do not revisit
     }
+    if (st->pos >= a.n_op && st->slice_acc > st->max_slice_acc) st->max_slice_acc = st->slice_acc;
   }
 }
 
@@ -468,12 +531,13 @@ int chain_max_delay(int n) {
 template <int RPT, int KT>
 static int launch_chain2(const ChainArgs& a, hipStream_t s, KTimer* t) {
   const size_t lds = sizeof(Visit) * (size_t)(a.n_op > 0 ? a.n_op : 1);
-  static bool attr_set = false;
-  if (lds > 64 * 1024 && !attr_set) {
-    HIP_OK(hipFuncSetAttribute((const void*)chain2_kernel<RPT, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               96 * 1024));
-    attr_set = true;
-  }
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [] {
+    attr_err = hipFuncSetAttribute((const void*)chain2_kernel<RPT, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   96 * 1024);
+  });
+  HIP_OK(attr_err);
   if (lds > 96 * 1024) { alfq_set_error("launch_chain: too many operators per slice for the LDS tables"); return ST_GENERIC; }
   if (t && t->on) {
     if (t->used == t->ev.size()) {

@@ -55,8 +55,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   v4d acc_re = {0, 0, 0, 0};
   v4d acc_im = {0, 0, 0, 0};
 
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // ---- stage opA tile (BM x BK) ----
+  // global -> register loads of one K-tile (2 opA and 2 opB elements per thread)
+  double ra_re[2], ra_im[2], rb_re[2], rb_im[2];
+  auto load_tile = [&](int k0) {
     if (g.opA == OP_N) {
       const int i = tid & 31;
 #pragma unroll
@@ -69,8 +70,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
           const long long off = gi + (long long)col * g.lda;
           if (A_CPLX) { cplx v = Ac[off]; re = v.x; im = v.y; } else { re = Ar[off]; }
         }
-        As_re[kk][i] = re;
-        if (A_CPLX) As_im[kk][i] = im;
+        ra_re[p] = re;
+        ra_im[p] = im;
       }
     } else {
       const int kk = tid & 15;
@@ -84,11 +85,10 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
           const long long off = row + (long long)gi * g.lda;
           if (A_CPLX) { cplx v = Ac[off]; re = v.x; im = (g.opA == OP_C) ? -v.y : v.y; } else { re = Ar[off]; }
         }
-        As_re[kk][i] = re;
-        if (A_CPLX) As_im[kk][i] = im;
+        ra_re[p] = re;
+        ra_im[p] = im;
       }
     }
-    // ---- stage opB tile (BK x BN) ----
     if (g.opB == OP_N) {
       const int kk = tid & 15;
 #pragma unroll
@@ -100,10 +100,10 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
           const int row = g.bmap ? g.bmap[gk] : gk;
           const long long off = row + (long long)gj * g.ldb;
           if (g.b_cplx) { cplx v = Bc[off]; re = v.x; im = v.y; } else { re = Br[off]; }
-          if (g.bs) { cplx s = g.bs[gk]; double r2 = re * s.x - im * s.y; im = re * s.y + im * s.x; re = r2; }
+          if (g.bs) { cplx sc = g.bs[gk]; double r2 = re * sc.x - im * sc.y; im = re * sc.y + im * sc.x; re = r2; }
         }
-        Bs_re[kk][j] = re;
-        if (B_CPLX) Bs_im[kk][j] = im;
+        rb_re[p] = re;
+        rb_im[p] = im;
       }
     } else {
       const int j = tid & 31;
@@ -116,13 +116,55 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
           const int col = g.bmap ? g.bmap[gk] : gk;
           const long long off = gj + (long long)col * g.ldb;
           if (g.b_cplx) { cplx v = Bc[off]; re = v.x; im = (g.opB == OP_C) ? -v.y : v.y; } else { re = Br[off]; }
-          if (g.bs) { cplx s = g.bs[gk]; double r2 = re * s.x - im * s.y; im = re * s.y + im * s.x; re = r2; }
+          if (g.bs) { cplx sc = g.bs[gk]; double r2 = re * sc.x - im * sc.y; im = re * sc.y + im * sc.x; re = r2; }
         }
-        Bs_re[kk][j] = re;
-        if (B_CPLX) Bs_im[kk][j] = im;
+        rb_re[p] = re;
+        rb_im[p] = im;
       }
     }
+  };
+  auto store_tile = [&]() {
+    if (g.opA == OP_N) {
+      const int i = tid & 31;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int kk = (tid >> 5) + 8 * p;
+        As_re[kk][i] = ra_re[p];
+        if (A_CPLX) As_im[kk][i] = ra_im[p];
+      }
+    } else {
+      const int kk = tid & 15;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int i = (tid >> 4) + 16 * p;
+        As_re[kk][i] = ra_re[p];
+        if (A_CPLX) As_im[kk][i] = ra_im[p];
+      }
+    }
+    if (g.opB == OP_N) {
+      const int kk = tid & 15;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int j = (tid >> 4) + 16 * p;
+        Bs_re[kk][j] = rb_re[p];
+        if (B_CPLX) Bs_im[kk][j] = rb_im[p];
+      }
+    } else {
+      const int j = tid & 31;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int kk = (tid >> 5) + 8 * p;
+        Bs_re[kk][j] = rb_re[p];
+        if (B_CPLX) Bs_im[kk][j] = rb_im[p];
+      }
+    }
+  };
+
+  if (K > 0) load_tile(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store_tile();
     __syncthreads();
+    if (k0 + BK < K) load_tile(k0 + BK);   // next tile's loads overlap this tile's MFMAs
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const double a_re = As_re[kk + lk][wr + lr];

@@ -14,6 +14,7 @@
 // and forms the compact-WY factor T (zlarft, forward columnwise); the trailing
 // matrix and the explicit Q are updated with the MFMA GEMM.
 #include <algorithm>
+#include <mutex>
 
 #include "alfq_qr.h"
 
@@ -265,9 +266,11 @@ __device__ __forceinline__ void cmfma16(int K, FA A, FB B, v4d_t& re, v4d_t& im)
 }
 
 constexpr int PB = 32;          // widest LDS panel
-constexpr int QPW = PB / 4;     // panel columns per wave
+constexpr int PLT = 1024;       // threads of the LDS panel kernel (16 waves)
+constexpr int PLW = PLT / 64;   // its waves
+constexpr int QPW = PB / PLW;   // panel columns per wave
 
-__global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int n, int k, int nb,
+__global__ __launch_bounds__(PLT) void panel_lds_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                        cplx* __restrict__ tau, cplx* __restrict__ Vw,
                                                        cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
                                                        cplx* __restrict__ Tw, cplx* __restrict__ pdet) {
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int
   __shared__ cplx s_TG[PB * PB];               // upper incl. diagonal: T; strict lower: Gram
   __shared__ cplx s_tau[PB], s_sc[PB];
   __shared__ double s_beta[PB], s_norm[PB + 1], s_vn2[PB];
-  __shared__ double red[PT / 64];
+  __shared__ double red[PLW];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int rows = n - k;
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int
       const cplx xr = xj[r];
 #pragma unroll
       for (int i = 0; i < QPW; ++i) {
-        const int q = j + 1 + wave + 4 * i;
+        const int q = j + 1 + wave + PLW * i;
         if (q < ncol) {
           const cplx a = sp[r + (long long)q * rows];
           d[i].x += xr.x * a.x + xr.y * a.y;
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int
     const cplx tc = cconj(t), scc = cconj(sc);
 #pragma unroll
     for (int i = 0; i < QPW; ++i) {
-      const int q = j + 1 + wave + 4 * i;
+      const int q = j + 1 + wave + PLW * i;
       if (q < ncol) {
         cplx* aq = sp + (long long)q * rows;
         const cplx w = cadd(aq[j], cmul(scc, d[i]));
@@ -425,7 +428,7 @@ __global__ __launch_bounds__(PT) void panel_lds_kernel(cplx* __restrict__ A, int
   {
     const int tr = (rows + 15) / 16, tc = (ncol + 15) / 16;
     const int lr = lane & 15, lk = lane >> 4;
-    for (int tI = wave; tI < tr * tc; tI += PT / 64) {
+    for (int tI = wave; tI < tr * tc; tI += PLW) {
       const int r0 = 16 * (tI % tr), c0 = 16 * (tI / tr);
       v4d_t re1 = {0, 0, 0, 0}, im1 = {0, 0, 0, 0}, re2 = {0, 0, 0, 0}, im2 = {0, 0, 0, 0};
       const int Kc = (ncol + 3) & ~3;
@@ -480,8 +483,6 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
                                                        cplx* __restrict__ phase_out,
                                                        cplx* __restrict__ detU_out) {
   __shared__ cplx sh[PT];
-  __shared__ int s_parity;
-  __shared__ cplx s_phase;
   const int tid = threadIdx.x;
   // product of normalised diagonal entries
   cplx pd = make_double2(1.0, 0.0);
@@ -515,7 +516,6 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
       }
       if ((L & 1) == 0) sgn = -sgn;
     }
-    s_parity = sgn;
     cplx ph;
     if (mode <= 1) {
       ph = cscale(pd, (double)sgn);
@@ -531,19 +531,22 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
       ph = cmul(cmul(drhs, cscale(pd, (double)sgn)), qph);
       *phase_out = ph;
     }
-    s_phase = ph;
   }
-  __syncthreads();
-  const cplx rowscale0 = (mode <= 1) ? cinv(s_phase) : make_double2(1.0, 0.0);
-  for (long long idx = tid; idx < (long long)n * n; idx += blockDim.x) {
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    cplx v = make_double2(0.0, 0.0);
-    if (i <= j) {
-      v = cscale(A[idx], 1.0 / cabs_(A[i + (long long)i * n]));
-      if (i == 0) v = cmul(v, rowscale0);
-    }
-    Rt[idx] = v;
+}
+
+// Rt = R with row i divided by |R_ii| (zero below the diagonal); for the UDV decompose
+// (mode 0/1) row 0 is further divided by the decompose phase.
+__global__ void qr_rt_kernel(const cplx* __restrict__ A, int n, int mode, const cplx* __restrict__ phase,
+                             cplx* __restrict__ Rt) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  cplx v = make_double2(0.0, 0.0);
+  if (i <= j) {
+    v = cscale(A[idx], 1.0 / cabs_(A[i + (long long)i * n]));
+    if (i == 0 && mode <= 1) v = cmul(v, cinv(*phase));
   }
+  Rt[idx] = v;
 }
 
 // X(:, 0) *= s
@@ -647,13 +650,14 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
   for (int k = 0; k < n; k += nb) {
     if (w.lds_panel) {
       const size_t lds = (size_t)(n - k) * std::min(nb, n - k) * sizeof(cplx);
-      static bool attr = false;
-      if (!attr) {
-        HIP_OK(hipFuncSetAttribute((const void*)panel_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   128 * 1024));
-        attr = true;
-      }
-      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
+      static std::once_flag once;
+      static hipError_t attr_err = hipSuccess;
+      std::call_once(once, [] {
+        attr_err = hipFuncSetAttribute((const void*)panel_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       128 * 1024);
+      });
+      HIP_OK(attr_err);
+      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PLT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
                          w.pdet);
     } else {
       hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
@@ -704,6 +708,8 @@ int udv_decompose_dev(QRWork& w, const cplx* Usrc, const cplx* Dsrc, const cplx*
   if ((st = qr_factor(w, Usrc, n, Dsrc, 0, s))) return st;
   hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet, side_l ? 1 : 0,
                      (const cplx*)nullptr, (const cplx*)nullptr, Dout, w.Rt, phase_out, detU_out);
+  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, w.A, n, side_l ? 1 : 0,
+                     (const cplx*)phase_out, w.Rt);
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   if (!side_l) {
@@ -730,6 +736,8 @@ int qr_factor_cgr(QRWork& w, const cplx* TPUP, int nvar, const cplx* detUr, cons
   if ((st = qr_factor(w, TPUP, n, nullptr, nvar == 1 ? 0 : 1, s))) return st;
   hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet,
                      nvar == 1 ? 2 : 3, detUr, detUl, DUP, w.Rt, z_out, (cplx*)nullptr);
+  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, w.A, n, nvar == 1 ? 2 : 3,
+                     (const cplx*)z_out, w.Rt);
   HIP_OK(hipGetLastError());
   if ((st = qr_form_q(w, Qout, s))) return st;
   return ST_NONE;

@@ -28,7 +28,7 @@ EXPORTS = [
     "alfq_phase_set", "alfq_phase_get", "alfq_udv_set", "alfq_udv_get", "alfq_udv_reset", "alfq_udv_assign",
     "alfq_rng_set_tape", "alfq_rng_set_philox", "alfq_rng_position",
     "alfq_wrapgrup", "alfq_wrapgrdo", "alfq_wrapur", "alfq_wrapul", "alfq_cgr",
-    "alfq_init", "alfq_sweep", "alfq_sweep_timed",
+    "alfq_init", "alfq_sweep", "alfq_sweep_timed", "alfq_sweep_many",
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
 ]
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_init": [P],
         "alfq_sweep": [P, C.c_int32],
         "alfq_sweep_timed": [P, C.c_int32, dp],
+        "alfq_sweep_many": [C.POINTER(P), C.c_int32, C.c_int32, dp],
         "alfq_sync": [P],
         "alfq_control_get": [P, C.POINTER(Control)],
         "alfq_control_reset": [P],
@@ -324,3 +325,12 @@ def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None):
     check(lib, lib.alfq_gemm(M, N, K, opa.encode(), int(a_real), _dptr(Aa), max(Aa.shape[0], 1), opb.encode(),
                              int(b_real), _dptr(Ba), max(Ba.shape[0], 1), _dptr(al), _dptr(be), _dptr(Cc), max(M, 1)))
     return Cc
+
+
+def sweep_many(engines, nsweep=1):
+    """Advance several independent chains concurrently (alfq_sweep_many); returns wall ms."""
+    lib = load_library()
+    arr = (C.c_void_p * len(engines))(*[e.ctx for e in engines])
+    ms = C.c_double()
+    check(lib, lib.alfq_sweep_many(arr, len(engines), nsweep, C.byref(ms)))
+    return ms.value

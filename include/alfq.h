@@ -121,6 +121,13 @@ int alfq_sweep(alfq_ctx* ctx, int32_t nsweep);
 /* Like alfq_sweep, timed with HIP events on the context's stream; *ms = elapsed. */
 int alfq_sweep_timed(alfq_ctx* ctx, int32_t nsweep, double* ms);
 
+/* nsweep sweeps of each of count independent chains (contexts), advanced
+ * concurrently: one host thread per context enqueues on the context's own
+ * stream, so the device overlaps the chains.  *ms (may be NULL) is the wall time.
+ * Walker parallelism of the reference (one chain per MPI rank,
+ * Documentation/performance.tex:29-40) inside one GPU. */
+int alfq_sweep_many(alfq_ctx* const* ctxs, int32_t count, int32_t nsweep, double* ms);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 int alfq_sync(alfq_ctx* ctx);
 int alfq_control_get(alfq_ctx* ctx, alfq_control* out);

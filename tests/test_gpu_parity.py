@@ -232,3 +232,33 @@ def test_real_size_single_sweep_matches_oracle(rt):
     assert rel(e.get_green(), s.G) < TOL
     assert abs(e.get_phase() - s.phase) < TOL
     e.close()
+
+
+def test_concurrent_chains_equal_their_solo_runs(rt):
+    """Chains advanced together (alfq_sweep_many, one stream each) give exactly the
+    trajectory each gives alone."""
+    hams = [hubbard(L1=4, L2=4, beta=2.0, nwrap=10), hubbard(L1=6, L2=6, beta=1.0, nwrap=5),
+            hubbard(L1=4, L2=4, beta=3.0, nwrap=10, U=2.0)]
+    fs = [fields_for(h, 10 + i) for i, h in enumerate(hams)]
+    solo = []
+    for i, (h, f) in enumerate(zip(hams, fs)):
+        e = rt.Engine(h, delay=8)
+        e.set_fields(f)
+        e.set_philox(100 + i)
+        e.init()
+        e.sweep(2)
+        solo.append((e.get_fields(), e.get_green(), e.get_phase()))
+        e.close()
+    engs = []
+    for i, (h, f) in enumerate(zip(hams, fs)):
+        e = rt.Engine(h, delay=8)
+        e.set_fields(f)
+        e.set_philox(100 + i)
+        e.init()
+        engs.append(e)
+    rt.sweep_many(engs, 2)
+    for e, (fl, G, ph) in zip(engs, solo):
+        assert np.array_equal(e.get_fields(), fl)
+        assert np.array_equal(e.get_green(), G)
+        assert e.get_phase() == ph
+        e.close()
