@@ -46,7 +46,7 @@ struct KTimer {
 };
 
 int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* timer = nullptr);
-// largest delay depth the register-resident chain kernel supports for n sites (0: generic kernel)
+// largest delay depth the submatrix block kernel supports (deeper delays use the generic kernel)
 int chain_max_delay(int n);
 int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,
                        unsigned stream, hipStream_t s);

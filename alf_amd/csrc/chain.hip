@@ -191,19 +191,26 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// Register-resident chain kernel.  Thread t owns rows t, t + 256, ... (RPT rows) and
-// keeps X(r, 0:KT), Y(r, 0:KT) of its rows in VGPRs.  Everything about a visit that
-// does not depend on earlier decisions -- the proposal (the field of an operator
-// changes only at its own visit), the table entries, the uniforms, the stale G_PP --
-// is precomputed into LDS in a parallel prologue.  The sequential part per visit:
-// the owner of row P forms G_cur(P,P) from its registers, decides, and publishes
-// (accept, x, X(P,:), Y(P,:)) in a double-buffered LDS slot; one barrier; on accept
-// every thread extends its rows of X and Y.  Column and row P of the stale G are
-// prefetched one visit ahead.
-struct Visit {
-  cplx myexp, expal, gpp;
-  double gr, u3;
-  int P, snew, nop, ok;
+// Submatrix block kernel.  The visits of a round are taken in blocks of up to SB_V
+// consecutive visits.  For the sites S of a block the kernel holds g = G_cur[S, S] in
+// LDS and decides each visit from g alone (G_cur(P,P) = g(p,p)); every thread computes
+// the decision itself, so a rejected proposal needs no barrier and no global load.
+// An accepted flip is the rank-1 update of upgrade_mod.F90:266-269 restricted to S,
+//   g += u w^T,  u = -x g(:, p),  w = e_p - g(p, :)^T,
+// and u, w are the restrictions to S of the delayed-update columns X(:, j), Y(:, j).
+// At the end of the block the full columns follow from the Green's function at the
+// block start, G_bs = G + X(:, 0:m0) Y(:, 0:m0)^T, one row per thread:
+//   X(:, j) = -x_j [G_bs(:, P_j) + sum_{i<j in block} X(:, i) w_i(p_j)],
+//   Y(:, j) = e_{P_j} - [G_bs(P_j, :)^T + sum_{i<j in block} Y(:, i) u_i(p_j)],
+// and the next block starts from g = G[S', S'] + X[S', :] Y[S', :]^T.
+constexpr int SB_V = 32;   // visits per block: g is SB_V x SB_V
+constexpr int SB_A = 16;   // accepted flips per block
+constexpr int SB_K = 32;   // accepted flips per round (delayed-update depth)
+
+struct BVisit {
+  cplx myexp, expal;        // exp(Z1 E) and exp(Z1 alpha) of the proposal
+  double gr, u3;            // Gama(new)/Gama(old), Metropolis uniform
+  int P, snew, nop, loc;    // site, proposed field, operator, index of the site in the block
 };
 
 __device__ __forceinline__ void block_sync_lds() {
@@ -212,255 +219,242 @@ __device__ __forceinline__ void block_sync_lds() {
   __asm__ volatile("" ::: "memory");
 }
 
-template <int RPT, int KT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void chain2_kernel(ChainArgs a) {
-  extern __shared__ char smem_raw[];
-  Visit* vis = (Visit*)smem_raw;                        // n_op entries
-  __shared__ cplx sX[2][KT], sY[2][KT];
-  __shared__ cplx s_x[2];
-  __shared__ int s_acc[2];
-  __shared__ cplx s_phase;
-  __shared__ double s_iph;                              // 1 / Re(PHASE)
-  __shared__ long long s_nc, s_accn;
-  __shared__ int s_stop_at, s_pos0;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void chain_block_kernel(ChainArgs a) {
+  static_assert(SB_V == 32 && SB_A <= SB_V, "block layout assumes 32 visits and 8 column groups");
+  __shared__ cplx gs[2][SB_V * SB_V];           // g(r, c) at [r + SB_V c], double-buffered
+  __shared__ cplx Xb[SB_K][SB_V], Yb[SB_K][SB_V]; // X(P_r, i), Y(P_r, i) of the round's earlier flips
+  __shared__ cplx Us[SB_A][SB_V], Ws[SB_A][SB_V]; // u_j(r), w_j(r) of the block's flips
+  __shared__ BVisit vb[SB_V];
+  __shared__ cplx s_xj[SB_A];
+  __shared__ int s_aloc[SB_A], s_aP[SB_A];
+  __shared__ int s_nvis, s_pos0;
   __shared__ unsigned long long s_base;
+  __shared__ cplx s_phase;
+  __shared__ long long s_nc, s_accn;
 
   const int tid = threadIdx.x;
   const int n = a.n;
+  const int ga = tid & (SB_V - 1), gq = tid >> 5;   // g entries (ga, gq + 8 q), q = 0..3
   ChainState* st = a.st;
-  long long acc_at_start = 0;
   if (tid == 0) {
     s_pos0 = a.first_round ? 0 : st->pos;
     s_base = a.first_round ? st->rng_pos : st->rng_slice_base;
     s_phase = st->phase;
-    s_iph = 1.0 / st->phase.x;
     s_nc = st->nc;
     s_accn = st->acc;
-    acc_at_start = st->acc;
     if (a.first_round) st->slice_acc = 0;
   }
   __syncthreads();
   const unsigned long long base = s_base;
   int pos = s_pos0, m = 0;
-  const int kmax = a.Kmax < KT ? a.Kmax : KT;
+  cplx ph = s_phase;
+  double iph = 1.0 / ph.x;
+  long long nc = s_nc, accn = s_accn;
+  const long long acc_at_start = accn;
+  const int kmax = a.Kmax < SB_K ? a.Kmax : SB_K;
+  bool stopped = false;
 
-  cplx Xr[RPT][KT], Yr[RPT][KT], dcorr[RPT];
-#pragma unroll
-  for (int rr = 0; rr < RPT; ++rr) {
-    dcorr[rr] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < KT; ++j) { Xr[rr][j] = make_double2(0.0, 0.0); Yr[rr][j] = make_double2(0.0, 0.0); }
-  }
-  // column and row P of the stale G for the next PD visits, one ring slot per visit
-  constexpr int PD = (RPT == 1) ? 4 : 2;
-  cplx rc[PD][RPT], rw[PD][RPT];
-  int stop_at = a.n_op;
-  int buf = 0;
-  bool prologue = true;
-  while (true) {
-    if (prologue) {
-      // ---- per-visit proposal, tables, uniforms and stale G_PP for visits pos.. ----
-      for (int v = pos + tid; v < a.n_op; v += blockDim.x) {
-        Visit w;
+  while (pos < a.n_op) {
+    if (m >= kmax) {
+      if (!a.final_round) break;
+      // last round of the slice and the round is full: flush G += X Y^T in place
+      for (int r = tid; r < n; r += 256) {
+        for (int c = 0; c < n; ++c) {
+          cplx acc = a.G[r + (long long)c * n];
+          for (int j = 0; j < m; ++j) acc = cfma(a.X[r + (long long)j * n], a.Y[c + (long long)j * n], acc);
+          a.G[r + (long long)c * n] = acc;
+        }
+      }
+      __syncthreads();
+      m = 0;
+    }
+    const int nb = a.n_op - pos < SB_V ? a.n_op - pos : SB_V;
+    // ---- the block's proposals, table entries and uniforms (a field changes only at its own visit) ----
+    if (tid < SB_V) {
+      BVisit w;
+      w.myexp = make_double2(1.0, 0.0); w.expal = make_double2(1.0, 0.0);
+      w.gr = 1.0; w.u3 = 2.0; w.P = 0; w.snew = 0; w.nop = 0;
+      bool ok = false;
+      if (tid < nb) {
+        const int v = pos + tid;
         const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
         const int typ = a.op_type[nop];
         const int s_old = a.f[(long long)a.slice * a.n_op + nop];
         const unsigned long long q = base + (unsigned long long)a.uoff[v];
         const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
-        w.ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
+        ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
         int s_new = -s_old;
-        double u3 = 2.0;
-        if (w.ok) {
+        if (ok) {
           const double* u = a.u + (q - a.u_base);
-          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
+          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); w.u3 = u[2]; } else { w.u3 = u[1]; }
         }
         const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
-        const int P = a.op_site[nop];
         w.myexp = a.op_myexp[ti];
         w.expal = a.op_expalpha[ti];
         w.gr = a.gama_ratio[ti];
-        w.u3 = u3;
-        w.P = P;
+        w.P = a.op_site[nop];
         w.snew = s_new;
         w.nop = nop;
-        w.gpp = a.G[P + (long long)P * n];
-        vis[v] = w;
       }
-      __syncthreads();
-      if (tid == 0) {
-        s_stop_at = a.n_op;
-        for (int v = pos; v < a.n_op; ++v)
-          if (!vis[v].ok) { s_stop_at = v; break; }
+      // index of the site in the block: its first visit in the block
+      int loc = tid;
+      for (int t = 0; t < SB_V; ++t) {
+        const int Pt = __builtin_amdgcn_readlane(w.P, t);
+        if (t < tid && t < nb && tid < nb && Pt == w.P && loc == tid) loc = t;
       }
-      __syncthreads();
-      stop_at = s_stop_at;
-      if (pos < stop_at) {
-#pragma unroll
-        for (int S = 0; S < PD; ++S) {
-          const int v = pos + S < stop_at ? pos + S : stop_at - 1;
-          const int P = vis[v].P;
-#pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            const int r = tid + 256 * rr;
-            const int rcl = r < n ? r : n - 1;
-            rc[S][rr] = a.G[rcl + (long long)P * n];
-            rw[S][rr] = a.G[P + (long long)rcl * n];
-          }
-        }
-      }
-      prologue = false;
+      w.loc = tid < nb ? loc : 0;
+      vb[tid] = w;
+      const unsigned long long bad = __ballot(tid < nb && !ok);
+      if (tid == 0) s_nvis = bad ? (int)__builtin_ctzll(bad) : nb;
     }
-    // One visit: the owner of row P decides; on acceptance every thread extends its rows
-    // of X, Y (and the running diagonal correction) using column/row P of the stale G
-    // held in COLV/ROWV.
-#define ALFQ_CHAIN_VISIT(COLV, ROWV)                                                            \
-    {                                                                                         \
-      const Visit w = vis[pos];                                                               \
-      const int P = w.P;                                                                      \
-      const int owner = P & 255, orr = P >> 8;                                                \
-      if (tid == owner) {                                                                     \
-        cplx d = w.gpp;                                                                       \
-        _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) if (rr == orr) d = cadd(d, dcorr[rr]); \
-        const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);                          \
-        const cplx dmat = csub(w.myexp, cmul(delta, d));                                      \
-        const cplx ratio = cmul(dmat, w.expal);                                               \
-        cplx rt = ratio;                                                                      \
-        for (int p = 1; p < a.n_sun; ++p) rt = cmul(rt, ratio);                               \
-        rt = cscale(rt, w.gr);                                                                \
-        const cplx ph = s_phase;                                                              \
-        const double weight = fabs(cmul(ph, rt).x * s_iph);                                   \
-        s_nc += 1;                                                                            \
-        int acc = 0;                                                                          \
-        if (weight > w.u3) {                                                                  \
-          acc = 1;                                                                            \
-          s_accn += 1;                                                                        \
-          const cplx nph = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));                       \
-          s_phase = nph;                                                                      \
-          s_iph = 1.0 / nph.x;                                                                \
-          a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;                          \
-          const cplx vp = make_double2(1.0 - d.x, -d.y);                                      \
-          s_x[buf] = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));              \
-          _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) {                                \
-            if (rr == orr) {                                                                  \
-              _Pragma("unroll") for (int j = 0; j < KT; ++j) {                                \
-                if (j < m) { sX[buf][j] = Xr[rr][j]; sY[buf][j] = Yr[rr][j]; }                \
-              }                                                                               \
-            }                                                                                 \
-          }                                                                                   \
-        }                                                                                     \
-        s_acc[buf] = acc;                                                                     \
-      }                                                                                       \
-      block_sync_lds();                                                                       \
-      if (s_acc[buf]) {                                                                       \
-        const cplx x = s_x[buf];                                                              \
-        const cplx mx = make_double2(-x.x, -x.y);                                             \
-        _Pragma("unroll") for (int rr = 0; rr < RPT; ++rr) {                                  \
-          const int r = tid + 256 * rr;                                                       \
-          cplx col = COLV[rr], row = ROWV[rr];                                                \
-          _Pragma("unroll") for (int j = 0; j < KT; ++j) {                                    \
-            if (j < m) {                                                                      \
-              col = cfma(Xr[rr][j], sY[buf][j], col);                                         \
-              row = cfma(sX[buf][j], Yr[rr][j], row);                                         \
-            }                                                                                 \
-          }                                                                                   \
-          const cplx xn = cmul(mx, col);                                                      \
-          const cplx yn = make_double2((r == P ? 1.0 : 0.0) - row.x, -row.y);                 \
-          dcorr[rr] = cfma(xn, yn, dcorr[rr]);                                                \
-          _Pragma("unroll") for (int j = 0; j < KT; ++j)                                      \
-            if (j == m) { Xr[rr][j] = xn; Yr[rr][j] = yn; }                                   \
-        }                                                                                     \
-        m = __builtin_amdgcn_readfirstlane(m + 1);                                            \
-      }                                                                                       \
-      pos = __builtin_amdgcn_readfirstlane(pos + 1);                                          \
-      buf ^= 1;                                                                               \
+    __syncthreads();
+    const int nvis = s_nvis;
+    // rows of X, Y at the block's sites for the round's earlier flips, and G at the block's sites
+    for (int e = tid; e < SB_V * m; e += 256) {
+      const int r = e & (SB_V - 1), i = e >> 5;
+      cplx xv = make_double2(0.0, 0.0), yv = make_double2(0.0, 0.0);
+      if (r < nb) {
+        const int P = vb[r].P;
+        xv = a.X[P + (long long)i * n];
+        yv = a.Y[P + (long long)i * n];
+      }
+      Xb[i][r] = xv;
+      Yb[i][r] = yv;
     }
+    cplx gown[4];
+    {
+      const int Pa = vb[ga].P;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int b = gq + 8 * q;
+        gown[q] = (ga < nb && b < nb) ? a.G[Pa + (long long)vb[b].P * n] : make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    for (int i = 0; i < m; ++i) {
+      const cplx xa = Xb[i][ga];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gown[q] = cfma(xa, Yb[i][gq + 8 * q], gown[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) gs[0][ga + SB_V * (gq + 8 * q)] = gown[q];
+    __syncthreads();
 
-    // fast path: PD visits per iteration with no per-visit guard, so the refills of the
-    // ring stay in flight across iterations; taken while PD more visits and accepts fit
-    while (pos + PD <= stop_at && m + PD <= kmax) {
+    // ---- the visits: decisions from g, rank-1 updates of g on acceptance ----
+    int buf = 0, mb = 0, k = 0;
+    BVisit w = vb[0];
+    BVisit wn = vb[1];
+    cplx d = gs[0][w.loc * (SB_V + 1)];
+    for (; k < nvis; ++k) {
+      if (m + mb >= kmax || mb >= SB_A) break;
+      const BVisit wnn = vb[k + 2 < SB_V ? k + 2 : SB_V - 1];
+      const int p = w.loc, pn = wn.loc;
+      const cplx dn_rej = gs[buf][pn * (SB_V + 1)];
+      const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
+      const cplx dmat = csub(w.myexp, cmul(delta, d));
+      const cplx ratio = cmul(dmat, w.expal);
+      cplx rt = ratio;
+      for (int s = 1; s < a.n_sun; ++s) rt = cmul(rt, ratio);
+      rt = cscale(rt, w.gr);
+      const double weight = fabs(cmul(ph, rt).x * iph);
+      nc += 1;
+      const int accept = __builtin_amdgcn_readfirstlane(weight > w.u3 ? 1 : 0);
+      cplx dn = dn_rej;
+      if (accept) {
+        accn += 1;
+        ph = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
+        iph = 1.0 / ph.x;
+        // x = Delta / (1 + v_P Delta), v_P = 1 - G_cur(P,P)
+        const cplx vp = make_double2(1.0 - d.x, -d.y);
+        const cplx x = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
+        const cplx mx = make_double2(-x.x, -x.y);
+        if (tid == 0) {
+          a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
+          s_xj[mb] = x;
+          s_aloc[mb] = p;
+          s_aP[mb] = w.P;
+        }
+        const cplx ua = cmul(mx, gs[buf][ga + SB_V * p]);
+        const cplx g_np = gs[buf][pn + SB_V * p];
+        const cplx g_pn = gs[buf][p + SB_V * pn];
+        if (gq == 0) Us[mb][ga] = ua;
 #pragma unroll
-      for (int S = 0; S < PD; ++S) {
-        ALFQ_CHAIN_VISIT(rc[S], rw[S])
-        {
-          const int v = pos - 1 + PD < stop_at ? pos - 1 + PD : stop_at - 1;
-          const int Pn = vis[v].P;
+        for (int q = 0; q < 4; ++q) {
+          const int b = gq + 8 * q;
+          const cplx gpb = gs[buf][p + SB_V * b];
+          const cplx wb = make_double2((b == p ? 1.0 : 0.0) - gpb.x, -gpb.y);
+          if (ga == 0) Ws[mb][b] = wb;
+          gown[q] = cfma(ua, wb, gown[q]);
+          gs[buf ^ 1][ga + SB_V * b] = gown[q];
+        }
+        // the next visit's diagonal entry after this update, as its owner computes it
+        dn = cfma(cmul(mx, g_np), make_double2((p == pn ? 1.0 : 0.0) - g_pn.x, -g_pn.y), dn_rej);
+        mb += 1;
+        buf ^= 1;
+        block_sync_lds();
+      }
+      w = wn;
+      wn = wnn;
+      d = dn;
+    }
+    pos += k;
+    if (k == nvis && nvis < nb) stopped = true;   // random stream exhausted at this visit
+
+    // ---- full columns X(:, m:m+mb), Y(:, m:m+mb) of the block's flips ----
+    if (mb > 0) {
+      for (int r = tid; r < n; r += 256) {
+        cplx xc[SB_A], yc[SB_A];
 #pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            const int rcl = (tid + 256 * rr) < n ? tid + 256 * rr : n - 1;
-            rc[S][rr] = a.G[rcl + (long long)Pn * n];
-            rw[S][rr] = a.G[Pn + (long long)rcl * n];
+        for (int j = 0; j < SB_A; ++j) {
+          if (j < mb) {
+            const int Pj = s_aP[j];
+            xc[j] = a.G[r + (long long)Pj * n];
+            yc[j] = a.G[Pj + (long long)r * n];
+          }
+        }
+        for (int i = 0; i < m; ++i) {
+          const cplx xr = a.X[r + (long long)i * n], yr = a.Y[r + (long long)i * n];
+#pragma unroll
+          for (int j = 0; j < SB_A; ++j) {
+            if (j < mb) {
+              const int pj = s_aloc[j];
+              xc[j] = cfma(xr, Yb[i][pj], xc[j]);
+              yc[j] = cfma(Xb[i][pj], yr, yc[j]);
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < SB_A; ++j) {
+          if (j < mb) {
+            const int pj = s_aloc[j];
+#pragma unroll
+            for (int i = 0; i < j; ++i) {
+              xc[j] = cfma(xc[i], Ws[i][pj], xc[j]);
+              yc[j] = cfma(Us[i][pj], yc[i], yc[j]);
+            }
+            const cplx x = s_xj[j];
+            xc[j] = cmul(make_double2(-x.x, -x.y), xc[j]);
+            yc[j] = make_double2((r == s_aP[j] ? 1.0 : 0.0) - yc[j].x, -yc[j].y);
+            a.X[r + (long long)(m + j) * n] = xc[j];
+            a.Y[r + (long long)(m + j) * n] = yc[j];
           }
         }
       }
+      m += mb;
     }
-    // slow path for the last few visits or accepts of the round: direct loads
-    while (pos < stop_at && m < kmax) {
-      cplx dc[RPT], dr[RPT];
-      const int P = vis[pos].P;
-#pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        const int rcl = (tid + 256 * rr) < n ? tid + 256 * rr : n - 1;
-        dc[rr] = a.G[rcl + (long long)P * n];
-        dr[rr] = a.G[P + (long long)rcl * n];
-      }
-      ALFQ_CHAIN_VISIT(dc, dr)
-    }
-#undef ALFQ_CHAIN_VISIT
-    if (pos >= stop_at || !a.final_round) break;
-    // ---- last round of the slice and K accepts pending: flush G += X Y^T in place ----
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-      const int r = tid + 256 * rr;
-      if (r < n) {
-#pragma unroll
-        for (int j = 0; j < KT; ++j)
-          if (j < m) a.Y[r + (long long)j * n] = Yr[rr][j];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-      const int r = tid + 256 * rr;
-      if (r < n) {
-        for (int c = 0; c < n; ++c) {
-          cplx acc = a.G[r + (long long)c * n];
-#pragma unroll
-          for (int j = 0; j < KT; ++j)
-            if (j < m) acc = cfma(Xr[rr][j], a.Y[c + (long long)j * n], acc);
-          a.G[r + (long long)c * n] = acc;
-        }
-        dcorr[rr] = make_double2(0.0, 0.0);
-      }
-    }
-    __syncthreads();
-    m = 0;
-    prologue = true;
+    __syncthreads();   // the new columns are visible to the next block, the LDS tables are free
+    if (stopped) break;
   }
-  // ---- hand the pending columns to the flush GEMM and save the chain state ----
-#pragma unroll
-  for (int rr = 0; rr < RPT; ++rr) {
-    const int r = tid + 256 * rr;
-    if (r < n) {
-#pragma unroll
-      for (int j = 0; j < KT; ++j) {
-        if (j < m) {
-          a.X[r + (long long)j * n] = Xr[rr][j];
-          a.Y[r + (long long)j * n] = Yr[rr][j];
-        }
-      }
-    }
-  }
-  __syncthreads();
   if (tid == 0) {
-    st->phase = s_phase;
-    st->nc = s_nc;
-    st->acc = s_accn;
+    st->phase = ph;
+    st->nc = nc;
+    st->acc = accn;
     st->pos = pos;
     st->m = m;
     st->rng_slice_base = base;
     st->rng_pos = base + (unsigned long long)a.uoff[pos];
-    st->slice_acc += (int)(s_accn - acc_at_start);
-    if (pos == stop_at && stop_at < a.n_op) {
+    st->slice_acc += (int)(accn - acc_at_start);
+    if (stopped) {
       st->status = 1;
       st->pos = a.n_op;    // do not revisit
     }
@@ -521,38 +515,8 @@ int launch_philox_fill(double* u, unsigned long long base, long long count, unsi
 }
 
 int chain_max_delay(int n) {
-  if (n <= 256) return 32;
-  if (n <= 512) return 16;
-  if (n <= 1024) return 8;
-  if (n <= 2048) return 4;
-  return 0;
-}
-
-template <int RPT, int KT>
-static int launch_chain2(const ChainArgs& a, hipStream_t s, KTimer* t) {
-  const size_t lds = sizeof(Visit) * (size_t)(a.n_op > 0 ? a.n_op : 1);
-  static std::once_flag once;
-  static hipError_t attr_err = hipSuccess;
-  std::call_once(once, [] {
-    attr_err = hipFuncSetAttribute((const void*)chain2_kernel<RPT, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   96 * 1024);
-  });
-  HIP_OK(attr_err);
-  if (lds > 96 * 1024) { alfq_set_error("launch_chain: too many operators per slice for the LDS tables"); return ST_GENERIC; }
-  if (t && t->on) {
-    if (t->used == t->ev.size()) {
-      hipEvent_t e0, e1;
-      HIP_OK(hipEventCreate(&e0));
-      HIP_OK(hipEventCreate(&e1));
-      t->ev.emplace_back(e0, e1);
-    }
-    auto& pr = t->ev[t->used++];
-    hipExtLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, pr.first, pr.second, 0, a);
-  } else {
-    hipLaunchKernelGGL((chain2_kernel<RPT, KT>), dim3(1), dim3(256), lds, s, a);
-  }
-  HIP_OK(hipGetLastError());
-  return ST_NONE;
+  (void)n;
+  return SB_K;
 }
 
 int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
@@ -560,12 +524,21 @@ int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
     alfq_set_error("launch_chain: K outside [1, CHAIN_KMAX]");
     return ST_GENERIC;
   }
-  const int kr = chain_max_delay(a.n);
-  if (kr > 0 && a.Kmax <= kr && a.uoff) {
-    if (a.n <= 256) return launch_chain2<1, 32>(a, s, t);
-    if (a.n <= 512) return launch_chain2<2, 16>(a, s, t);
-    if (a.n <= 1024) return launch_chain2<4, 8>(a, s, t);
-    return launch_chain2<8, 4>(a, s, t);
+  if (a.Kmax <= SB_K && a.uoff) {
+    if (t && t->on) {
+      if (t->used == t->ev.size()) {
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        t->ev.emplace_back(e0, e1);
+      }
+      auto& pr = t->ev[t->used++];
+      hipExtLaunchKernelGGL(chain_block_kernel, dim3(1), dim3(256), 0, s, pr.first, pr.second, 0, a);
+    } else {
+      hipLaunchKernelGGL(chain_block_kernel, dim3(1), dim3(256), 0, s, a);
+    }
+    HIP_OK(hipGetLastError());
+    return ST_NONE;
   }
   hipLaunchKernelGGL(chain_kernel, dim3(1), dim3(256), 0, s, a);
   HIP_OK(hipGetLastError());
