@@ -7,9 +7,10 @@
 //   G_cur = G + X(:, 0:m) Y(:, 0:m)^T,
 //   X(:, j) = -x_j G_cur_j(:, P_j),  Y(:, j) = e_P - G_cur_j(P_j, :)^T,
 // which is the rank-1 update of upgrade_mod.F90:266-269 (ZGERU) kept
-// unapplied.  A rejected proposal only needs G_cur(P,P) = G(P,P) + X(P,:).Y(P,:);
-// an accepted one materialises the new column/row pair in O(n m).  The kernel
-// stops after K accepted flips; the flush G += X Y^T is an MFMA GEMM.
+// unapplied.  The block kernel decides from a submatrix of G_cur held in LDS
+// and forms the columns of X and Y in bulk per block of visits; the generic
+// kernel (deep delays) forms them one accepted flip at a time.  A kernel stops
+// after K accepted flips; the flush G += X Y^T is an MFMA GEMM.
 //
 // Diagonal operator exponentials (the type-1 Op_Wrapup/Op_Wrapdo scalings,
 // Operator_mod.F90:767-782, 884-899) are not applied here: for diagonal

@@ -1,35 +1,45 @@
 // FP64 MFMA GEMM for gfx950: real x complex and complex x complex products with
 // fused diagonal scalings, gathers and conjugation.
 //
-// Tiling: one workgroup (4 waves) computes a 32 x 32 complex output tile; each
-// wave owns a 16 x 16 sub-tile held as two (re, im) v_mfma_f64_16x16x4f64
-// accumulators.  K advances in steps of 16 through LDS, where the operands are
-// split into real and imaginary planes so each MFMA consumes one plane.
-// A complex product costs 4 MFMAs per k-step of 4, a real x complex one 2.
+// Tiling: a workgroup of 4 waves computes a (16 WM) x (16 WN) complex output tile;
+// the waves are arranged WM x WN x KS, so with KS > 1 the waves of one tile split
+// K and their partial tiles are summed through LDS in a fixed order (the result
+// does not depend on scheduling).  Each wave holds a 16 x 16 sub-tile as two
+// (re, im) v_mfma_f64_16x16x4f64 accumulators.  K advances in stages of 16 KS
+// through LDS, where the operands are split into real and imaginary planes so each
+// MFMA consumes one plane; a complex product costs 4 MFMAs per k-step of 4, a
+// real x complex one 2.  The host picks the tile shape per launch: at Ndim = 256 a
+// 32 x 32 tiling leaves most of the 256 CUs idle, a 16 x 16 tiling with KS = 4 fills
+// them with a quarter of the per-wave K.
 //
 // This one kernel carries every dense product of the sweep:
-//   * hopping wraps  G <- D exp(-dtau T) G exp(dtau T) D^-1       (K1, real T: 2 MFMA/step)
+//   * hopping wraps  G <- D exp(-dtau T) G exp(dtau T) D^-1       (real T: 2 MFMA/step)
 //   * UDV chain      U <- D exp(-dtau T) U,  U <- exp(-dtau T) conj(D) U
-//   * delayed-update flush  G += X Y^T                              (K2b)
-//   * QR trailing updates / Q formation / TRMM / CGR products       (K3, K4)
+//   * delayed-update flush  G += X Y^T
+//   * QR trailing updates / Q formation / TRMM / CGR products
 #include "alfq_common.h"
 
 namespace {
 
-constexpr int BM = 32, BN = 32, BK = 16, NT = 256;
+constexpr int NT = 256;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <bool A_CPLX, bool B_CPLX>
+template <bool A_CPLX, bool B_CPLX, int WM, int WN>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
-  __shared__ double As_re[BK][BM + 1];
-  __shared__ double As_im[A_CPLX ? BK : 1][BM + 1];
-  __shared__ double Bs_re[BK][BN + 1];
-  __shared__ double Bs_im[B_CPLX ? BK : 1][BN + 1];
+  constexpr int KS = 4 / (WM * WN);
+  constexpr int TM = 16 * WM, TN = 16 * WN, BK = 16 * KS;
+  constexpr int LA = TM * BK / NT, LB = BK * TN / NT;   // elements per thread per stage
+  static_assert(WM * WN * KS == 4 && LA >= 1 && LB >= 1, "4 waves per tile");
+  __shared__ double As_re[BK][TM + 1];
+  __shared__ double As_im[A_CPLX ? BK : 1][TM + 1];
+  __shared__ double Bs_re[BK][TN + 1];
+  __shared__ double Bs_im[B_CPLX ? BK : 1][TN + 1];
+  __shared__ cplx red[KS > 1 ? KS - 1 : 1][KS > 1 ? TM * TN : 1];
 
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * BM;
-  const int j0 = blockIdx.y * BN;
+  const int i0 = blockIdx.x * TM;
+  const int j0 = blockIdx.y * TN;
   const int z = blockIdx.z;
   int K = g.K;
   if (g.kdev) {
@@ -49,114 +59,86 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   cplx* C = (cplx*)g.C + (g.batch > 1 ? z * g.strideC : 0);
 
   const int wave = tid >> 6, lane = tid & 63;
-  const int wr = (wave & 1) * 16, wc = (wave >> 1) * 16;
+  const int wm = wave % WM, wn = (wave / WM) % WN, ks = wave / (WM * WN);
+  const int wr = 16 * wm, wc = 16 * wn;
   const int lr = lane & 15, lk = lane >> 4;
 
   v4d acc_re = {0, 0, 0, 0};
   v4d acc_im = {0, 0, 0, 0};
 
-  // global -> register loads of one K-tile (2 opA and 2 opB elements per thread)
-  double ra_re[2], ra_im[2], rb_re[2], rb_im[2];
+  // global -> register loads of one K stage; consecutive threads walk the contiguous
+  // index of each operand's storage
+  double ra_re[LA], ra_im[LA], rb_re[LB], rb_im[LB];
+  auto a_index = [&](int p, int& i, int& kk) {
+    const int idx = tid + NT * p;
+    if (g.opA == OP_N) { i = idx % TM; kk = idx / TM; } else { kk = idx % BK; i = idx / BK; }
+  };
+  auto b_index = [&](int p, int& kk, int& j) {
+    const int idx = tid + NT * p;
+    if (g.opB == OP_N) { kk = idx % BK; j = idx / BK; } else { j = idx % TN; kk = idx / TN; }
+  };
   auto load_tile = [&](int k0) {
-    if (g.opA == OP_N) {
-      const int i = tid & 31;
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int kk = (tid >> 5) + 8 * p;
-        const int gi = i0 + i, gk = k0 + kk;
-        double re = 0.0, im = 0.0;
-        if (gi < g.M && gk < K) {
-          const int col = g.amap ? g.amap[gk] : gk;
-          const long long off = gi + (long long)col * g.lda;
-          if (A_CPLX) { cplx v = Ac[off]; re = v.x; im = v.y; } else { re = Ar[off]; }
+    for (int p = 0; p < LA; ++p) {
+      int i, kk;
+      a_index(p, i, kk);
+      const int gi = i0 + i, gk = k0 + kk;
+      double re = 0.0, im = 0.0;
+      if (gi < g.M && gk < K) {
+        const int kcol = g.amap ? g.amap[gk] : gk;
+        const long long off = (g.opA == OP_N) ? gi + (long long)kcol * g.lda : kcol + (long long)gi * g.lda;
+        if (A_CPLX) {
+          const cplx v = Ac[off];
+          re = v.x;
+          im = (g.opA == OP_C) ? -v.y : v.y;
+        } else {
+          re = Ar[off];
         }
-        ra_re[p] = re;
-        ra_im[p] = im;
       }
-    } else {
-      const int kk = tid & 15;
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int i = (tid >> 4) + 16 * p;
-        const int gi = i0 + i, gk = k0 + kk;
-        double re = 0.0, im = 0.0;
-        if (gi < g.M && gk < K) {
-          const int row = g.amap ? g.amap[gk] : gk;
-          const long long off = row + (long long)gi * g.lda;
-          if (A_CPLX) { cplx v = Ac[off]; re = v.x; im = (g.opA == OP_C) ? -v.y : v.y; } else { re = Ar[off]; }
-        }
-        ra_re[p] = re;
-        ra_im[p] = im;
-      }
+      ra_re[p] = re;
+      ra_im[p] = im;
     }
-    if (g.opB == OP_N) {
-      const int kk = tid & 15;
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int j = (tid >> 4) + 16 * p;
-        const int gk = k0 + kk, gj = j0 + j;
-        double re = 0.0, im = 0.0;
-        if (gk < K && gj < g.N) {
-          const int row = g.bmap ? g.bmap[gk] : gk;
-          const long long off = row + (long long)gj * g.ldb;
-          if (g.b_cplx) { cplx v = Bc[off]; re = v.x; im = v.y; } else { re = Br[off]; }
-          if (g.bs) { cplx sc = g.bs[gk]; double r2 = re * sc.x - im * sc.y; im = re * sc.y + im * sc.x; re = r2; }
+    for (int p = 0; p < LB; ++p) {
+      int kk, j;
+      b_index(p, kk, j);
+      const int gk = k0 + kk, gj = j0 + j;
+      double re = 0.0, im = 0.0;
+      if (gk < K && gj < g.N) {
+        const int krow = g.bmap ? g.bmap[gk] : gk;
+        const long long off = (g.opB == OP_N) ? krow + (long long)gj * g.ldb : gj + (long long)krow * g.ldb;
+        if (g.b_cplx) {
+          const cplx v = Bc[off];
+          re = v.x;
+          im = (g.opB == OP_C) ? -v.y : v.y;
+        } else {
+          re = Br[off];
         }
-        rb_re[p] = re;
-        rb_im[p] = im;
-      }
-    } else {
-      const int j = tid & 31;
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int kk = (tid >> 5) + 8 * p;
-        const int gk = k0 + kk, gj = j0 + j;
-        double re = 0.0, im = 0.0;
-        if (gk < K && gj < g.N) {
-          const int col = g.bmap ? g.bmap[gk] : gk;
-          const long long off = gj + (long long)col * g.ldb;
-          if (g.b_cplx) { cplx v = Bc[off]; re = v.x; im = (g.opB == OP_C) ? -v.y : v.y; } else { re = Br[off]; }
-          if (g.bs) { cplx sc = g.bs[gk]; double r2 = re * sc.x - im * sc.y; im = re * sc.y + im * sc.x; re = r2; }
+        if (g.bs) {
+          const cplx sc = g.bs[gk];
+          const double r2 = re * sc.x - im * sc.y;
+          im = re * sc.y + im * sc.x;
+          re = r2;
         }
-        rb_re[p] = re;
-        rb_im[p] = im;
       }
+      rb_re[p] = re;
+      rb_im[p] = im;
     }
   };
   auto store_tile = [&]() {
-    if (g.opA == OP_N) {
-      const int i = tid & 31;
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int kk = (tid >> 5) + 8 * p;
-        As_re[kk][i] = ra_re[p];
-        if (A_CPLX) As_im[kk][i] = ra_im[p];
-      }
-    } else {
-      const int kk = tid & 15;
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int i = (tid >> 4) + 16 * p;
-        As_re[kk][i] = ra_re[p];
-        if (A_CPLX) As_im[kk][i] = ra_im[p];
-      }
+    for (int p = 0; p < LA; ++p) {
+      int i, kk;
+      a_index(p, i, kk);
+      As_re[kk][i] = ra_re[p];
+      if (A_CPLX) As_im[kk][i] = ra_im[p];
     }
-    if (g.opB == OP_N) {
-      const int kk = tid & 15;
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int j = (tid >> 4) + 16 * p;
-        Bs_re[kk][j] = rb_re[p];
-        if (B_CPLX) Bs_im[kk][j] = rb_im[p];
-      }
-    } else {
-      const int j = tid & 31;
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int kk = (tid >> 5) + 8 * p;
-        Bs_re[kk][j] = rb_re[p];
-        if (B_CPLX) Bs_im[kk][j] = rb_im[p];
-      }
+    for (int p = 0; p < LB; ++p) {
+      int kk, j;
+      b_index(p, kk, j);
+      Bs_re[kk][j] = rb_re[p];
+      if (B_CPLX) Bs_im[kk][j] = rb_im[p];
     }
   };
 
@@ -164,9 +146,10 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   for (int k0 = 0; k0 < K; k0 += BK) {
     store_tile();
     __syncthreads();
-    if (k0 + BK < K) load_tile(k0 + BK);   // next tile's loads overlap this tile's MFMAs
+    if (k0 + BK < K) load_tile(k0 + BK);   // next stage's loads overlap this stage's MFMAs
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
+    for (int kq = 0; kq < 16; kq += 4) {
+      const int kk = 16 * ks + kq;
       const double a_re = As_re[kk + lk][wr + lr];
       const double b_re = Bs_re[kk + lk][wc + lr];
       acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re, b_re, acc_re, 0, 0, 0);
@@ -188,6 +171,25 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 
   // ---- epilogue: v_mfma_f64_16x16x4f64 leaves D(i, j) in lane j + 16*(i%4), register i/4,
   //      so lane holds rows wr + 4*r + lane/16 (r = 0..3) of column wc + lane%16 ----
+  if (KS > 1) {
+    // waves ks >= 1 park their partial tiles; the ks = 0 waves add them in order ks = 1, 2, ...
+    if (ks > 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        red[ks - 1][(wr + 4 * r + lk) + TM * (wc + lr)] = make_double2(acc_re[r], acc_im[r]);
+    }
+    __syncthreads();
+    if (ks > 0) return;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const cplx p = red[q][(wr + 4 * r + lk) + TM * (wc + lr)];
+        acc_re[r] += p.x;
+        acc_im[r] += p.y;
+      }
+    }
+  }
   const int gj = j0 + wc + lr;
   if (gj >= g.N) return;
   cplx csj = g.cs ? g.cs[gj] : make_double2(1.0, 0.0);
@@ -206,6 +208,18 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   }
 }
 
+inline long long cdivl(long long a, long long b) { return (a + b - 1) / b; }
+
+template <int WM, int WN>
+void launch_shape(const GemmArgs& g, hipStream_t s) {
+  dim3 grid((unsigned)cdivl(g.M, 16 * WM), (unsigned)cdivl(g.N, 16 * WN), g.batch > 1 ? g.batch : 1);
+  const bool bc = g.b_cplx || g.bs != nullptr;
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_kernel<true, true, WM, WN>), grid, dim3(NT), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false, WM, WN>), grid, dim3(NT), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_kernel<false, true, WM, WN>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gemm_kernel<false, false, WM, WN>), grid, dim3(NT), 0, s, g);
+}
+
 }  // namespace
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,
@@ -221,15 +235,31 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
   return g;
 }
 
+// Tile shape with the least estimated time: every SIMD of the 256 CUs holds one wave of a
+// full round, and a wave's time is proportional to the K it covers (K / KS).
+static int gemm_tile_shape(int M, int N, int K, int batch) {
+  const long long b = batch > 1 ? batch : 1;
+  const long long slots = 256LL * 4;
+  struct Opt { int code, ks; long long tm, tn; };
+  const Opt opts[3] = {{0, 1, 32, 32}, {1, 2, 16, 32}, {2, 4, 16, 16}};
+  int best = 0;
+  double best_cost = 0.0;
+  for (int o = 0; o < 3; ++o) {
+    const long long waves = 4 * b * cdivl(M, opts[o].tm) * cdivl(N, opts[o].tn);
+    const double cost = (double)cdivl(waves, slots) * (double)cdivl(K > 0 ? K : 1, 16 * opts[o].ks);
+    if (o == 0 || cost < best_cost) { best = o; best_cost = cost; }
+  }
+  return opts[best].code;
+}
+
 int gemm_launch(const GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return ST_NONE;
   if (g.batch > 65535) { alfq_set_error("gemm_launch: batch too large"); return ST_GENERIC; }
-  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, g.batch > 1 ? g.batch : 1);
-  const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_kernel<true, true>), grid, dim3(NT), 0, s, g);
-  else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false>), grid, dim3(NT), 0, s, g);
-  else if (bc) hipLaunchKernelGGL((gemm_kernel<false, true>), grid, dim3(NT), 0, s, g);
-  else hipLaunchKernelGGL((gemm_kernel<false, false>), grid, dim3(NT), 0, s, g);
+  switch (gemm_tile_shape(g.M, g.N, g.K, g.batch)) {
+    case 0: launch_shape<2, 2>(g, s); break;
+    case 1: launch_shape<1, 2>(g, s); break;
+    default: launch_shape<1, 1>(g, s); break;
+  }
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }

@@ -107,8 +107,9 @@ __global__ void identity_kernel(cplx* __restrict__ X, int n) {
 
 // ---- panel factorisation (one workgroup) ----------------------------------------
 // Factors A(k:n-1, k:k+nb-1) in place.  Writes tau(k..), the explicit V panel into
-// Vw(k:n-1, k:k+nb-1) (unit diagonal, zeros above), T (nb x nb, upper), Y1 = V T^H and
-// Y2 = V T into the same positions of Y1w/Y2w, and det(H_k...H_{k+nb-1}) into pdet[b].
+// Vw(k:n-1, k:k+nb-1) (unit diagonal, zeros above), T (nb x nb, upper), Y1 = V T^H and,
+// if Y2w is given, Y2 = V T into the same positions of Y1w/Y2w, and
+// det(H_k...H_{k+nb-1}) into pdet[b].
 __global__ __launch_bounds__(PT) void panel_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                    cplx* __restrict__ tau, cplx* __restrict__ Vw,
                                                    cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(cplx* __restrict__ A, int n, 
       y2 = cfma(v, s_T[l + NBMAX * j], y2);
     }
     Y1w[r + (long long)(k + j) * n] = y1;
-    Y2w[r + (long long)(k + j) * n] = y2;
+    if (Y2w) Y2w[r + (long long)(k + j) * n] = y2;
   }
   if (tid == 0) {
     cplx d = make_double2(1.0, 0.0);
@@ -435,16 +436,187 @@ __global__ __launch_bounds__(PLT) void panel_lds_kernel(cplx* __restrict__ A, in
       cmfma16(
           Kc, [&](int i, int kk) { return Vget(r0 + i, kk); },
           [&](int kk, int jj) { return cconj(Tget(c0 + jj, kk)); }, re1, im1);
-      cmfma16(
-          Kc, [&](int i, int kk) { return Vget(r0 + i, kk); }, [&](int kk, int jj) { return Tget(kk, c0 + jj); },
-          re2, im2);
+      if (Y2w)
+        cmfma16(
+            Kc, [&](int i, int kk) { return Vget(r0 + i, kk); }, [&](int kk, int jj) { return Tget(kk, c0 + jj); },
+            re2, im2);
 #pragma unroll
       for (int r4 = 0; r4 < 4; ++r4) {
         const int gi = r0 + 4 * r4 + lk, gj = c0 + lr;
         if (gi < rows && gj < ncol) {
           Y1w[(k + gi) + (long long)(k + gj) * n] = make_double2(re1[r4], im1[r4]);
-          Y2w[(k + gi) + (long long)(k + gj) * n] = make_double2(re2[r4], im2[r4]);
+          if (Y2w) Y2w[(k + gi) + (long long)(k + gj) * n] = make_double2(re2[r4], im2[r4]);
         }
+      }
+    }
+  }
+  if (tid == 0) {
+    cplx dd = make_double2(1.0, 0.0);
+    for (int j = 0; j < ncol; ++j) dd = cmul(dd, csub(make_double2(1.0, 0.0), cscale(s_tau[j], s_vn2[j])));
+    pdet[b] = dd;
+  }
+}
+
+// ---- panel factorisation with the panel resident in registers (rows <= 256) -----
+// Same contract as panel_kernel except that Y2 = V T is not formed here (qr_form_q
+// forms it with the GEMM).  1024 threads: the 32 lanes of half-wave c hold column c,
+// rows l, l + 32, ..., l + 224 (l = lane in the half-wave).  Column j's reflector is
+// formed by its own half-wave and handed to the others through a double-buffered LDS
+// vector, one barrier per column; columns right of j apply H^H in registers, columns
+// left of j (already final) use the step to accumulate the Gram entry v_c^H v_j.
+// T follows from T^-1 = striu(V^H V) + diag(1/tau) (the columnwise zlarft recurrence
+// T(0:j, j) = -tau_j T(0:j, 0:j) V^H v_j, inverted), by a fan-in solve per column,
+// and Y1 = V T^H is an MFMA product out of LDS.
+constexpr int RPL = 8;          // rows per lane: 256 rows
+constexpr int RP = 32 * RPL;
+
+__device__ __forceinline__ double hsum32(double v) {
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+  return v;
+}
+
+__global__ __launch_bounds__(1024) void panel_reg_kernel(cplx* __restrict__ A, int n, int k, int nb,
+                                                         cplx* __restrict__ tau, cplx* __restrict__ Vw,
+                                                         cplx* __restrict__ Y1w, cplx* __restrict__ Tw,
+                                                         cplx* __restrict__ pdet) {
+  extern __shared__ cplx smem[];      // loop: vbuf[2][RP]; afterwards: V[RP x 32], column-major
+  __shared__ cplx s_S[32 * 32];       // Gram above the diagonal, then T
+  __shared__ cplx s_tau[32];
+  __shared__ double s_vn2[32];
+  const int tid = threadIdx.x;
+  const int l = tid & 31, c = tid >> 5;
+  const int rows = n - k;
+  const int ncol = (k + nb <= n) ? nb : n - k;
+
+  cplx a[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = l + 32 * t;
+    a[t] = (c < ncol && r < rows) ? A[(k + r) + (long long)(k + c) * n] : make_double2(0.0, 0.0);
+  }
+  for (int j = 0; j < ncol; ++j) {
+    cplx* vb = smem + (j & 1) * RP;
+    if (c == j) {
+      // zlarfg on rows j.. of this column (alpha = row j: lane j, slot 0)
+      const cplx alpha = make_double2(__shfl(a[0].x, j, 32), __shfl(a[0].y, j, 32));
+      double xs = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t)
+        if (l + 32 * t > j) xs += cabs2(a[t]);
+      xs = hsum32(xs);
+      cplx tj = make_double2(0.0, 0.0), sc = make_double2(1.0, 0.0);
+      double beta = alpha.x;
+      if (!(xs == 0.0 && alpha.y == 0.0)) {
+        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs), alpha.x);
+        tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        sc = cinv(make_double2(alpha.x - beta, alpha.y));
+      }
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = l + 32 * t;
+        cplx v = make_double2(r == j ? 1.0 : 0.0, 0.0);
+        if (r > j) v = cmul(a[t], sc);
+        vb[r] = v;
+        if (r == j) a[t] = make_double2(beta, 0.0);
+        else if (r > j) a[t] = v;
+      }
+      if (l == 0) {
+        s_tau[j] = tj;
+        tau[k + j] = tj;
+      }
+    }
+    __syncthreads();
+    if (c < ncol && c != j) {
+      cplx vr[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) vr[t] = vb[l + 32 * t];
+      if (c > j) {
+        // a <- a - conj(tau) v (v^H a)
+        double dx = 0.0, dy = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          dx += vr[t].x * a[t].x + vr[t].y * a[t].y;
+          dy += vr[t].x * a[t].y - vr[t].y * a[t].x;
+        }
+        dx = hsum32(dx);
+        dy = hsum32(dy);
+        const cplx f = cmul(cconj(s_tau[j]), make_double2(dx, dy));
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) a[t] = csub(a[t], cmul(f, vr[t]));
+      } else {
+        // Gram entry v_c^H v_j; column c holds R above its diagonal and v below
+        double gx = 0.0, gy = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int r = l + 32 * t;
+          const cplx vc = r > c ? a[t] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+          gx += vc.x * vr[t].x + vc.y * vr[t].y;
+          gy += vc.x * vr[t].y - vc.y * vr[t].x;
+        }
+        gx = hsum32(gx);
+        gy = hsum32(gy);
+        if (l == 0) s_S[c + 32 * j] = make_double2(gx, gy);
+      }
+    }
+  }
+  __syncthreads();
+  // T column c by a fan-in solve of T^-1 T = 1 from the bottom row up:
+  //   T(i, c) = tau_i (delta_ic - sum_{m > i} S(i, m) T(m, c))
+  cplx tcol = make_double2(0.0, 0.0);
+  if (c < ncol) {
+    const cplx tl_own = l <= c ? s_tau[l] : make_double2(0.0, 0.0);
+    cplx acc = make_double2(0.0, 0.0);
+    for (int i = c; i >= 0; --i) {
+      const cplx mine = cmul(tl_own, make_double2((l == c ? 1.0 : 0.0) - acc.x, -acc.y));
+      const cplx ti = make_double2(__shfl(mine.x, i, 32), __shfl(mine.y, i, 32));
+      if (l == i) tcol = ti;
+      if (l < i) acc = cfma(s_S[l + 32 * i], ti, acc);
+    }
+  }
+  // V into LDS (the reflector buffers are free), the panel and V to global memory
+  double vn = 0.0;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = l + 32 * t;
+    cplx vc = make_double2(0.0, 0.0);
+    if (c < ncol && r < rows) {
+      vc = r > c ? a[t] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+      A[(k + r) + (long long)(k + c) * n] = a[t];
+      Vw[(k + r) + (long long)(k + c) * n] = vc;
+    }
+    smem[r + RP * c] = vc;
+    vn += cabs2(vc);
+  }
+  vn = hsum32(vn);
+  __syncthreads();   // all reads of the Gram entries are done
+  if (l == 0 && c < ncol) s_vn2[c] = vn;
+  s_S[l + 32 * c] = (c < ncol && l <= c) ? tcol : make_double2(0.0, 0.0);
+  const int b = k / nb;
+  if (c < ncol && l < ncol) Tw[(long long)b * nb * nb + l + c * nb] = (l <= c) ? tcol : make_double2(0.0, 0.0);
+  __syncthreads();
+  // Y1 = V T^H: Y1(r, j) = sum_{m >= j} V(r, m) conj(T(j, m)), 16 x 16 MFMA tiles
+  {
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int tr = (rows + 15) / 16, tcn = (ncol + 15) / 16;
+    const int kc = (ncol + 3) & ~3;
+    for (int ti = wave; ti < tr * tcn; ti += 16) {
+      const int r0 = 16 * (ti % tr), c0 = 16 * (ti / tr);
+      v4d_t re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+      for (int k0 = c0; k0 < kc; k0 += 4) {
+        const cplx av = smem[(r0 + lr) + RP * (k0 + lk)];
+        const cplx tv = s_S[(c0 + lr) + 32 * (k0 + lk)];
+        const cplx bv = cconj(tv);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.y, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, im, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int gi = r0 + 4 * r4 + lk, gj = c0 + lr;
+        if (gi < rows && gj < ncol) Y1w[(k + gi) + (long long)(k + gj) * n] = make_double2(re[r4], im[r4]);
       }
     }
   }
@@ -601,6 +773,7 @@ int qr_work_alloc(QRWork& w, int n) {
   const size_t nn = (size_t)n * n;
   HIP_OK(hipMalloc(&w.A, nn * sizeof(cplx)));
   HIP_OK(hipMalloc(&w.Vw, nn * sizeof(cplx)));
+  HIP_OK(hipMemset(w.Vw, 0, nn * sizeof(cplx)));   // the panels never write above their diagonal block
   HIP_OK(hipMalloc(&w.Y1, nn * sizeof(cplx)));
   HIP_OK(hipMalloc(&w.Y2, nn * sizeof(cplx)));
   HIP_OK(hipMalloc(&w.Rt, nn * sizeof(cplx)));
@@ -648,7 +821,17 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   for (int k = 0; k < n; k += nb) {
-    if (w.lds_panel) {
+    if (n <= RP && nb <= 32) {
+      static std::once_flag once_r;
+      static hipError_t attr_r = hipSuccess;
+      std::call_once(once_r, [] {
+        attr_r = hipFuncSetAttribute((const void*)panel_reg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(RP * 32 * sizeof(cplx)));
+      });
+      HIP_OK(attr_r);
+      hipLaunchKernelGGL(panel_reg_kernel, dim3(1), dim3(1024), RP * 32 * sizeof(cplx), s, w.A, n, k, nb, w.tau, w.Vw,
+                         w.Y1, w.T, w.pdet);
+    } else if (w.lds_panel) {
       const size_t lds = (size_t)(n - k) * std::min(nb, n - k) * sizeof(cplx);
       static std::once_flag once;
       static hipError_t attr_err = hipSuccess;
@@ -657,11 +840,11 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
                                        128 * 1024);
       });
       HIP_OK(attr_err);
-      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PLT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
-                         w.pdet);
+      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PLT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1,
+                         (cplx*)nullptr, w.T, w.pdet);
     } else {
-      hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, w.Y2, w.T,
-                         w.pdet);
+      hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, (cplx*)nullptr,
+                         w.T, w.pdet);
     }
     HIP_OK(hipGetLastError());
     const int kc = k + nb;
@@ -685,6 +868,23 @@ static int qr_form_q(QRWork& w, cplx* Q, hipStream_t s) {
   int st;
   if ((st = launch_identity(Q, n, s))) return st;
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+  // Y2 = V T for every panel (V is zero above each panel's diagonal block)
+  if (n % nb == 0) {
+    GemmArgs gy = gemm_args(n, nb, nb, w.Vw, n, OP_N, true, w.T, nb, OP_N, true, w.Y2, n, one, zero);
+    gy.batch = w.nblk;
+    gy.strideA = (long long)n * nb;
+    gy.strideB = (long long)nb * nb;
+    gy.strideC = (long long)n * nb;
+    if ((st = gemm_launch(gy, s))) return st;
+  } else {
+    for (int b = 0; b < w.nblk; ++b) {
+      const int k = b * nb;
+      const int ncol = (k + nb <= n) ? nb : n - k;
+      GemmArgs gy = gemm_args(n - k, ncol, ncol, w.Vw + k + (long long)k * n, n, OP_N, true,
+                              w.T + (long long)b * nb * nb, nb, OP_N, true, w.Y2 + k + (long long)k * n, n, one, zero);
+      if ((st = gemm_launch(gy, s))) return st;
+    }
+  }
   for (int b = w.nblk - 1; b >= 0; --b) {
     const int k = b * nb;
     const int ncol = (k + nb <= n) ? nb : n - k;

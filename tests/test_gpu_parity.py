@@ -31,7 +31,10 @@ def engine(rt, ham, f, u, delay=8):
 
 
 # ---------------------------------------------------------------- K1: the MFMA GEMM
-@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (37, 50, 29), (100, 36, 64), (256, 256, 256)])
+# the shapes reach all three tilings: 32x32 (16x16x16, 300x17x5), 16x32 with K split in 2
+# (32x32x32, 37x50x29, 256x224x32) and 16x16 with K split in 4 (the others)
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (37, 50, 29), (100, 36, 64), (256, 256, 256),
+                                   (256, 224, 32), (32, 224, 256), (16, 200, 300), (300, 17, 5)])
 def test_gemm_against_numpy(rt, shape):
     M, N, K = shape
     r = np.random.default_rng(M * 7 + N)
