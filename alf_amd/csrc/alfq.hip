@@ -169,6 +169,8 @@ struct alfq_ctx {
   cplx* H = nullptr;
   cplx* X = nullptr;
   cplx* Y = nullptr;
+  cplx* Wk = nullptr;       // block kernel: W = C D^T of the last round (32 x 32)
+  int* amapk = nullptr;     // block kernel: sites of the last round's accepted flips
   ChainState* st = nullptr;
   DevScalars* sc = nullptr;
   double* log = nullptr;
@@ -292,6 +294,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.op_myexp = c->myexp; a.op_expalpha = c->expalpha; a.gama_ratio = c->gama_ratio;
   a.n_sun = c->n_sun;
   a.u = c->u; a.u_base = c->u_base; a.u_len = c->u_len; a.st = c->st;
+  a.W = c->Wk; a.amap = c->amapk;
   a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
   const int worst = c->n_op / c->K + 1;
   const bool reg_kernel = chain_max_delay(c->n) >= c->K;
@@ -307,6 +310,14 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
     a.first_round = (r == 0);
     a.final_round = (r == rounds - 1);
     if ((st = launch_chain(a, c->s, &c->chain_timer))) return st;
+    if (reg_kernel) {
+      // X = G(:, A) W: the delayed-update columns of the round from its sites and W = C D^T
+      GemmArgs gx = gemm_args(c->n, c->K, c->K, c->G, c->n, OP_N, true, c->Wk, CHAIN_W_LD, OP_N, true, c->X, c->n,
+                              ONE, ZERO);
+      gx.amap = c->amapk;
+      gx.kdev = &c->st->m;
+      if ((st = gemm_launch(gx, c->s))) return st;
+    }
     // G += X Y^T with the rank read from the chain state
     GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
     g.kdev = &c->st->m;
@@ -325,7 +336,9 @@ int adapt_rounds(alfq_ctx* c) {
   const int zero = 0;
   HIP_OK(hipMemcpyAsync(&c->st->max_slice_acc, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
   const int worst = c->n_op / c->K + 1;
-  const int want = (int)std::ceil(1.25 * mx / c->K) + 1;
+  // enough rounds for the busiest slice seen, with a 10% margin; a slice that needs more
+  // is finished by the last round's in-place flush
+  const int want = (int)std::ceil(1.1 * mx / c->K);
   c->rounds = std::max(2, std::min(worst, want));
   HIP_OK(hipStreamSynchronize(c->s));
   return ST_NONE;
@@ -707,7 +720,9 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
   if (hipMalloc(&c->f, fb) || hipMemset(c->f, 1, fb)) return fail("fields");
   if (hipMalloc(&c->G, nn * sizeof(cplx)) || hipMalloc(&c->Gtmp, nn * sizeof(cplx)) ||
       hipMalloc(&c->H, nn * sizeof(cplx)) || hipMalloc(&c->X, (size_t)n * CHAIN_KMAX * sizeof(cplx)) ||
-      hipMalloc(&c->Y, (size_t)n * CHAIN_KMAX * sizeof(cplx)))
+      hipMalloc(&c->Y, (size_t)n * CHAIN_KMAX * sizeof(cplx)) ||
+      hipMalloc(&c->Wk, (size_t)CHAIN_KMAX * CHAIN_KMAX * sizeof(cplx)) ||
+      hipMalloc(&c->amapk, CHAIN_KMAX * sizeof(int)) || hipMemset(c->amapk, 0, CHAIN_KMAX * sizeof(int)))
     return fail("G buffers");
   if (hipMalloc(&c->st, sizeof(ChainState)) || hipMalloc(&c->sc, sizeof(DevScalars)) ||
       hipMalloc(&c->log, 3 * LOG_CAP * sizeof(double)))
@@ -743,7 +758,7 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (c->s) hipStreamSynchronize(c->s);
   for (auto& sl : c->pool) { hipFree(sl.U); hipFree(sl.V); hipFree(sl.D); hipFree(sl.sc); }
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
-                  c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->st, c->sc,
+                  c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->Wk, c->amapk, c->st, c->sc,
                   c->log, c->dv_d, c->dv_dinv, c->dv_dconj, c->U0, c->U1, c->TP, c->Qc, c->DUP, c->DUPinv,
                   c->Xc, c->Yc, c->cscal, c->u};
   for (void* p : ptrs)

@@ -6,6 +6,7 @@
 #include <vector>
 
 #define CHAIN_KMAX 64   // largest delayed-update depth
+#define CHAIN_W_LD 32   // leading dimension of the block kernel's W
 
 // Per-chain device state touched by the sequential Metropolis kernel.
 struct ChainState {
@@ -35,6 +36,8 @@ struct ChainArgs {
   int first_round;                // start of the slice: visit 0, rng base = current position
   int final_round;                // last scheduled round: flush in place instead of stopping at K
   ChainState* st;
+  // block kernel outputs for the flush G += (G(:, amap) W) Y^T: W (32 x 32, ld 32), site list
+  cplx* W; int* amap;
 };
 
 // Optional per-launch timing of the chain kernel: start/stop events recorded by the

@@ -199,20 +199,45 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainArgs a) {
 // An accepted flip is the rank-1 update of upgrade_mod.F90:266-269 restricted to S,
 //   g += u w^T,  u = -x g(:, p),  w = e_p - g(p, :)^T,
 // and u, w are the restrictions to S of the delayed-update columns X(:, j), Y(:, j).
-// At the end of the block the full columns follow from the Green's function at the
-// block start, G_bs = G + X(:, 0:m0) Y(:, 0:m0)^T, one row per thread:
-//   X(:, j) = -x_j [G_bs(:, P_j) + sum_{i<j in block} X(:, i) w_i(p_j)],
-//   Y(:, j) = e_{P_j} - [G_bs(P_j, :)^T + sum_{i<j in block} Y(:, i) u_i(p_j)],
-// and the next block starts from g = G[S', S'] + X[S', :] Y[S', :]^T.
+//
+// The full columns are never formed one flip at a time.  With A = (P_0, P_1, ...) the
+// sites of the round's accepted flips in order and G the Green's function at the start
+// of the round,
+//   X = G(:, A) C,   Y = Yhat D,   Yhat(:, j) = e_{P_j} - G(P_j, :)^T,
+//   C(:, j) = -x_j (e_j + sum_{i<j} C(:, i) Y(P_j, i)),
+//   D(:, j) =        e_j - sum_{i<j} D(:, i) X(P_j, i),
+// with C, D upper triangular of the round's size; the entries X(P_j, i), Y(P_j, i)
+// are already in LDS (u, w of the block, or the block-start rows below).  A block
+// starts from X[S, :] = G[S, A] C, Y[S, :] = Yhat[S, :] D and
+// g = G[S, S] + X[S, :] Y[S, :]^T.  At the end of the round the kernel leaves
+// W = C D^T, the site list A and Yhat; the flush is G += (G(:, A) W) Yhat^T, two GEMMs
+// over the whole device.
 constexpr int SB_V = 32;   // visits per block: g is SB_V x SB_V
 constexpr int SB_A = 16;   // accepted flips per block
 constexpr int SB_K = 32;   // accepted flips per round (delayed-update depth)
+static_assert(SB_K == CHAIN_W_LD, "W is stored with the round's size as leading dimension");
 
+// One visit of a block.  With Delta = exp(Z1 E) - 1 the single-colour ratio is
+// exp(Z1 alpha) (exp(Z1 E) - Delta G(P,P)) = ca - cb G(P,P); the Metropolis test
+// |Re(PHASE ratio^N_SUN Gama(new)/Gama(old))| / |Re(PHASE)| > u becomes
+// |Re(z ratio^N_SUN)| > thr |Re(z)| with thr = u / |Gama ratio| and z any positive
+// multiple of PHASE, so the decision needs no division.
 struct BVisit {
-  cplx myexp, expal;        // exp(Z1 E) and exp(Z1 alpha) of the proposal
-  double gr, u3;            // Gama(new)/Gama(old), Metropolis uniform
-  int P, snew, nop, loc;    // site, proposed field, operator, index of the site in the block
+  cplx ca, cb, delta;
+  double thr;
+  int loc;                  // index of the site in the block (its first visit)
+  int P;                    // site
 };
+struct BVisitAux {
+  int snew, nop, pad0, pad1;
+};
+
+// z * 2^-e with |z| in [0.5, 1): an exact rescaling that keeps the running phase in range
+__device__ __forceinline__ cplx renorm_pow2(cplx z) {
+  const double mx = fmax(fabs(z.x), fabs(z.y));
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  return make_double2(ldexp(z.x, -e), ldexp(z.y, -e));
+}
 
 __device__ __forceinline__ void block_sync_lds() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -221,17 +246,21 @@ __device__ __forceinline__ void block_sync_lds() {
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void chain_block_kernel(ChainArgs a) {
-  static_assert(SB_V == 32 && SB_A <= SB_V, "block layout assumes 32 visits and 8 column groups");
-  __shared__ cplx gs[2][SB_V * SB_V];           // g(r, c) at [r + SB_V c], double-buffered
-  __shared__ cplx Xb[SB_K][SB_V], Yb[SB_K][SB_V]; // X(P_r, i), Y(P_r, i) of the round's earlier flips
-  __shared__ cplx Us[SB_A][SB_V], Ws[SB_A][SB_V]; // u_j(r), w_j(r) of the block's flips
+  static_assert(SB_V == 32 && SB_K == 32 && SB_A <= SB_V, "block layout assumes 32 visits, 32 flips per round");
+  __shared__ cplx gs[2][SB_V * SB_V];              // g(r, c) at [r + SB_V c]; gs[1] also stages G[S, A]
+  __shared__ cplx Xb[SB_K][SB_V], Yb[SB_K][SB_V];  // X(P_s, i), Y(P_s, i) of the round's earlier flips
+  __shared__ cplx UW[2 * SB_A * SB_V];             // u_j(s), w_j(s) of the block's flips; also stages Yhat[S, A]
+  __shared__ cplx Cm[SB_K][SB_K], Dm[SB_K][SB_K];  // C(l, j) at Cm[j][l], D(l, j) at Dm[j][l]
   __shared__ BVisit vb[SB_V];
-  __shared__ cplx s_xj[SB_A];
-  __shared__ int s_aloc[SB_A], s_aP[SB_A];
+  __shared__ BVisitAux vx[SB_V];
+  __shared__ cplx s_x[SB_K];
+  __shared__ int s_aP[SB_K], s_aloc[SB_A];
   __shared__ int s_nvis, s_pos0;
   __shared__ unsigned long long s_base;
   __shared__ cplx s_phase;
   __shared__ long long s_nc, s_accn;
+  cplx* const Us = UW;
+  cplx* const Ws = UW + SB_A * SB_V;
 
   const int tid = threadIdx.x;
   const int n = a.n;
@@ -248,17 +277,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __syncthreads();
   const unsigned long long base = s_base;
   int pos = s_pos0, m = 0;
-  cplx ph = s_phase;
-  double iph = 1.0 / ph.x;
-  long long nc = s_nc, accn = s_accn;
-  const long long acc_at_start = accn;
+  cplx z = s_phase;         // the phase up to a positive factor
+  long long nvisits = 0, naccepts = 0;
   const int kmax = a.Kmax < SB_K ? a.Kmax : SB_K;
   bool stopped = false;
+
+  // W = C D^T into gs[1] (SB_K x SB_K, zero outside m x m): W(l, j) = sum_{i >= max(l, j)} C(l, i) D(j, i)
+  auto form_w = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int l = ga, j = gq + 8 * q;
+      cplx acc = make_double2(0.0, 0.0);
+      if (l < m && j < m) {
+        for (int i = (l > j ? l : j); i < m; ++i) acc = cfma(Cm[i][l], Dm[i][j], acc);
+      }
+      gs[1][l + SB_K * j] = acc;
+    }
+  };
+  // Yhat(r, j) = delta(r, P_j) - G(P_j, r) into Y, rows of this thread
+  auto form_yhat = [&]() {
+    for (int r = tid; r < n; r += 256) {
+      for (int j = 0; j < m; ++j) {
+        const int Pj = s_aP[j];
+        const cplx gv = a.G[Pj + (long long)r * n];
+        a.Y[r + (long long)j * n] = make_double2((r == Pj ? 1.0 : 0.0) - gv.x, -gv.y);
+      }
+    }
+  };
 
   while (pos < a.n_op) {
     if (m >= kmax) {
       if (!a.final_round) break;
-      // last round of the slice and the round is full: flush G += X Y^T in place
+      // last round of the slice and the round is full: G += (G(:, A) W) Yhat^T in place
+      form_w();
+      form_yhat();
+      __syncthreads();
+      for (int r = tid; r < n; r += 256) {
+        for (int j = 0; j < m; ++j) {
+          cplx acc = make_double2(0.0, 0.0);
+          for (int l = 0; l < m; ++l) acc = cfma(a.G[r + (long long)s_aP[l] * n], gs[1][l + SB_K * j], acc);
+          a.X[r + (long long)j * n] = acc;
+        }
+      }
+      __syncthreads();
       for (int r = tid; r < n; r += 256) {
         for (int c = 0; c < n; ++c) {
           cplx acc = a.G[r + (long long)c * n];
@@ -273,8 +334,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // ---- the block's proposals, table entries and uniforms (a field changes only at its own visit) ----
     if (tid < SB_V) {
       BVisit w;
-      w.myexp = make_double2(1.0, 0.0); w.expal = make_double2(1.0, 0.0);
-      w.gr = 1.0; w.u3 = 2.0; w.P = 0; w.snew = 0; w.nop = 0;
+      BVisitAux wx;
+      w.ca = make_double2(1.0, 0.0); w.cb = make_double2(0.0, 0.0); w.delta = make_double2(0.0, 0.0);
+      w.thr = 2.0; w.P = 0;
+      wx.snew = 0; wx.nop = 0; wx.pad0 = 0; wx.pad1 = 0;
       bool ok = false;
       if (tid < nb) {
         const int v = pos + tid;
@@ -285,19 +348,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
         ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
         int s_new = -s_old;
+        double u3 = 2.0;
         if (ok) {
           const double* u = a.u + (q - a.u_base);
-          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); w.u3 = u[2]; } else { w.u3 = u[1]; }
+          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
         }
         const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
-        w.myexp = a.op_myexp[ti];
-        w.expal = a.op_expalpha[ti];
-        w.gr = a.gama_ratio[ti];
+        const cplx myexp = a.op_myexp[ti], expal = a.op_expalpha[ti];
+        w.delta = make_double2(myexp.x - 1.0, myexp.y);
+        w.ca = cmul(expal, myexp);
+        w.cb = cmul(expal, w.delta);
+        w.thr = u3 / fabs(a.gama_ratio[ti]);
         w.P = a.op_site[nop];
-        w.snew = s_new;
-        w.nop = nop;
+        wx.snew = s_new;
+        wx.nop = nop;
       }
-      // index of the site in the block: its first visit in the block
       int loc = tid;
       for (int t = 0; t < SB_V; ++t) {
         const int Pt = __builtin_amdgcn_readlane(w.P, t);
@@ -305,23 +370,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
       w.loc = tid < nb ? loc : 0;
       vb[tid] = w;
+      vx[tid] = wx;
       const unsigned long long bad = __ballot(tid < nb && !ok);
       if (tid == 0) s_nvis = bad ? (int)__builtin_ctzll(bad) : nb;
     }
     __syncthreads();
     const int nvis = s_nvis;
-    // rows of X, Y at the block's sites for the round's earlier flips, and G at the block's sites
-    for (int e = tid; e < SB_V * m; e += 256) {
-      const int r = e & (SB_V - 1), i = e >> 5;
-      cplx xv = make_double2(0.0, 0.0), yv = make_double2(0.0, 0.0);
-      if (r < nb) {
-        const int P = vb[r].P;
-        xv = a.X[P + (long long)i * n];
-        yv = a.Y[P + (long long)i * n];
-      }
-      Xb[i][r] = xv;
-      Yb[i][r] = yv;
-    }
+    // every lane l of every wave holds the site index of visit l, read with readlane
+    const int loc_lane = vb[tid & (SB_V - 1)].loc;
+    // ---- G at the block's sites; G[S, A] and Yhat[S, A] for the round's earlier flips ----
     cplx gown[4];
     {
       const int Pa = vb[ga].P;
@@ -329,6 +386,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       for (int q = 0; q < 4; ++q) {
         const int b = gq + 8 * q;
         gown[q] = (ga < nb && b < nb) ? a.G[Pa + (long long)vb[b].P * n] : make_double2(0.0, 0.0);
+      }
+    }
+    for (int e = tid; e < SB_V * m; e += 256) {
+      const int s = e & (SB_V - 1), l = e >> 5;
+      cplx gsa = make_double2(0.0, 0.0), yh = make_double2(0.0, 0.0);
+      if (s < nb) {
+        const int Ps = vb[s].P, Pl = s_aP[l];
+        gsa = a.G[Ps + (long long)Pl * n];
+        const cplx gas = a.G[Pl + (long long)Ps * n];
+        yh = make_double2((Ps == Pl ? 1.0 : 0.0) - gas.x, -gas.y);
+      }
+      gs[1][s + SB_V * l] = gsa;
+      UW[s + SB_V * l] = yh;
+    }
+    __syncthreads();
+    // X[S, j] = sum_{l <= j} G[S, A_l] C(l, j),  Y[S, j] = sum_{l <= j} Yhat[S, A_l] D(l, j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = gq + 8 * q;
+      if (j < m) {
+        cplx ax = make_double2(0.0, 0.0), ay = make_double2(0.0, 0.0);
+        for (int l = 0; l <= j; ++l) {
+          ax = cfma(gs[1][ga + SB_V * l], Cm[j][l], ax);
+          ay = cfma(UW[ga + SB_V * l], Dm[j][l], ay);
+        }
+        Xb[j][ga] = ax;
+        Yb[j][ga] = ay;
       }
     }
     __syncthreads();
@@ -343,118 +427,131 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
     // ---- the visits: decisions from g, rank-1 updates of g on acceptance ----
     int buf = 0, mb = 0, k = 0;
-    BVisit w = vb[0];
-    BVisit wn = vb[1];
-    cplx d = gs[0][w.loc * (SB_V + 1)];
-    for (; k < nvis; ++k) {
-      if (m + mb >= kmax || mb >= SB_A) break;
-      const BVisit wnn = vb[k + 2 < SB_V ? k + 2 : SB_V - 1];
-      const int p = w.loc, pn = wn.loc;
-      const cplx dn_rej = gs[buf][pn * (SB_V + 1)];
-      const cplx delta = make_double2(w.myexp.x - 1.0, w.myexp.y);
-      const cplx dmat = csub(w.myexp, cmul(delta, d));
-      const cplx ratio = cmul(dmat, w.expal);
-      cplx rt = ratio;
-      for (int s = 1; s < a.n_sun; ++s) rt = cmul(rt, ratio);
-      rt = cscale(rt, w.gr);
-      const double weight = fabs(cmul(ph, rt).x * iph);
-      nc += 1;
-      const int accept = __builtin_amdgcn_readfirstlane(weight > w.u3 ? 1 : 0);
-      cplx dn = dn_rej;
-      if (accept) {
-        accn += 1;
-        ph = cmul(ph, cscale(rt, 1.0 / sqrt(cabs2(rt))));
-        iph = 1.0 / ph.x;
-        // x = Delta / (1 + v_P Delta), v_P = 1 - G_cur(P,P)
-        const cplx vp = make_double2(1.0 - d.x, -d.y);
-        const cplx x = cdiv(delta, cadd(make_double2(1.0, 0.0), cmul(vp, delta)));
-        const cplx mx = make_double2(-x.x, -x.y);
-        if (tid == 0) {
-          a.f[(long long)a.slice * a.n_op + w.nop] = (int8_t)w.snew;
-          s_xj[mb] = x;
-          s_aloc[mb] = p;
-          s_aP[mb] = w.P;
-        }
-        const cplx ua = cmul(mx, gs[buf][ga + SB_V * p]);
-        const cplx g_np = gs[buf][pn + SB_V * p];
-        const cplx g_pn = gs[buf][p + SB_V * pn];
-        if (gq == 0) Us[mb][ga] = ua;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int b = gq + 8 * q;
-          const cplx gpb = gs[buf][p + SB_V * b];
-          const cplx wb = make_double2((b == p ? 1.0 : 0.0) - gpb.x, -gpb.y);
-          if (ga == 0) Ws[mb][b] = wb;
-          gown[q] = cfma(ua, wb, gown[q]);
-          gs[buf ^ 1][ga + SB_V * b] = gown[q];
-        }
-        // the next visit's diagonal entry after this update, as its owner computes it
-        dn = cfma(cmul(mx, g_np), make_double2((p == pn ? 1.0 : 0.0) - g_pn.x, -g_pn.y), dn_rej);
-        mb += 1;
-        buf ^= 1;
-        block_sync_lds();
-      }
-      w = wn;
-      wn = wnn;
-      d = dn;
+    BVisit wA = vb[0], wB;
+    cplx d = gs[0][wA.loc * (SB_V + 1)];
+    // One visit with record CUR; loads the next record into NXT and, speculating a
+    // rejection, the next diagonal entry.  Leaves the loop when the round is full.
+#define ALFQ_VISIT(CUR, NXT)                                                                     \
+    {                                                                                          \
+      const int kn = k + 1 < SB_V ? k + 1 : SB_V - 1;                                          \
+      NXT = vb[kn];                                                                            \
+      const int pn = __builtin_amdgcn_readlane(loc_lane, kn);                                  \
+      const cplx dn_rej = gs[buf][pn * (SB_V + 1)];                                            \
+      const int p = CUR.loc;                                                                   \
+      const cplx ratio = make_double2(fma(-CUR.cb.x, d.x, fma(CUR.cb.y, d.y, CUR.ca.x)),         \
+                                      fma(-CUR.cb.x, d.y, fma(-CUR.cb.y, d.x, CUR.ca.y)));       \
+      cplx rt = ratio;                                                                         \
+      if (a.n_sun == 2) rt = cmul(ratio, ratio);                                               \
+      else for (int s = 1; s < a.n_sun; ++s) rt = cmul(rt, ratio);                             \
+      const double lhs = fabs(fma(z.x, rt.x, -z.y * rt.y));                                    \
+      const int accept = __builtin_amdgcn_readfirstlane(lhs > CUR.thr * fabs(z.x) ? 1 : 0);    \
+      ++k;                                                                                     \
+      if (accept) {                                                                            \
+        z = renorm_pow2(cmul(z, rt));                                                          \
+        /* x = Delta / (1 + v_P Delta), v_P = 1 - G_cur(P,P) */                                \
+        const cplx delta = CUR.delta;                                                          \
+        const cplx den = make_double2(1.0 + (1.0 - d.x) * delta.x + d.y * delta.y,             \
+                                      (1.0 - d.x) * delta.y - d.y * delta.x);                  \
+        const double rden = 1.0 / cabs2(den);                                                  \
+        const cplx x = cscale(cmul(delta, cconj(den)), rden);                                  \
+        const cplx mx = make_double2(-x.x, -x.y);                                              \
+        const cplx gcol = gs[buf][ga + SB_V * p];                                              \
+        const cplx g_np = gs[buf][pn + SB_V * p];                                              \
+        const cplx g_pn = gs[buf][p + SB_V * pn];                                              \
+        cplx gpb[4];                                                                           \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) gpb[q] = gs[buf][p + SB_V * (gq + 8 * q)]; \
+        const cplx ua = cmul(mx, gcol);                                                        \
+        cplx wb[4];                                                                            \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                        \
+          const int b = gq + 8 * q;                                                            \
+          wb[q] = make_double2((b == p ? 1.0 : 0.0) - gpb[q].x, -gpb[q].y);                    \
+          gown[q] = cfma(ua, wb[q], gown[q]);                                                  \
+          gs[buf ^ 1][ga + SB_V * b] = gown[q];                                                \
+        }                                                                                      \
+        if (gq == 0) Us[mb * SB_V + ga] = ua;                                                  \
+        if (ga == 0) {                                                                         \
+          _Pragma("unroll") for (int q = 0; q < 4; ++q) Ws[mb * SB_V + gq + 8 * q] = wb[q];    \
+          if (tid == 0) {                                                                      \
+            const BVisitAux ax = vx[k - 1];                                                    \
+            a.f[(long long)a.slice * a.n_op + ax.nop] = (int8_t)ax.snew;                       \
+            s_x[m + mb] = x;                                                                   \
+            s_aP[m + mb] = CUR.P;                                                              \
+            s_aloc[mb] = p;                                                                    \
+          }                                                                                    \
+        }                                                                                      \
+        /* the next visit's diagonal entry after this update, as its owner computes it */      \
+        d = cfma(cmul(mx, g_np), make_double2((p == pn ? 1.0 : 0.0) - g_pn.x, -g_pn.y), dn_rej); \
+        mb += 1;                                                                               \
+        buf ^= 1;                                                                              \
+        block_sync_lds();                                                                      \
+        if (m + mb >= kmax || mb >= SB_A) break;                                               \
+      } else {                                                                                 \
+        d = dn_rej;                                                                            \
+      }                                                                                        \
     }
+    while (k < nvis) {
+      ALFQ_VISIT(wA, wB)
+      if (k >= nvis) break;
+      ALFQ_VISIT(wB, wA)
+    }
+#undef ALFQ_VISIT
+    nvisits += k;
+    naccepts += mb;
     pos += k;
     if (k == nvis && nvis < nb) stopped = true;   // random stream exhausted at this visit
 
-    // ---- full columns X(:, m:m+mb), Y(:, m:m+mb) of the block's flips ----
+    // ---- extend C and D by the block's flips: waves 0-1 take C, waves 2-3 take D;
+    //      four lanes per row l split the sum over i ----
     if (mb > 0) {
-      for (int r = tid; r < n; r += 256) {
-        cplx xc[SB_A], yc[SB_A];
-#pragma unroll
-        for (int j = 0; j < SB_A; ++j) {
-          if (j < mb) {
-            const int Pj = s_aP[j];
-            xc[j] = a.G[r + (long long)Pj * n];
-            yc[j] = a.G[Pj + (long long)r * n];
-          }
-        }
-        for (int i = 0; i < m; ++i) {
-          const cplx xr = a.X[r + (long long)i * n], yr = a.Y[r + (long long)i * n];
-#pragma unroll
-          for (int j = 0; j < SB_A; ++j) {
-            if (j < mb) {
-              const int pj = s_aloc[j];
-              xc[j] = cfma(xr, Yb[i][pj], xc[j]);
-              yc[j] = cfma(Xb[i][pj], yr, yc[j]);
+      const int mat = tid >> 7, l = (tid >> 2) & (SB_K - 1), c4 = tid & 3;
+      cplx(*Mm)[SB_K] = mat ? Dm : Cm;
+      const cplx* Lold = mat ? &Xb[0][0] : &Yb[0][0];   // X(P_j, i) / Y(P_j, i), i < m
+      const cplx* Lnew = mat ? Us : Ws;                  // u_i(p_j) / w_i(p_j), block flips
+      for (int jj = 0; jj < mb; ++jj) {
+        const int j = m + jj, pj = s_aloc[jj];
+        cplx acc = make_double2(0.0, 0.0);
+        int i = l + c4;
+        for (; i < j && i < m; i += 4) acc = cfma(Mm[i][l], Lold[i * SB_V + pj], acc);
+        for (; i < j; i += 4) acc = cfma(Mm[i][l], Lnew[(i - m) * SB_V + pj], acc);
+        acc.x += __shfl_xor(acc.x, 1, 64);
+        acc.y += __shfl_xor(acc.y, 1, 64);
+        acc.x += __shfl_xor(acc.x, 2, 64);
+        acc.y += __shfl_xor(acc.y, 2, 64);
+        if (c4 == 0) {
+          cplx val = make_double2(0.0, 0.0);
+          if (l <= j) {
+            if (mat) {
+              val = make_double2((l == j ? 1.0 : 0.0) - acc.x, -acc.y);
+            } else {
+              const cplx xj = s_x[j];
+              val = cmul(make_double2(-xj.x, -xj.y), make_double2((l == j ? 1.0 : 0.0) + acc.x, acc.y));
             }
           }
+          Mm[j][l] = val;
         }
-#pragma unroll
-        for (int j = 0; j < SB_A; ++j) {
-          if (j < mb) {
-            const int pj = s_aloc[j];
-#pragma unroll
-            for (int i = 0; i < j; ++i) {
-              xc[j] = cfma(xc[i], Ws[i][pj], xc[j]);
-              yc[j] = cfma(Us[i][pj], yc[i], yc[j]);
-            }
-            const cplx x = s_xj[j];
-            xc[j] = cmul(make_double2(-x.x, -x.y), xc[j]);
-            yc[j] = make_double2((r == s_aP[j] ? 1.0 : 0.0) - yc[j].x, -yc[j].y);
-            a.X[r + (long long)(m + j) * n] = xc[j];
-            a.Y[r + (long long)(m + j) * n] = yc[j];
-          }
-        }
+        block_sync_lds();
       }
       m += mb;
     }
-    __syncthreads();   // the new columns are visible to the next block, the LDS tables are free
+    __syncthreads();   // the LDS tables of the block are free
     if (stopped) break;
   }
+  // ---- the round's flush operands: W = C D^T, the site list, Yhat ----
+  form_w();
+  form_yhat();
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a.W[ga + SB_K * (gq + 8 * q)] = gs[1][ga + SB_K * (gq + 8 * q)];
+  if (tid < SB_K) a.amap[tid] = tid < m ? s_aP[tid] : 0;
   if (tid == 0) {
-    st->phase = ph;
-    st->nc = nc;
-    st->acc = accn;
+    st->phase = cscale(z, 1.0 / sqrt(cabs2(z)));
+    st->nc = s_nc + nvisits;
+    st->acc = s_accn + naccepts;
     st->pos = pos;
     st->m = m;
     st->rng_slice_base = base;
     st->rng_pos = base + (unsigned long long)a.uoff[pos];
-    st->slice_acc += (int)(accn - acc_at_start);
+    st->slice_acc += (int)naccepts;
     if (stopped) {
       st->status = 1;
       st->pos = a.n_op;    // do not revisit
