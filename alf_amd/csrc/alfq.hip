@@ -328,7 +328,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
 }
 
 // Choose the number of chain rounds per slice from the largest accepted count seen in
-// the previous sweep (with a margin); reads one integer from the device.
+// the previous sweep; reads one integer from the device.
 int adapt_rounds(alfq_ctx* c) {
   int mx = 0;
   HIP_OK(hipMemcpyAsync(&mx, &c->st->max_slice_acc, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -336,10 +336,10 @@ int adapt_rounds(alfq_ctx* c) {
   const int zero = 0;
   HIP_OK(hipMemcpyAsync(&c->st->max_slice_acc, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
   const int worst = c->n_op / c->K + 1;
-  // enough rounds for the busiest slice seen, with a 10% margin; a slice that needs more
-  // is finished by the last round's in-place flush
-  const int want = (int)std::ceil(1.1 * mx / c->K);
-  c->rounds = std::max(2, std::min(worst, want));
+  // enough rounds for the busiest slice seen; a slice that needs more is finished by the
+  // last round's in-place flush
+  const int want = (mx + c->K - 1) / c->K;
+  c->rounds = std::max(1, std::min(worst, want));
   HIP_OK(hipStreamSynchronize(c->s));
   return ST_NONE;
 }

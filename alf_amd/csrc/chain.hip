@@ -211,13 +211,15 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainArgs a) {
 // starts from X[S, :] = G[S, A] C, Y[S, :] = Yhat[S, :] D and
 // g = G[S, S] + X[S, :] Y[S, :]^T.  At the end of the round the kernel leaves
 // W = C D^T, the site list A and Yhat; the flush is G += (G(:, A) W) Yhat^T, two GEMMs
-// over the whole device.
-constexpr int SB_V = 32;   // visits per block: g is SB_V x SB_V
-constexpr int SB_A = 16;   // accepted flips per block
-constexpr int SB_K = 32;   // accepted flips per round (delayed-update depth)
+// over the whole device.  All round-sized tables are kept zero outside the round's
+// m x m corner, so their sums run in unconditional groups of four.
+constexpr int SB_V = 32;    // visits per block: g is SB_V x SB_V
+constexpr int SB_A = 16;    // accepted flips per block
+constexpr int SB_K = 32;    // accepted flips per round (delayed-update depth)
+constexpr int SB_R = 256;   // visit records prepared at once
 static_assert(SB_K == CHAIN_W_LD, "W is stored with the round's size as leading dimension");
 
-// One visit of a block.  With Delta = exp(Z1 E) - 1 the single-colour ratio is
+// One visit.  With Delta = exp(Z1 E) - 1 the single-colour ratio is
 // exp(Z1 alpha) (exp(Z1 E) - Delta G(P,P)) = ca - cb G(P,P); the Metropolis test
 // |Re(PHASE ratio^N_SUN Gama(new)/Gama(old))| / |Re(PHASE)| > u becomes
 // |Re(z ratio^N_SUN)| > thr |Re(z)| with thr = u / |Gama ratio| and z any positive
@@ -225,11 +227,8 @@ static_assert(SB_K == CHAIN_W_LD, "W is stored with the round's size as leading 
 struct BVisit {
   cplx ca, cb, delta;
   double thr;
-  int loc;                  // index of the site in the block (its first visit)
+  int loc;                  // index of the site in its block (the site's first visit there)
   int P;                    // site
-};
-struct BVisitAux {
-  int snew, nop, pad0, pad1;
 };
 
 // z * 2^-e with |z| in [0.5, 1): an exact rescaling that keeps the running phase in range
@@ -246,16 +245,16 @@ __device__ __forceinline__ void block_sync_lds() {
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void chain_block_kernel(ChainArgs a) {
-  static_assert(SB_V == 32 && SB_K == 32 && SB_A <= SB_V, "block layout assumes 32 visits, 32 flips per round");
+  static_assert(SB_V == 32 && SB_K == 32 && SB_A == 16 && SB_R == 256, "layout of the block kernel");
   __shared__ cplx gs[2][SB_V * SB_V];              // g(r, c) at [r + SB_V c]; gs[1] also stages G[S, A]
   __shared__ cplx Xb[SB_K][SB_V], Yb[SB_K][SB_V];  // X(P_s, i), Y(P_s, i) of the round's earlier flips
   __shared__ cplx UW[2 * SB_A * SB_V];             // u_j(s), w_j(s) of the block's flips; also stages Yhat[S, A]
   __shared__ cplx Cm[SB_K][SB_K], Dm[SB_K][SB_K];  // C(l, j) at Cm[j][l], D(l, j) at Dm[j][l]
-  __shared__ BVisit vb[SB_V];
-  __shared__ BVisitAux vx[SB_V];
+  __shared__ BVisit vr[SB_R];                      // visits [vbase, vbase + SB_R)
+  __shared__ int2 vaux[SB_R];                      // (proposed field, operator)
   __shared__ cplx s_x[SB_K];
   __shared__ int s_aP[SB_K], s_aloc[SB_A];
-  __shared__ int s_nvis, s_pos0;
+  __shared__ int s_pos0, s_stop;
   __shared__ unsigned long long s_base;
   __shared__ cplx s_phase;
   __shared__ long long s_nc, s_accn;
@@ -280,31 +279,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   cplx z = s_phase;         // the phase up to a positive factor
   long long nvisits = 0, naccepts = 0;
   const int kmax = a.Kmax < SB_K ? a.Kmax : SB_K;
-  bool stopped = false;
+  int vbase = 0, stop_at = a.n_op;
+  bool have_records = false, stopped = false;
 
-  // W = C D^T into gs[1] (SB_K x SB_K, zero outside m x m): W(l, j) = sum_{i >= max(l, j)} C(l, i) D(j, i)
-  auto form_w = [&]() {
+  auto zero_cd = [&]() {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int l = ga, j = gq + 8 * q;
-      cplx acc = make_double2(0.0, 0.0);
-      if (l < m && j < m) {
-        for (int i = (l > j ? l : j); i < m; ++i) acc = cfma(Cm[i][l], Dm[i][j], acc);
-      }
-      gs[1][l + SB_K * j] = acc;
+      Cm[gq + 8 * q][ga] = make_double2(0.0, 0.0);
+      Dm[gq + 8 * q][ga] = make_double2(0.0, 0.0);
     }
   };
-  // Yhat(r, j) = delta(r, P_j) - G(P_j, r) into Y, rows of this thread
+  // records of visits [vbase, vbase + SB_R): proposal, table entries, uniforms (a field
+  // changes only at its own visit, so these do not depend on earlier decisions)
+  auto build_records = [&]() {
+    if (tid == 0) s_stop = a.n_op;
+    __syncthreads();
+    const int v = vbase + tid;
+    BVisit w;
+    w.ca = make_double2(1.0, 0.0); w.cb = make_double2(0.0, 0.0); w.delta = make_double2(0.0, 0.0);
+    w.thr = 2.0; w.P = 0; w.loc = 0;
+    int2 ax = make_int2(0, 0);
+    if (v < a.n_op) {
+      const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
+      const int typ = a.op_type[nop];
+      const int s_old = a.f[(long long)a.slice * a.n_op + nop];
+      const unsigned long long q = base + (unsigned long long)a.uoff[v];
+      const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
+      const bool ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
+      int s_new = -s_old;
+      double u3 = 2.0;
+      if (ok) {
+        const double* u = a.u + (q - a.u_base);
+        if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
+      } else {
+        atomicMin(&s_stop, v);
+      }
+      const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
+      const cplx myexp = a.op_myexp[ti], expal = a.op_expalpha[ti];
+      w.delta = make_double2(myexp.x - 1.0, myexp.y);
+      w.ca = cmul(expal, myexp);
+      w.cb = cmul(expal, w.delta);
+      w.thr = u3 / fabs(a.gama_ratio[ti]);
+      w.P = a.op_site[nop];
+      ax = make_int2(s_new, nop);
+    }
+    vr[tid] = w;
+    vaux[tid] = ax;
+    __syncthreads();
+    stop_at = s_stop;
+    have_records = true;
+  };
+  // W = C D^T into gs[1] (zero outside m x m): W(l, j) = sum_i C(l, i) D(j, i)
+  auto form_w = [&]() {
+    cplx acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int i0 = 0; i0 < m; i0 += 4) {
+      cplx cli[4], dv[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        cli[t] = Cm[i0 + t][ga];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dv[t][q] = Dm[i0 + t][gq + 8 * q];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = cfma(cli[t], dv[t][q], acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) gs[1][ga + SB_K * (gq + 8 * q)] = acc[q];
+  };
+  // Yhat(r, j) = delta(r, P_j) - G(P_j, r) into Y for j < m, rows of this thread
   auto form_yhat = [&]() {
     for (int r = tid; r < n; r += 256) {
-      for (int j = 0; j < m; ++j) {
-        const int Pj = s_aP[j];
-        const cplx gv = a.G[Pj + (long long)r * n];
-        a.Y[r + (long long)j * n] = make_double2((r == Pj ? 1.0 : 0.0) - gv.x, -gv.y);
+      for (int j0 = 0; j0 < m; j0 += 8) {
+        cplx gv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) gv[t] = a.G[(j0 + t < m ? s_aP[j0 + t] : 0) + (long long)r * n];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if (j0 + t < m)
+            a.Y[r + (long long)(j0 + t) * n] = make_double2((r == s_aP[j0 + t] ? 1.0 : 0.0) - gv[t].x, -gv[t].y);
       }
     }
   };
 
+  zero_cd();
   while (pos < a.n_op) {
     if (m >= kmax) {
       if (!a.final_round) break;
@@ -327,99 +389,107 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           a.G[r + (long long)c * n] = acc;
         }
       }
-      __syncthreads();
       m = 0;
+      zero_cd();
+      __syncthreads();
     }
-    const int nb = a.n_op - pos < SB_V ? a.n_op - pos : SB_V;
-    // ---- the block's proposals, table entries and uniforms (a field changes only at its own visit) ----
+    int nb = a.n_op - pos < SB_V ? a.n_op - pos : SB_V;
+    if (!have_records || pos + nb > vbase + SB_R) {
+      vbase = pos;
+      build_records();
+    }
+    const int r0 = pos - vbase;
+    int nvis = stop_at - pos < nb ? stop_at - pos : nb;
+    if (nvis < 0) nvis = 0;
+    // ---- block start: the sites' indices in the block (wave 0), G at the block's sites,
+    //      G[S, A] and Yhat[S, A] for the round's earlier flips ----
     if (tid < SB_V) {
-      BVisit w;
-      BVisitAux wx;
-      w.ca = make_double2(1.0, 0.0); w.cb = make_double2(0.0, 0.0); w.delta = make_double2(0.0, 0.0);
-      w.thr = 2.0; w.P = 0;
-      wx.snew = 0; wx.nop = 0; wx.pad0 = 0; wx.pad1 = 0;
-      bool ok = false;
-      if (tid < nb) {
-        const int v = pos + tid;
-        const int nop = a.dir > 0 ? v : a.n_op - 1 - v;
-        const int typ = a.op_type[nop];
-        const int s_old = a.f[(long long)a.slice * a.n_op + nop];
-        const unsigned long long q = base + (unsigned long long)a.uoff[v];
-        const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
-        ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
-        int s_new = -s_old;
-        double u3 = 2.0;
-        if (ok) {
-          const double* u = a.u + (q - a.u_base);
-          if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
-        }
-        const int ti = nop * 25 + (s_old + 2) * 5 + (s_new + 2);
-        const cplx myexp = a.op_myexp[ti], expal = a.op_expalpha[ti];
-        w.delta = make_double2(myexp.x - 1.0, myexp.y);
-        w.ca = cmul(expal, myexp);
-        w.cb = cmul(expal, w.delta);
-        w.thr = u3 / fabs(a.gama_ratio[ti]);
-        w.P = a.op_site[nop];
-        wx.snew = s_new;
-        wx.nop = nop;
-      }
+      const int Pw = vr[r0 + (tid < nb ? tid : 0)].P;
       int loc = tid;
       for (int t = 0; t < SB_V; ++t) {
-        const int Pt = __builtin_amdgcn_readlane(w.P, t);
-        if (t < tid && t < nb && tid < nb && Pt == w.P && loc == tid) loc = t;
+        const int Pt = __builtin_amdgcn_readlane(Pw, t);
+        if (t < tid && t < nb && tid < nb && Pt == Pw && loc == tid) loc = t;
       }
-      w.loc = tid < nb ? loc : 0;
-      vb[tid] = w;
-      vx[tid] = wx;
-      const unsigned long long bad = __ballot(tid < nb && !ok);
-      if (tid == 0) s_nvis = bad ? (int)__builtin_ctzll(bad) : nb;
+      if (tid < nb) vr[r0 + tid].loc = loc;
     }
-    __syncthreads();
-    const int nvis = s_nvis;
-    // every lane l of every wave holds the site index of visit l, read with readlane
-    const int loc_lane = vb[tid & (SB_V - 1)].loc;
-    // ---- G at the block's sites; G[S, A] and Yhat[S, A] for the round's earlier flips ----
     cplx gown[4];
     {
-      const int Pa = vb[ga].P;
+      const int Pa = vr[r0 + (ga < nb ? ga : 0)].P;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int b = gq + 8 * q;
-        gown[q] = (ga < nb && b < nb) ? a.G[Pa + (long long)vb[b].P * n] : make_double2(0.0, 0.0);
+        const cplx gv = a.G[Pa + (long long)vr[r0 + (b < nb ? b : 0)].P * n];
+        gown[q] = (ga < nb && b < nb) ? gv : make_double2(0.0, 0.0);
       }
     }
-    for (int e = tid; e < SB_V * m; e += 256) {
-      const int s = e & (SB_V - 1), l = e >> 5;
-      cplx gsa = make_double2(0.0, 0.0), yh = make_double2(0.0, 0.0);
-      if (s < nb) {
-        const int Ps = vb[s].P, Pl = s_aP[l];
-        gsa = a.G[Ps + (long long)Pl * n];
-        const cplx gas = a.G[Pl + (long long)Ps * n];
-        yh = make_double2((Ps == Pl ? 1.0 : 0.0) - gas.x, -gas.y);
+    {
+      const int s = ga, Ps = vr[r0 + (s < nb ? s : 0)].P;
+      cplx gsa[4], gas[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int l = gq + 8 * q;
+        const int Pl = l < m ? s_aP[l] : 0;
+        gsa[q] = a.G[Ps + (long long)Pl * n];
+        gas[q] = a.G[Pl + (long long)Ps * n];
       }
-      gs[1][s + SB_V * l] = gsa;
-      UW[s + SB_V * l] = yh;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int l = gq + 8 * q;
+        const bool live = s < nb && l < m;
+        const int Pl = l < m ? s_aP[l] : 0;
+        gs[1][s + SB_V * l] = live ? gsa[q] : make_double2(0.0, 0.0);
+        UW[s + SB_V * l] = live ? make_double2((Ps == Pl ? 1.0 : 0.0) - gas[q].x, -gas[q].y) : make_double2(0.0, 0.0);
+      }
     }
     __syncthreads();
-    // X[S, j] = sum_{l <= j} G[S, A_l] C(l, j),  Y[S, j] = sum_{l <= j} Yhat[S, A_l] D(l, j)
+    // every lane l of every wave holds the site index of visit l, read with readlane
+    const int loc_lane = vr[r0 + (ga < nb ? ga : 0)].loc;
+    // X[S, j] = sum_l G[S, A_l] C(l, j),  Y[S, j] = sum_l Yhat[S, A_l] D(l, j)   (zero for j >= m)
+    {
+      cplx ax[4], ay[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int j = gq + 8 * q;
-      if (j < m) {
-        cplx ax = make_double2(0.0, 0.0), ay = make_double2(0.0, 0.0);
-        for (int l = 0; l <= j; ++l) {
-          ax = cfma(gs[1][ga + SB_V * l], Cm[j][l], ax);
-          ay = cfma(UW[ga + SB_V * l], Dm[j][l], ay);
+      for (int q = 0; q < 4; ++q) { ax[q] = make_double2(0.0, 0.0); ay[q] = make_double2(0.0, 0.0); }
+      for (int l0 = 0; l0 < m; l0 += 4) {
+        cplx gl[4], yl[4], cv[4][4], dv[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          gl[t] = gs[1][ga + SB_V * (l0 + t)];
+          yl[t] = UW[ga + SB_V * (l0 + t)];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            cv[t][q] = Cm[gq + 8 * q][l0 + t];
+            dv[t][q] = Dm[gq + 8 * q][l0 + t];
+          }
         }
-        Xb[j][ga] = ax;
-        Yb[j][ga] = ay;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            ax[q] = cfma(gl[t], cv[t][q], ax[q]);
+            ay[q] = cfma(yl[t], dv[t][q], ay[q]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Xb[gq + 8 * q][ga] = ax[q];
+        Yb[gq + 8 * q][ga] = ay[q];
       }
     }
     __syncthreads();
-    for (int i = 0; i < m; ++i) {
-      const cplx xa = Xb[i][ga];
+    for (int i0 = 0; i0 < m; i0 += 4) {
+      cplx xa[4], yv[4][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) gown[q] = cfma(xa, Yb[i][gq + 8 * q], gown[q]);
+      for (int t = 0; t < 4; ++t) {
+        xa[t] = Xb[i0 + t][ga];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yv[t][q] = Yb[i0 + t][gq + 8 * q];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gown[q] = cfma(xa[t], yv[t][q], gown[q]);
+      }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) gs[0][ga + SB_V * (gq + 8 * q)] = gown[q];
@@ -427,14 +497,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
     // ---- the visits: decisions from g, rank-1 updates of g on acceptance ----
     int buf = 0, mb = 0, k = 0;
-    BVisit wA = vb[0], wB;
+    BVisit wA = vr[r0], wB;
     cplx d = gs[0][wA.loc * (SB_V + 1)];
     // One visit with record CUR; loads the next record into NXT and, speculating a
     // rejection, the next diagonal entry.  Leaves the loop when the round is full.
 #define ALFQ_VISIT(CUR, NXT)                                                                     \
     {                                                                                          \
-      const int kn = k + 1 < SB_V ? k + 1 : SB_V - 1;                                          \
-      NXT = vb[kn];                                                                            \
+      const int kn = k + 1 < nb ? k + 1 : nb - 1;                                              \
+      NXT = vr[r0 + kn];                                                                       \
       const int pn = __builtin_amdgcn_readlane(loc_lane, kn);                                  \
       const cplx dn_rej = gs[buf][pn * (SB_V + 1)];                                            \
       const int p = CUR.loc;                                                                   \
@@ -472,8 +542,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if (ga == 0) {                                                                         \
           _Pragma("unroll") for (int q = 0; q < 4; ++q) Ws[mb * SB_V + gq + 8 * q] = wb[q];    \
           if (tid == 0) {                                                                      \
-            const BVisitAux ax = vx[k - 1];                                                    \
-            a.f[(long long)a.slice * a.n_op + ax.nop] = (int8_t)ax.snew;                       \
+            const int2 ax = vaux[r0 + k - 1];                                                  \
+            a.f[(long long)a.slice * a.n_op + ax.y] = (int8_t)ax.x;                            \
             s_x[m + mb] = x;                                                                   \
             s_aP[m + mb] = CUR.P;                                                              \
             s_aloc[mb] = p;                                                                    \
@@ -498,42 +568,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     nvisits += k;
     naccepts += mb;
     pos += k;
-    if (k == nvis && nvis < nb) stopped = true;   // random stream exhausted at this visit
+    if (k == nvis && pos == stop_at && stop_at < a.n_op) stopped = true;   // random stream exhausted here
 
-    // ---- extend C and D by the block's flips: waves 0-1 take C, waves 2-3 take D;
-    //      four lanes per row l split the sum over i ----
+    // ---- extend C and D by the block's flips ----
     if (mb > 0) {
-      const int mat = tid >> 7, l = (tid >> 2) & (SB_K - 1), c4 = tid & 3;
-      cplx(*Mm)[SB_K] = mat ? Dm : Cm;
-      const cplx* Lold = mat ? &Xb[0][0] : &Yb[0][0];   // X(P_j, i) / Y(P_j, i), i < m
-      const cplx* Lnew = mat ? Us : Ws;                  // u_i(p_j) / w_i(p_j), block flips
-      for (int jj = 0; jj < mb; ++jj) {
-        const int j = m + jj, pj = s_aloc[jj];
-        cplx acc = make_double2(0.0, 0.0);
-        int i = l + c4;
-        for (; i < j && i < m; i += 4) acc = cfma(Mm[i][l], Lold[i * SB_V + pj], acc);
-        for (; i < j; i += 4) acc = cfma(Mm[i][l], Lnew[(i - m) * SB_V + pj], acc);
-        acc.x += __shfl_xor(acc.x, 1, 64);
-        acc.y += __shfl_xor(acc.y, 1, 64);
-        acc.x += __shfl_xor(acc.x, 2, 64);
-        acc.y += __shfl_xor(acc.y, 2, 64);
-        if (c4 == 0) {
-          cplx val = make_double2(0.0, 0.0);
-          if (l <= j) {
-            if (mat) {
-              val = make_double2((l == j ? 1.0 : 0.0) - acc.x, -acc.y);
-            } else {
-              const cplx xj = s_x[j];
-              val = cmul(make_double2(-xj.x, -xj.y), make_double2((l == j ? 1.0 : 0.0) + acc.x, acc.y));
+      __syncthreads();   // every wave is past its last visit (gs[1] is reused below)
+      // (1) the part over the round's earlier flips, for every (row l, new column jj):
+      //     pre(l, jj) = sum_{i < m} M(l, i) L(i, p_jj), L = Y(P, i) for C and X(P, i) for D;
+      //     threads: matrix (tid / 128), row l, four columns jj = c4 + 4 t each
+      cplx* const Pre = gs[1];   // [matrix][jj][l]
+      {
+        const int mat = tid >> 7, l = (tid >> 2) & (SB_K - 1), c4 = tid & 3;
+        cplx(*Mm)[SB_K] = mat ? Dm : Cm;
+        cplx(*Lold)[SB_V] = mat ? Xb : Yb;
+        cplx acc[4];
+        int pj[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t] = make_double2(0.0, 0.0);
+          const int jj = c4 + 4 * t;
+          pj[t] = s_aloc[jj < mb ? jj : 0];
+        }
+        for (int i0 = 0; i0 < m; i0 += 4) {
+          cplx mli[4], lv[4][4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            mli[u] = Mm[i0 + u][l];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) lv[u][t] = Lold[i0 + u][pj[t]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = cfma(mli[u], lv[u][t], acc[t]);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) Pre[(mat * SB_A + c4 + 4 * t) * SB_K + l] = acc[t];
+      }
+      __syncthreads();
+      // (2) the in-block recurrence, one thread per (matrix, row), new entries in registers
+      if (tid < 2 * SB_K) {
+        const int mat = tid >> 5, l = tid & (SB_K - 1);
+        const cplx* Lnew = mat ? Us : Ws;   // X(P_j, i) = u_i(p_j) / Y(P_j, i) = w_i(p_j)
+        cplx nm[SB_A];
+#pragma unroll
+        for (int jj = 0; jj < SB_A; ++jj) {
+          nm[jj] = make_double2(0.0, 0.0);
+          if (jj < mb) {
+            const int j = m + jj, pjj = s_aloc[jj];
+            cplx acc = Pre[(mat * SB_A + jj) * SB_K + l];
+#pragma unroll
+            for (int ii = 0; ii < jj; ++ii) acc = cfma(nm[ii], Lnew[ii * SB_V + pjj], acc);
+            if (l <= j) {
+              if (mat) {
+                nm[jj] = make_double2((l == j ? 1.0 : 0.0) - acc.x, -acc.y);
+              } else {
+                const cplx xj = s_x[j];
+                nm[jj] = cmul(make_double2(-xj.x, -xj.y), make_double2((l == j ? 1.0 : 0.0) + acc.x, acc.y));
+              }
             }
           }
-          Mm[j][l] = val;
         }
-        block_sync_lds();
+        cplx(*Mm)[SB_K] = mat ? Dm : Cm;
+#pragma unroll
+        for (int jj = 0; jj < SB_A; ++jj)
+          if (jj < mb) Mm[m + jj][l] = nm[jj];
       }
       m += mb;
     }
-    __syncthreads();   // the LDS tables of the block are free
+    __syncthreads();   // the LDS tables of the block are free, C and D are complete
     if (stopped) break;
   }
   // ---- the round's flush operands: W = C D^T, the site list, Yhat ----
