@@ -206,6 +206,29 @@ struct alfq_ctx {
   KTimer chain_timer;
   int rounds = 1 << 30;                  // chain rounds per slice (adapted per sweep)
   bool uniforms_ready = false;           // a whole sweep's uniforms are already on the device
+  // sweeps replayed as HIP graphs, keyed by the host state they start from
+  struct HostState {
+    cplx* G; cplx* Gtmp; double* u;
+    int var_r, var_l, rounds, K, rng_mode;
+    unsigned long long seed;
+    unsigned stream_id;
+    std::vector<int> var_st, refs;
+    bool operator==(const HostState& o) const {
+      return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
+             K == o.K && rng_mode == o.rng_mode && seed == o.seed && stream_id == o.stream_id &&
+             var_st == o.var_st && refs == o.refs;
+    }
+  };
+  struct SweepGraph {
+    HostState start, end;
+    unsigned long long rng_advance;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<SweepGraph> graphs;
+  bool graphs_enabled = true;
+  bool capturing = false;
+  int eager_sweeps = 0;
 };
 
 namespace {
@@ -213,6 +236,10 @@ namespace {
 int slot_alloc(alfq_ctx* c) {
   for (size_t i = 0; i < c->pool.size(); ++i)
     if (c->pool[i].refs == 0) { c->pool[i].refs = 1; return (int)i; }
+  if (c->capturing) {
+    alfq_set_error("slot_alloc: the UDV pool cannot grow while a sweep is being captured");
+    return -1;
+  }
   UDVSlot sl;
   const size_t nn = (size_t)c->n * c->n;
   if (hipMalloc(&sl.U, nn * sizeof(cplx)) != hipSuccess || hipMalloc(&sl.V, nn * sizeof(cplx)) != hipSuccess ||
@@ -275,6 +302,7 @@ int fill_uniforms(alfq_ctx* c, long long count) {
   if (c->rng_mode == 1) return ST_NONE;   // the tape is resident
   if (c->rng_mode != 2) { alfq_set_error("no random stream: call alfq_rng_set_tape or alfq_rng_set_philox"); return ST_GENERIC; }
   if (count > c->u_cap) {
+    if (c->capturing) { alfq_set_error("uniform buffer cannot grow while a sweep is being captured"); return ST_GENERIC; }
     if (c->u) hipFree(c->u);
     c->u = nullptr;
     HIP_OK(hipMalloc(&c->u, count * sizeof(double)));
@@ -282,7 +310,7 @@ int fill_uniforms(alfq_ctx* c, long long count) {
   }
   c->u_base = c->rng_next;
   c->u_len = count;
-  return launch_philox_fill(c->u, c->u_base, count, c->seed, c->stream_id, c->s);
+  return launch_philox_fill(c->u, c->st, count, c->seed, c->stream_id, c->s);
 }
 
 int metropolis_slice(alfq_ctx* c, int slice, int dir) {
@@ -293,7 +321,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.op_site = c->op_site; a.op_type = c->op_type;
   a.op_myexp = c->myexp; a.op_expalpha = c->expalpha; a.gama_ratio = c->gama_ratio;
   a.n_sun = c->n_sun;
-  a.u = c->u; a.u_base = c->u_base; a.u_len = c->u_len; a.st = c->st;
+  a.u = c->u; a.st = c->st;
   a.W = c->Wk; a.amap = c->amapk;
   a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
   const int worst = c->n_op / c->K + 1;
@@ -554,12 +582,83 @@ int sweep_body(alfq_ctx* c) {
   return ST_NONE;
 }
 
+alfq_ctx::HostState host_state(const alfq_ctx* c) {
+  alfq_ctx::HostState h;
+  h.G = c->G; h.Gtmp = c->Gtmp; h.u = c->u;
+  h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
+  h.seed = c->seed; h.stream_id = c->stream_id;
+  h.var_st = c->var_st;
+  h.refs.reserve(c->pool.size());
+  for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
+  return h;
+}
+
+void set_host_state(alfq_ctx* c, const alfq_ctx::HostState& h) {
+  c->G = h.G; c->Gtmp = h.Gtmp;
+  c->var_r = h.var_r; c->var_l = h.var_l;
+  c->var_st = h.var_st;
+  for (size_t i = 0; i < c->pool.size() && i < h.refs.size(); ++i) c->pool[i].refs = h.refs[i];
+}
+
+void drop_graphs(alfq_ctx* c) {
+  for (auto& g : c->graphs) {
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+  }
+  c->graphs.clear();
+}
+
+// One sweep.  After a first eager sweep (which sizes every buffer), a sweep is captured
+// into a HIP graph the first time its starting host state (buffer parity, UDV slots,
+// rounds, delay, random stream) is seen and replayed with one launch afterwards; the
+// host bookkeeping then jumps to the state the captured sweep ended in.  Anything that
+// fails during capture falls back to eager launches for the rest of the context's life.
+int sweep_graph(alfq_ctx* c) {
+  if (!c->graphs_enabled || c->chain_timer.on || c->eager_sweeps < 1) {
+    const int st = sweep_dev(c);
+    if (st == ST_NONE) c->eager_sweeps += 1;
+    return st;
+  }
+  const alfq_ctx::HostState s0 = host_state(c);
+  for (const auto& g : c->graphs) {
+    if (g.start == s0) {
+      HIP_OK(hipGraphLaunch(g.exec, c->s));
+      set_host_state(c, g.end);
+      c->rng_next += g.rng_advance;
+      return ST_NONE;
+    }
+  }
+  if (c->graphs.size() >= 8) drop_graphs(c);
+  const unsigned long long rn0 = c->rng_next;
+  HIP_OK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int st = sweep_dev(c);
+  c->capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(c->s, &graph);
+  hipGraphExec_t exec = nullptr;
+  bool ok = st == ST_NONE && ee == hipSuccess && graph != nullptr;
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (!ok) {
+    if (graph) hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    set_host_state(c, s0);
+    c->rng_next = rn0;
+    c->graphs_enabled = false;
+    return sweep_dev(c);
+  }
+  c->graphs.push_back({s0, host_state(c), c->rng_next - rn0, graph, exec});
+  HIP_OK(hipGraphLaunch(exec, c->s));
+  return ST_NONE;
+}
+
 int check_status(alfq_ctx* c) {
-  HIP_OK(hipStreamSynchronize(c->s));
+  // copies on the context's own stream: another context may be capturing a sweep meanwhile
   ChainState cs;
   DevScalars ds;
-  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpyAsync(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipMemcpyAsync(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
   if (cs.status == 1) { alfq_set_error("random stream exhausted"); return ST_GENERIC; }
   if (ds.status == 1) { alfq_set_error("Green function contains NaN, calculation is being aborted!"); return ST_GENERIC; }
   if (ds.status == 3) {
@@ -756,6 +855,7 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
 int alfq_ctx_destroy(alfq_ctx* c) {
   if (!c) return ST_NONE;
   if (c->s) hipStreamSynchronize(c->s);
+  drop_graphs(c);
   for (auto& sl : c->pool) { hipFree(sl.U); hipFree(sl.V); hipFree(sl.D); hipFree(sl.sc); }
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
                   c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->Wk, c->amapk, c->st, c->sc,
@@ -926,7 +1026,10 @@ int alfq_rng_set_tape(alfq_ctx* c, const double* u, int64_t n) {
   c->rng_mode = 1;
   c->rng_next = 0;
   const unsigned long long zero = 0;
+  const long long len = n;
   HIP_OK(hipMemcpy(&c->st->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&c->st->u_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&c->st->u_len, &len, sizeof(len), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -940,7 +1043,10 @@ int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
   c->u_len = 0;
   c->u_base = 0;
   const unsigned long long zero = 0;
+  const long long len0 = 0;
   HIP_OK(hipMemcpy(&c->st->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&c->st->u_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&c->st->u_len, &len0, sizeof(len0), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -1027,7 +1133,7 @@ int alfq_sweep(alfq_ctx* c, int32_t nsweep) {
   if (!c) return ST_GENERIC;
   int st;
   for (int i = 0; i < nsweep; ++i) {
-    if ((st = sweep_dev(c))) return st;
+    if ((st = sweep_graph(c))) return st;
     if ((st = adapt_rounds(c))) return st;
   }
   return check_status(c);
@@ -1045,7 +1151,7 @@ int alfq_sweep_many(alfq_ctx* const* ctxs, int32_t count, int32_t nsweep, double
     alfq_ctx* c = ctxs[i];
     int st = ST_NONE;
     for (int k = 0; k < nsweep && st == ST_NONE; ++k) {
-      st = sweep_dev(c);
+      st = sweep_graph(c);
       if (st == ST_NONE) st = adapt_rounds(c);
     }
     if (st == ST_NONE) st = check_status(c);
@@ -1069,7 +1175,7 @@ int alfq_sweep_timed(alfq_ctx* c, int32_t nsweep, double* ms) {
   HIP_OK(hipStreamSynchronize(c->s));
   HIP_OK(hipEventRecord(c->ev0, c->s));
   for (int i = 0; i < nsweep; ++i)
-    if ((st = sweep_dev(c))) return st;
+    if ((st = sweep_graph(c))) return st;
   HIP_OK(hipEventRecord(c->ev1, c->s));
   if ((st = adapt_rounds(c))) return st;
   HIP_OK(hipEventSynchronize(c->ev1));
@@ -1126,6 +1232,14 @@ int alfq_precision_log(alfq_ctx* c, double* out3, int64_t cap, int64_t* count) {
   *count = ds.nlog;
   const int64_t k = std::min<int64_t>(avail, cap);
   if (out3 && k > 0) HIP_OK(hipMemcpy(out3, c->log, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
+int alfq_sweep_graphs(alfq_ctx* c, int32_t enable) {
+  if (!c) return ST_GENERIC;
+  HIP_OK(hipStreamSynchronize(c->s));
+  if (!enable) drop_graphs(c);
+  c->graphs_enabled = enable != 0;
   return ST_NONE;
 }
 

@@ -20,6 +20,8 @@ struct ChainState {
   unsigned long long rng_slice_base;  // rng_pos at the start of the current slice
   int slice_acc;                  // accepted flips in the current slice
   int max_slice_acc;              // largest slice_acc seen since the host last cleared it
+  unsigned long long u_base;      // global index of u[0] in the uniform buffer
+  long long u_len;                // uniforms held by the buffer
 };
 
 struct ChainArgs {
@@ -31,7 +33,7 @@ struct ChainArgs {
   const cplx* op_expalpha;        // n_op x 25: exp(Z1 alpha)
   const double* gama_ratio;       // n_op x 25: Gama(new)/Gama(old)
   int n_sun;
-  const double* u; unsigned long long u_base; long long u_len;
+  const double* u;                // uniforms u[q - st->u_base], q in [st->u_base, st->u_base + st->u_len)
   const int* uoff;                // uniforms consumed before visit v of the slice (n_op + 1 entries)
   int first_round;                // start of the slice: visit 0, rng base = current position
   int final_round;                // last scheduled round: flush in place instead of stopping at K
@@ -51,8 +53,9 @@ struct KTimer {
 int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* timer = nullptr);
 // largest delay depth the submatrix block kernel supports (deeper delays use the generic kernel)
 int chain_max_delay(int n);
-int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,
-                       unsigned stream, hipStream_t s);
+// uniforms of global indices [st->rng_pos, st->rng_pos + count) into u; records the window in st
+int launch_philox_fill(double* u, ChainState* st, long long count, unsigned long long seed, unsigned stream,
+                       hipStream_t s);
 int launch_dvec(const int8_t* f, int n_op, int s0, int ns, int n, const int* site_ptr, const int* site_ops,
                 const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, hipStream_t s);
 int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, hipStream_t s);

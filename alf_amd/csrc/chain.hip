@@ -45,11 +45,17 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
   return c;
 }
 
-// uniforms u[q - base] for global indices q in [base, base + count): one Philox block per index,
-// 53-bit mantissa from two words, u in [0, 1).
-__global__ void philox_fill_kernel(double* __restrict__ u, unsigned long long base, long long count,
+// uniforms u[q - base] for global indices q in [base, base + count), base = st->rng_pos: one
+// Philox block per index, 53-bit mantissa from two words, u in [0, 1).  The window is read
+// from and recorded in device memory, so a captured sweep replays with the stream's position.
+__global__ void philox_fill_kernel(double* __restrict__ u, ChainState* st, long long count,
                                    unsigned long long seed, unsigned stream) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long base = st->rng_pos;
+  if (i == 0) {
+    st->u_base = base;
+    st->u_len = count;
+  }
   if (i >= count) return;
   const unsigned long long q = base + (unsigned long long)i;
   const uint4 c = make_uint4((uint32_t)q, (uint32_t)(q >> 32), stream, 0x414c4651u);
@@ -126,11 +132,11 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainArgs a) {
         unsigned long long q = st->rng_pos;
         const unsigned long long need = (typ == 2) ? 3ull : 2ull;
         int acc = 0;
-        if (q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len) {
+        if (q < st->u_base || q + need > st->u_base + (unsigned long long)st->u_len) {
           st->status = 1;   // random stream exhausted: stop without touching G
           s_stop = 1;
         } else {
-          const double* u = a.u + (q - a.u_base);
+          const double* u = a.u + (q - st->u_base);
           int s_new, k = 0;
           if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); k = 1; } else { s_new = -s_old; }
           // u[k]: T0_proposal = 1.5 > ranf, always taken
@@ -255,7 +261,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __shared__ cplx s_x[SB_K];
   __shared__ int s_aP[SB_K], s_aloc[SB_A];
   __shared__ int s_pos0, s_stop;
-  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_base, s_ubase;
+  __shared__ long long s_ulen;
   __shared__ cplx s_phase;
   __shared__ long long s_nc, s_accn;
   cplx* const Us = UW;
@@ -271,10 +278,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     s_phase = st->phase;
     s_nc = st->nc;
     s_accn = st->acc;
+    s_ubase = st->u_base;
+    s_ulen = st->u_len;
     if (a.first_round) st->slice_acc = 0;
   }
   __syncthreads();
-  const unsigned long long base = s_base;
+  const unsigned long long base = s_base, u_base = s_ubase;
+  const long long u_len = s_ulen;
   int pos = s_pos0, m = 0;
   cplx z = s_phase;         // the phase up to a positive factor
   long long nvisits = 0, naccepts = 0;
@@ -305,11 +315,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const int s_old = a.f[(long long)a.slice * a.n_op + nop];
       const unsigned long long q = base + (unsigned long long)a.uoff[v];
       const unsigned long long need = (unsigned long long)(a.uoff[v + 1] - a.uoff[v]);
-      const bool ok = !(q < a.u_base || q + need > a.u_base + (unsigned long long)a.u_len);
+      const bool ok = !(q < u_base || q + need > u_base + (unsigned long long)u_len);
       int s_new = -s_old;
       double u3 = 2.0;
       if (ok) {
-        const double* u = a.u + (q - a.u_base);
+        const double* u = a.u + (q - u_base);
         if (typ == 2) { s_new = flip_st(s_old, nranf3(u[0])); u3 = u[2]; } else { u3 = u[1]; }
       } else {
         atomicMin(&s_stop, v);
@@ -707,11 +717,11 @@ __global__ void opphase_kernel(const int8_t* __restrict__ f, long long count, in
 
 }  // namespace
 
-int launch_philox_fill(double* u, unsigned long long base, long long count, unsigned long long seed,
-                       unsigned stream, hipStream_t s) {
+int launch_philox_fill(double* u, ChainState* st, long long count, unsigned long long seed, unsigned stream,
+                       hipStream_t s) {
   if (count <= 0) return ST_NONE;
-  hipLaunchKernelGGL(philox_fill_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, u, base, count,
-                     seed, stream);
+  hipLaunchKernelGGL(philox_fill_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, u, st, count, seed,
+                     stream);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }

@@ -220,6 +220,144 @@ void launch_shape(const GemmArgs& g, hipStream_t s) {
   else hipLaunchKernelGGL((gemm_kernel<false, false, WM, WN>), grid, dim3(NT), 0, s, g);
 }
 
+
+// Direct-load split-K kernel for grids that would not fill the device: a workgroup of
+// NW waves owns one 16 x 16 output tile; wave w takes the w-th contiguous quarter
+// (eighth, ...) of K and feeds its MFMAs straight from global memory in the
+// v_mfma_f64_16x16x4f64 operand layout (lane = row + 16 k for A, column + 16 k for B),
+// four or eight k-steps per batch with the next batch in flight.  The NW partial tiles are
+// summed through LDS in wave order.
+template <bool A_CPLX, bool B_CPLX, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
+  __shared__ cplx red[NW > 1 ? NW - 1 : 1][256];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, z = blockIdx.z;
+  int K = g.K;
+  if (g.kdev) {
+    const int kd = *g.kdev;
+    K = kd < K ? kd : K;
+  }
+  if (K <= 0 && g.beta.x == 1.0 && g.beta.y == 0.0) return;  // nothing to add
+  const double* Ar = (const double*)g.A;
+  const cplx* Ac = (const cplx*)g.A;
+  const double* Br = (const double*)g.B;
+  const cplx* Bc = (const cplx*)g.B;
+  if (g.batch > 1) {
+    Ar += z * g.strideA; Ac += z * g.strideA;
+    Br += z * g.strideB; Bc += z * g.strideB;
+  }
+  cplx* C = (cplx*)g.C + (g.batch > 1 ? z * g.strideC : 0);
+  // K range of this wave, in whole k-steps of 4
+  const int ksteps = (K + 3) >> 2;
+  const int per = (ksteps + NW - 1) / NW;
+  const int kb = wave * per * 4, ke0 = (wave + 1) * per * 4;
+  const int ke = ke0 < K ? ke0 : K;
+  const int gi = i0 + lr, gj = j0 + lr;
+  const bool row_ok = gi < g.M, col_ok = gj < g.N;
+
+  auto load_a = [&](int k) -> cplx {
+    cplx v = make_double2(0.0, 0.0);
+    if (row_ok && k < ke) {
+      const int kc = g.amap ? g.amap[k] : k;
+      const long long off = (g.opA == OP_N) ? gi + (long long)kc * g.lda : kc + (long long)gi * g.lda;
+      if (A_CPLX) {
+        v = Ac[off];
+        if (g.opA == OP_C) v.y = -v.y;
+      } else {
+        v.x = Ar[off];
+      }
+    }
+    return v;
+  };
+  auto load_b = [&](int k) -> cplx {
+    cplx v = make_double2(0.0, 0.0);
+    if (col_ok && k < ke) {
+      const int kr = g.bmap ? g.bmap[k] : k;
+      const long long off = (g.opB == OP_N) ? kr + (long long)gj * g.ldb : gj + (long long)kr * g.ldb;
+      if (g.b_cplx) {
+        v = Bc[off];
+        if (g.opB == OP_C) v.y = -v.y;
+      } else {
+        v.x = Br[off];
+      }
+      if (g.bs) v = cmul(v, g.bs[k]);
+    }
+    return v;
+  };
+
+  v4d acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
+  constexpr int U = NW >= 4 ? 8 : 4;   // k-steps per batch: one batch covers a wave's K at Ndim = 256
+  cplx av[U], bv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    av[u] = load_a(kb + 4 * u + lk);
+    bv[u] = load_b(kb + 4 * u + lk);
+  }
+  for (int k0 = kb; k0 < ke; k0 += 4 * U) {
+    cplx an[U], bn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      an[u] = load_a(k0 + 4 * U + 4 * u + lk);
+      bn[u] = load_b(k0 + 4 * U + 4 * u + lk);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc_re, 0, 0, 0);
+      if (B_CPLX) acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].y, acc_im, 0, 0, 0);
+      if (A_CPLX) {
+        if (B_CPLX) acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[u].y, bv[u].y, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, acc_im, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) { av[u] = an[u]; bv[u] = bn[u]; }
+  }
+  // D(i, j) in lane j + 16 (i % 4), register i / 4
+  if (NW > 1) {
+    if (wave > 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[wave - 1][(4 * r + lk) + 16 * lr] = make_double2(acc_re[r], acc_im[r]);
+    }
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int w = 0; w < NW - 1; ++w) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const cplx p = red[w][(4 * r + lk) + 16 * lr];
+        acc_re[r] += p.x;
+        acc_im[r] += p.y;
+      }
+    }
+  }
+  if (!col_ok) return;
+  const cplx csj = g.cs ? g.cs[gj] : make_double2(1.0, 0.0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ri = i0 + 4 * r + lk;
+    if (ri >= g.M) continue;
+    cplx v = make_double2(acc_re[r], (A_CPLX || B_CPLX) ? acc_im[r] : 0.0);
+    v = cmul(g.alpha, v);
+    if (g.rs) v = cmul(g.rs[ri], v);
+    if (g.rs_real) v = cscale(v, g.rs_real[ri]);
+    if (g.cs) v = cmul(v, csj);
+    const long long off = ri + (long long)gj * g.ldc;
+    if (g.beta.x != 0.0 || g.beta.y != 0.0) v = cadd(v, cmul(g.beta, C[off]));
+    C[off] = v;
+  }
+}
+
+template <int NW>
+void launch_direct(const GemmArgs& g, hipStream_t s) {
+  dim3 grid((unsigned)cdivl(g.M, 16), (unsigned)cdivl(g.N, 16), g.batch > 1 ? g.batch : 1);
+  const bool bc = g.b_cplx || g.bs != nullptr;
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_direct_kernel<true, true, NW>), grid, dim3(64 * NW), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_direct_kernel<true, false, NW>), grid, dim3(64 * NW), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_direct_kernel<false, true, NW>), grid, dim3(64 * NW), 0, s, g);
+  else hipLaunchKernelGGL((gemm_direct_kernel<false, false, NW>), grid, dim3(64 * NW), 0, s, g);
+}
 }  // namespace
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,
@@ -255,6 +393,17 @@ static int gemm_tile_shape(int M, int N, int K, int batch) {
 int gemm_launch(const GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return ST_NONE;
   if (g.batch > 65535) { alfq_set_error("gemm_launch: batch too large"); return ST_GENERIC; }
+  const long long b = g.batch > 1 ? g.batch : 1;
+  if (cdivl(g.M, 32) * cdivl(g.N, 32) * b < 256) {
+    // too few 32 x 32 tiles to fill the device: 16 x 16 tiles, K split over up to 8 waves
+    const int ksteps = g.K > 0 ? (g.K + 3) / 4 : 1;
+    if (ksteps >= 32) launch_direct<8>(g, s);
+    else if (ksteps >= 16) launch_direct<4>(g, s);
+    else if (ksteps >= 8) launch_direct<2>(g, s);
+    else launch_direct<1>(g, s);
+    HIP_OK(hipGetLastError());
+    return ST_NONE;
+  }
   switch (gemm_tile_shape(g.M, g.N, g.K, g.batch)) {
     case 0: launch_shape<2, 2>(g, s); break;
     case 1: launch_shape<1, 2>(g, s); break;

@@ -146,6 +146,11 @@ module alfq_mod
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsweep
     end function
+    integer(c_int) function alfq_sweep_graphs(ctx, enable) bind(C, name="alfq_sweep_graphs")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function
     integer(c_int) function alfq_sync(ctx) bind(C, name="alfq_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -31,6 +31,7 @@ EXPORTS = [
     "alfq_init", "alfq_sweep", "alfq_sweep_timed", "alfq_sweep_many",
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
+    "alfq_sweep_graphs",
 ]
 
 
@@ -100,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_control_reset": [P],
         "alfq_precision_log": [P, dp, C.c_int64, C.POINTER(C.c_int64)],
         "alfq_kernel_timing": [P, C.c_int32],
+        "alfq_sweep_graphs": [P, C.c_int32],
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
                       C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32],
@@ -288,6 +290,10 @@ class Engine:
         return {"NC_up": c.nc_up, "ACC_up": c.acc_up, "NCG": c.ncg, "XMEANG": c.xmean_g_sum,
                 "XMAXG": c.xmax_g, "XMAXP": c.xmax_p,
                 "precision_green_mean": (c.xmean_g_sum / c.ncg) if c.ncg else 0.0}
+
+    def sweep_graphs(self, enable=True):
+        """Replay sweeps as captured HIP graphs (default) or launch every kernel from the host."""
+        self._c(self.lib.alfq_sweep_graphs(self.ctx, int(enable)))
 
     def kernel_timing(self, enable=True):
         self._c(self.lib.alfq_kernel_timing(self.ctx, int(enable)))

@@ -146,6 +146,11 @@ int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, cons
  * *total_ms is the summed kernel time of the recorded launches. */
 int alfq_kernel_timing(alfq_ctx* ctx, int32_t enable);
 int alfq_kernel_timing_get(alfq_ctx* ctx, double* total_ms, int64_t* launches);
+/* Whole-sweep replay (no counterpart in the reference, which launches nothing): with
+ * enable != 0 (the default) alfq_sweep, alfq_sweep_timed and alfq_sweep_many capture a
+ * sweep into a HIP graph the first time its starting state is seen and replay it with
+ * one launch afterwards; enable = 0 launches every kernel from the host. */
+int alfq_sweep_graphs(alfq_ctx* ctx, int32_t enable);
 /* Library build identification. */
 const char* alfq_version(void);
 
