@@ -52,8 +52,11 @@ struct UDVSlot {
 constexpr int LOG_CAP = 1 << 16;
 
 __global__ void cgr_scales_kernel(const cplx* __restrict__ DR, const cplx* __restrict__ DL, int n,
-                                  cplx* __restrict__ out) {
+                                  cplx* __restrict__ out, long long cs) {
   // out: [0] 1/DR+  [1] 1/DL+  [2] DR-  [3] DL-   (cgr1_mod.F90:238-268 split at 1.0)
+  DR = chain_ptr(DR, cs);
+  DL = chain_ptr(DL, cs);
+  out = chain_ptr(out, cs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dr = DR[i].x, dl = DL[i].x;
@@ -63,7 +66,9 @@ __global__ void cgr_scales_kernel(const cplx* __restrict__ DR, const cplx* __res
   out[3 * n + i] = make_double2(dl <= 1.0 ? dl : 1.0, 0.0);
 }
 
-__global__ void recip_real_kernel(const cplx* __restrict__ D, int n, cplx* __restrict__ out) {
+__global__ void recip_real_kernel(const cplx* __restrict__ D, int n, cplx* __restrict__ out, long long cs) {
+  D = chain_ptr(D, cs);
+  out = chain_ptr(out, cs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = make_double2(1.0 / D[i].x, 0.0);
 }
@@ -72,8 +77,13 @@ __global__ void recip_real_kernel(const cplx* __restrict__ D, int n, cplx* __res
 // PHASE = Z (main.F90:744-753).  With check == 0 only PHASE = Z.
 __global__ __launch_bounds__(1024) void stabilise_kernel(const cplx* __restrict__ Gf, const cplx* __restrict__ Gw,
                                                          int n, int check, int n_sun, DevScalars* sc,
-                                                         ChainState* st, double* __restrict__ log) {
+                                                         ChainState* st, double* __restrict__ log, long long cs) {
   __shared__ double smax[1024];
+  Gf = chain_ptr(Gf, cs);
+  Gw = chain_ptr(Gw, cs);
+  sc = chain_ptr(sc, cs);
+  st = chain_ptr(st, cs);
+  log = chain_ptr(log, cs);
   __shared__ double ssum[1024];
   __shared__ int snan;
   const int tid = threadIdx.x;
@@ -131,7 +141,8 @@ __global__ __launch_bounds__(1024) void stabilise_kernel(const cplx* __restrict_
   }
 }
 
-__global__ void chain_reset_kernel(ChainState* st) {
+__global__ void chain_reset_kernel(ChainState* st, long long cs) {
+  st = chain_ptr(st, cs);
   st->pos = 0;
   st->m = 0;
   st->rng_slice_base = st->rng_pos;
@@ -146,6 +157,11 @@ struct alfq_ctx {
   bool t_real = true;
   int K = 32;
   hipStream_t s = nullptr;
+  // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
+  int B = 1;
+  long long cs = 0;
+  char* arena = nullptr;
+  int cur = 0;                           // chain addressed by the per-chain accessors
   std::vector<int> stab;                 // Stab_nt(0:NSTM)
   // model
   void* expT = nullptr;
@@ -194,8 +210,11 @@ struct alfq_ctx {
   QRWork qr;
   // random numbers
   int rng_mode = 0;                      // 0: none, 1: tape, 2: philox
-  double* u = nullptr;
+  double* u = nullptr;                   // uniforms in use: the arena's Philox buffer or a tape
+  double* u_arena = nullptr;             // the arena's Philox buffer (u_cap uniforms per chain)
+  double* u_tape = nullptr;              // caller-supplied tape (single chains)
   long long u_cap = 0;
+  long long u_arena_cap = 0;
   long long u_len = 0;
   unsigned long long u_base = 0;
   unsigned long long rng_next = 0;       // host mirror of st->rng_pos
@@ -210,13 +229,10 @@ struct alfq_ctx {
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
     int var_r, var_l, rounds, K, rng_mode;
-    unsigned long long seed;
-    unsigned stream_id;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
-             K == o.K && rng_mode == o.rng_mode && seed == o.seed && stream_id == o.stream_id &&
-             var_st == o.var_st && refs == o.refs;
+             K == o.K && rng_mode == o.rng_mode && var_st == o.var_st && refs == o.refs;
     }
   };
   struct SweepGraph {
@@ -233,23 +249,19 @@ struct alfq_ctx {
 
 namespace {
 
+// GEMM over every chain of the context
+int gemm_c(alfq_ctx* c, GemmArgs g) {
+  g.nchain = c->B;
+  g.cstride = c->cs;
+  return gemm_launch(g, c->s);
+}
+
 int slot_alloc(alfq_ctx* c) {
+  // the slots are placed in the chains' arenas at creation; a sweep needs at most NSTM + 4
   for (size_t i = 0; i < c->pool.size(); ++i)
     if (c->pool[i].refs == 0) { c->pool[i].refs = 1; return (int)i; }
-  if (c->capturing) {
-    alfq_set_error("slot_alloc: the UDV pool cannot grow while a sweep is being captured");
-    return -1;
-  }
-  UDVSlot sl;
-  const size_t nn = (size_t)c->n * c->n;
-  if (hipMalloc(&sl.U, nn * sizeof(cplx)) != hipSuccess || hipMalloc(&sl.V, nn * sizeof(cplx)) != hipSuccess ||
-      hipMalloc(&sl.D, c->n * sizeof(cplx)) != hipSuccess || hipMalloc(&sl.sc, 2 * sizeof(cplx)) != hipSuccess) {
-    alfq_set_error("slot_alloc: hipMalloc failed");
-    return -1;
-  }
-  sl.refs = 1;
-  c->pool.push_back(sl);
-  return (int)c->pool.size() - 1;
+  alfq_set_error("slot_alloc: all UDV slots are in use");
+  return -1;
 }
 
 void slot_release(alfq_ctx* c, int id) {
@@ -269,7 +281,11 @@ void var_bind(alfq_ctx* c, int* v, int slot) {
   *v = slot;
 }
 
-__global__ void udv_identity_kernel(cplx* U, cplx* V, cplx* D, cplx* sc, int n) {
+__global__ void udv_identity_kernel(cplx* U, cplx* V, cplx* D, cplx* sc, int n, long long cs) {
+  U = chain_ptr(U, cs);
+  V = chain_ptr(V, cs);
+  D = chain_ptr(D, cs);
+  sc = chain_ptr(sc, cs);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < (long long)n * n) {
     const int r = (int)(idx % n), cc = (int)(idx / n);
@@ -287,8 +303,8 @@ int udv_reset_dev(alfq_ctx* c, int* v, char side) {
   if (id < 0) return ST_GENERIC;
   UDVSlot& sl = c->pool[id];
   sl.side = side;
-  hipLaunchKernelGGL(udv_identity_kernel, dim3(nblk((long long)c->n * c->n, 256)), dim3(256), 0, c->s, sl.U, sl.V,
-                     sl.D, sl.sc, c->n);
+  hipLaunchKernelGGL(udv_identity_kernel, dim3(nblk((long long)c->n * c->n, 256), 1, c->B), dim3(256), 0, c->s,
+                     sl.U, sl.V, sl.D, sl.sc, c->n, c->cs);
   HIP_OK(hipGetLastError());
   var_bind(c, v, id);
   slot_release(c, id);
@@ -301,16 +317,10 @@ int fill_uniforms(alfq_ctx* c, long long count) {
   // make uniforms [rng_next, rng_next + count) available on the device
   if (c->rng_mode == 1) return ST_NONE;   // the tape is resident
   if (c->rng_mode != 2) { alfq_set_error("no random stream: call alfq_rng_set_tape or alfq_rng_set_philox"); return ST_GENERIC; }
-  if (count > c->u_cap) {
-    if (c->capturing) { alfq_set_error("uniform buffer cannot grow while a sweep is being captured"); return ST_GENERIC; }
-    if (c->u) hipFree(c->u);
-    c->u = nullptr;
-    HIP_OK(hipMalloc(&c->u, count * sizeof(double)));
-    c->u_cap = count;
-  }
+  if (count > c->u_cap) { alfq_set_error("uniform buffer too small for the request"); return ST_GENERIC; }
   c->u_base = c->rng_next;
   c->u_len = count;
-  return launch_philox_fill(c->u, c->st, count, c->seed, c->stream_id, c->s);
+  return launch_philox_fill(c->u, c->st, count, c->B, c->cs, c->s);
 }
 
 int metropolis_slice(alfq_ctx* c, int slice, int dir) {
@@ -323,6 +333,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.n_sun = c->n_sun;
   a.u = c->u; a.st = c->st;
   a.W = c->Wk; a.amap = c->amapk;
+  a.cs = c->cs; a.nchain = c->B;
   a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
   const int worst = c->n_op / c->K + 1;
   const bool reg_kernel = chain_max_delay(c->n) >= c->K;
@@ -331,7 +342,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
     // the final round flushes in place if the slice needs more, so any count is correct
     rounds = std::max(1, std::min(c->rounds, worst));
   } else {
-    hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(1), 0, c->s, c->st);
+    hipLaunchKernelGGL(chain_reset_kernel, dim3(1, 1, c->B), dim3(1), 0, c->s, c->st, c->cs);
     HIP_OK(hipGetLastError());
   }
   for (int r = 0; r < rounds; ++r) {
@@ -344,12 +355,12 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
                               ONE, ZERO);
       gx.amap = c->amapk;
       gx.kdev = &c->st->m;
-      if ((st = gemm_launch(gx, c->s))) return st;
+      if ((st = gemm_c(c, gx))) return st;
     }
     // G += X Y^T with the rank read from the chain state
     GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
     g.kdev = &c->st->m;
-    if ((st = gemm_launch(g, c->s))) return st;
+    if ((st = gemm_c(c, g))) return st;
   }
   c->rng_next += c->per_visit_sum;
   return ST_NONE;
@@ -358,11 +369,18 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
 // Choose the number of chain rounds per slice from the largest accepted count seen in
 // the previous sweep; reads one integer from the device.
 int adapt_rounds(alfq_ctx* c) {
-  int mx = 0;
-  HIP_OK(hipMemcpyAsync(&mx, &c->st->max_slice_acc, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  std::vector<int> mxz(c->B, 0);
+  for (int z = 0; z < c->B; ++z) {
+    int* p = (int*)((char*)&c->st->max_slice_acc + (long long)z * c->cs);
+    HIP_OK(hipMemcpyAsync(&mxz[z], p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  }
   HIP_OK(hipStreamSynchronize(c->s));
+  const int mx = *std::max_element(mxz.begin(), mxz.end());
   const int zero = 0;
-  HIP_OK(hipMemcpyAsync(&c->st->max_slice_acc, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
+  for (int z = 0; z < c->B; ++z) {
+    int* p = (int*)((char*)&c->st->max_slice_acc + (long long)z * c->cs);
+    HIP_OK(hipMemcpyAsync(p, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
+  }
   const int worst = c->n_op / c->K + 1;
   // enough rounds for the busiest slice seen; a slice that needs more is finished by the
   // last round's in-place flush
@@ -376,18 +394,18 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
   int st;
   const int slice = ntau;   // fields column of time slice NTAU1 = NTAU + 1
   if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
-                        c->dv_dconj, c->s)))
+                        c->dv_dconj, c->B, c->cs, c->s)))
     return st;
   // H = exp(-dtau T) G
   GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
                           ONE, ZERO);
-  if ((st = gemm_launch(g1, c->s))) return st;
+  if ((st = gemm_c(c, g1))) return st;
   // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
   GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
                           c->n, ONE, ZERO);
   g2.rs = c->dv_d;
   g2.cs = c->dv_dinv;
-  if ((st = gemm_launch(g2, c->s))) return st;
+  if ((st = gemm_c(c, g2))) return st;
   std::swap(c->G, c->Gtmp);
   if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   return metropolis_slice(c, slice, +1);
@@ -399,18 +417,18 @@ int wrapgrdo_dev(alfq_ctx* c, int ntau) {
   if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   if ((st = metropolis_slice(c, slice, -1))) return st;
   if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
-                        c->dv_dconj, c->s)))
+                        c->dv_dconj, c->B, c->cs, c->s)))
     return st;
   // H = D^-1 G D exp(-dtau T)   (Op_Wrapdo type 1 with the new fields, then Hop_mod_mmthl)
   GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->G, c->n, OP_N, true, c->expT, c->n, OP_N, !c->t_real, c->H, c->n,
                           ONE, ZERO);
   g1.rs = c->dv_dinv;
   g1.bs = c->dv_d;
-  if ((st = gemm_launch(g1, c->s))) return st;
+  if ((st = gemm_c(c, g1))) return st;
   // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
   GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
                           c->n, ONE, ZERO);
-  if ((st = gemm_launch(g2, c->s))) return st;
+  if ((st = gemm_c(c, g2))) return st;
   std::swap(c->G, c->Gtmp);
   return ST_NONE;
 }
@@ -426,7 +444,7 @@ int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
   const cplx* cur = src.U;
   if (ns > 0) {
     if ((st = launch_dvec(c->f, c->n_op, nt_lo, ns, n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
-                          c->dv_dconj, c->s)))
+                          c->dv_dconj, c->B, c->cs, c->s)))
       return st;
   }
   cplx* bufs[2] = {c->U0, c->U1};
@@ -438,7 +456,7 @@ int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
     GemmArgs g = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, cur, n, OP_N, true, bufs[which], n, ONE, ZERO);
     if (left) g.bs = c->dv_dconj + (long long)sl * n;
     else g.rs = c->dv_d + (long long)sl * n;
-    if ((st = gemm_launch(g, c->s))) return st;
+    if ((st = gemm_c(c, g))) return st;
     cur = bufs[which];
     which ^= 1;
   }
@@ -463,20 +481,22 @@ int cgr_dev(alfq_ctx* c, int nvar, int slot_r, int slot_l, int check) {
   cplx* lbinv = c->cscal + n;
   cplx* rsm = c->cscal + 2 * n;
   cplx* lsm = c->cscal + 3 * n;
-  hipLaunchKernelGGL(cgr_scales_kernel, dim3(nblk(n, 256)), dim3(256), 0, c->s, R.D, Lf.D, n, c->cscal);
+  hipLaunchKernelGGL(cgr_scales_kernel, dim3(nblk(n, 256), 1, c->B), dim3(256), 0, c->s, R.D, Lf.D, n, c->cscal,
+                     c->cs);
   HIP_OK(hipGetLastError());
   // TP = DR+^-1 (UR^H UL) DL+^-1
   GemmArgs g1 = gemm_args(n, n, n, R.U, n, OP_C, true, Lf.U, n, OP_N, true, c->TP, n, ONE, ZERO);
   g1.rs = rbinv;
   g1.cs = lbinv;
-  if ((st = gemm_launch(g1, c->s))) return st;
+  if ((st = gemm_c(c, g1))) return st;
   // TP += DR- (VR VL) DL-
   GemmArgs g2 = gemm_args(n, n, n, R.V, n, OP_N, true, Lf.V, n, OP_N, true, c->TP, n, ONE, ONE);
   g2.rs = rsm;
   g2.cs = lsm;
-  if ((st = gemm_launch(g2, c->s))) return st;
+  if ((st = gemm_c(c, g2))) return st;
   if ((st = qr_factor_cgr(c->qr, c->TP, nvar, R.sc + 1, Lf.sc + 1, c->Qc, c->DUP, &c->sc->z_cgr, c->s))) return st;
-  hipLaunchKernelGGL(recip_real_kernel, dim3(nblk(n, 256)), dim3(256), 0, c->s, c->DUP, n, c->DUPinv);
+  hipLaunchKernelGGL(recip_real_kernel, dim3(nblk(n, 256), 1, c->B), dim3(256), 0, c->s, c->DUP, n, c->DUPinv,
+                     c->cs);
   HIP_OK(hipGetLastError());
   if ((st = tri_inverse_dev(c->qr, c->qr.Rt, c->qr.Rinv, c->s))) return st;
   if (nvar == 1) {
@@ -484,33 +504,34 @@ int cgr_dev(alfq_ctx* c, int nvar, int slot_r, int slot_l, int check) {
     GemmArgs a = gemm_args(n, n, n, c->Qc, n, OP_C, true, R.U, n, OP_C, true, c->Xc, n, ONE, ZERO);
     a.bs = rbinv;
     a.rs = c->DUPinv;
-    if ((st = gemm_launch(a, c->s))) return st;
+    if ((st = gemm_c(c, a))) return st;
     // Yc = Rt^-1 Xc
     GemmArgs b = gemm_args(n, n, n, c->qr.Rinv, n, OP_N, true, c->Xc, n, OP_N, true, c->Yc, n, ONE, ZERO);
-    if ((st = gemm_launch(b, c->s))) return st;
+    if ((st = gemm_c(c, b))) return st;
     // G = UL DL+^-1 P Yc
     GemmArgs d = gemm_args(n, n, n, Lf.U, n, OP_N, true, c->Yc, n, OP_N, true, c->Gtmp, n, ONE, ZERO);
     d.bmap = c->qr.pinv;
     d.bs = lbinv;
-    if ((st = gemm_launch(d, c->s))) return st;
+    if ((st = gemm_c(c, d))) return st;
   } else {
     // Xc = UL DL+^-1 Q DUP^-1
     GemmArgs a = gemm_args(n, n, n, Lf.U, n, OP_N, true, c->Qc, n, OP_N, true, c->Xc, n, ONE, ZERO);
     a.bs = lbinv;
     a.cs = c->DUPinv;
-    if ((st = gemm_launch(a, c->s))) return st;
+    if ((st = gemm_c(c, a))) return st;
     // Yc = Xc Rt^-H
     GemmArgs b = gemm_args(n, n, n, c->Xc, n, OP_N, true, c->qr.Rinv, n, OP_C, true, c->Yc, n, ONE, ZERO);
-    if ((st = gemm_launch(b, c->s))) return st;
+    if ((st = gemm_c(c, b))) return st;
     // G = Yc P^T DR+^-1 UR^H
     GemmArgs d = gemm_args(n, n, n, c->Yc, n, OP_N, true, R.U, n, OP_C, true, c->Gtmp, n, ONE, ZERO);
     d.amap = c->qr.pinv;
     d.bs = rbinv;
-    if ((st = gemm_launch(d, c->s))) return st;
+    if ((st = gemm_c(c, d))) return st;
   }
-  if ((st = launch_opphase(c->f, (long long)c->n_op * c->L, c->n_op, c->phasefac, &c->sc->opph, c->s))) return st;
-  hipLaunchKernelGGL(stabilise_kernel, dim3(1), dim3(1024), 0, c->s, c->Gtmp, c->G, n, check, c->n_sun, c->sc,
-                     c->st, c->log);
+  if ((st = launch_opphase(c->f, (long long)c->n_op * c->L, c->n_op, c->phasefac, &c->sc->opph, c->B, c->cs, c->s)))
+    return st;
+  hipLaunchKernelGGL(stabilise_kernel, dim3(1, 1, c->B), dim3(1024), 0, c->s, c->Gtmp, c->G, n, check, c->n_sun,
+                     c->sc, c->st, c->log, c->cs);
   HIP_OK(hipGetLastError());
   std::swap(c->G, c->Gtmp);
   return ST_NONE;
@@ -586,7 +607,6 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   alfq_ctx::HostState h;
   h.G = c->G; h.Gtmp = c->Gtmp; h.u = c->u;
   h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
-  h.seed = c->seed; h.stream_id = c->stream_id;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -652,18 +672,36 @@ int sweep_graph(alfq_ctx* c) {
   return ST_NONE;
 }
 
+// chain z's copy of a per-chain buffer
+template <class T>
+T* at_chain(const alfq_ctx* c, T* p, int z) {
+  return (T*)((char*)p + (long long)z * c->cs);
+}
+template <class T>
+T* at(const alfq_ctx* c, T* p) {
+  return at_chain(c, p, c->cur);
+}
+
 int check_status(alfq_ctx* c) {
   // copies on the context's own stream: another context may be capturing a sweep meanwhile
-  ChainState cs;
-  DevScalars ds;
-  HIP_OK(hipMemcpyAsync(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost, c->s));
-  HIP_OK(hipMemcpyAsync(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost, c->s));
+  std::vector<ChainState> cs(c->B);
+  std::vector<DevScalars> ds(c->B);
+  for (int z = 0; z < c->B; ++z) {
+    HIP_OK(hipMemcpyAsync(&cs[z], at_chain(c, c->st, z), sizeof(ChainState), hipMemcpyDeviceToHost, c->s));
+    HIP_OK(hipMemcpyAsync(&ds[z], at_chain(c, c->sc, z), sizeof(DevScalars), hipMemcpyDeviceToHost, c->s));
+  }
   HIP_OK(hipStreamSynchronize(c->s));
-  if (cs.status == 1) { alfq_set_error("random stream exhausted"); return ST_GENERIC; }
-  if (ds.status == 1) { alfq_set_error("Green function contains NaN, calculation is being aborted!"); return ST_GENERIC; }
-  if (ds.status == 3) {
-    alfq_set_error("XMAX is exceeding the threshold of 10 for G difference! Try with smaller Nwrap or dtau.");
-    return ST_UNSTABLE_MATRIX;
+  for (int z = 0; z < c->B; ++z) {
+    const std::string who = c->B > 1 ? "chain " + std::to_string(z) + ": " : "";
+    if (cs[z].status == 1) { alfq_set_error(who + "random stream exhausted"); return ST_GENERIC; }
+    if (ds[z].status == 1) {
+      alfq_set_error(who + "Green function contains NaN, calculation is being aborted!");
+      return ST_GENERIC;
+    }
+    if (ds[z].status == 3) {
+      alfq_set_error(who + "XMAX is exceeding the threshold of 10 for G difference! Try with smaller Nwrap or dtau.");
+      return ST_UNSTABLE_MATRIX;
+    }
   }
   return ST_NONE;
 }
@@ -694,9 +732,13 @@ extern "C" {
 const char* alfq_last_error(void) { return g_last_error.c_str(); }
 const char* alfq_version(void) { return "alfq 0.1 gfx950 fp64"; }
 
-int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
-  if (!d || !out) { alfq_set_error("null argument"); return ST_GENERIC; }
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Checks one descriptor on its own and against the batch's first chain (shapes, operator
+// sites and types must agree so that the chains advance in lockstep).
+int check_desc(const alfq_model_desc* d, const alfq_model_desc* d0) {
   if (d->n_fl != 1) { alfq_set_error("N_FL must be 1"); return ST_HAMILTONIAN; }
   if (d->ndim < 1 || d->ltrot < 1 || d->nwrap < 1 || d->n_op < 0 || d->n_sun < 1) {
     alfq_set_error("invalid model sizes");
@@ -709,40 +751,33 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
       return ST_HAMILTONIAN;
     }
   }
-  alfq_ctx* c = new alfq_ctx();
-  c->n = d->ndim; c->L = d->ltrot; c->W = d->nwrap; c->n_op = d->n_op; c->n_sun = d->n_sun;
-  c->t_real = d->t_is_real != 0;
-  c->nstm = (c->L % c->W == 0) ? c->L / c->W : c->L / c->W + 1;
-  c->stab.assign(c->nstm + 1, 0);
-  for (int k = 1; k < c->nstm; ++k) c->stab[k] = c->W * k;
-  c->stab[c->nstm] = c->L;
-  c->K = std::max(1, std::min(chain_max_delay(d->ndim) > 0 ? chain_max_delay(d->ndim) : 32, std::max(1, d->n_op)));
-  const int n = c->n;
-  const size_t nn = (size_t)n * n;
-  auto fail = [&](const char* what) {
-    alfq_set_error(std::string("alfq_ctx_create: ") + what);
-    alfq_ctx_destroy(c);
-    return ST_GENERIC;
-  };
-  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail("stream");
-  const size_t tb = nn * (c->t_real ? sizeof(double) : sizeof(cplx));
-  if (hipMalloc(&c->expT, tb) || hipMalloc(&c->expTinv, tb)) return fail("hipMalloc");
-  if (hipMemcpy(c->expT, d->expT, tb, hipMemcpyHostToDevice) ||
-      hipMemcpy(c->expTinv, d->expT_inv, tb, hipMemcpyHostToDevice))
-    return fail("copy exp(T)");
-  // operator tables
-  const int no = c->n_op;
-  std::vector<int> site(no), typ(no), sptr(n + 1, 0), sops(no);
-  std::vector<cplx> eexp(5 * no), myexp(25 * no), expal(25 * no), phf(5 * no);
-  std::vector<double> gr(25 * no);
+  if (d0 && d != d0) {
+    if (d->ndim != d0->ndim || d->ltrot != d0->ltrot || d->nwrap != d0->nwrap || d->n_sun != d0->n_sun ||
+        d->n_op != d0->n_op || d->t_is_real != d0->t_is_real) {
+      alfq_set_error("batch: the chains differ in Ndim, Ltrot, Nwrap, N_SUN, the operator count or the type of T");
+      return ST_HAMILTONIAN;
+    }
+    for (int i = 0; i < d->n_op; ++i)
+      if (d->op_P[i] != d0->op_P[i] || d->op_type[i] != d0->op_type[i]) {
+        alfq_set_error("batch: the chains' operators act on different sites or have different field types");
+        return ST_HAMILTONIAN;
+      }
+  }
+  return ST_NONE;
+}
+
+// Per-operator tables of one chain's couplings: E_exp (5 per op), Z1-exponentials and
+// Gama ratios (25 per op: old x new field), Op_phase factors (5 per op).
+void coupling_tables(const alfq_model_desc* d, std::vector<cplx>& eexp, std::vector<cplx>& myexp,
+                     std::vector<cplx>& expal, std::vector<cplx>& phf, std::vector<double>& gr) {
+  const int no = d->n_op;
+  eexp.assign(5 * no, cplx{}); myexp.assign(25 * no, cplx{}); expal.assign(25 * no, cplx{});
+  phf.assign(5 * no, cplx{}); gr.assign(25 * no, 0.0);
   for (int i = 0; i < no; ++i) {
-    site[i] = d->op_P[i] - 1;
-    typ[i] = d->op_type[i];
-    sptr[site[i] + 1] += 1;
     const std::complex<double> g(d->op_g[2 * i], d->op_g[2 * i + 1]);
     const std::complex<double> al(d->op_alpha[2 * i], d->op_alpha[2 * i + 1]);
     const double E = d->op_E[i];
-    const int T = typ[i];
+    const int T = d->op_type[i];
     for (int sp = -2; sp <= 2; ++sp) {
       std::complex<double> e(1.0, 0.0);
       if (sp != 0 && !(T == 1 && (sp == 2 || sp == -2))) {
@@ -777,6 +812,89 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
       }
     }
   }
+}
+
+// Places every per-chain buffer of c in one chain's arena (sizes it when ar.base is null).
+void place_chain(alfq_ctx* c, ArenaLayout& ar, int nslot) {
+  const int n = c->n, no = std::max(c->n_op, 1);
+  const size_t nn = (size_t)n * n;
+  const size_t tsz = c->t_real ? sizeof(double) : sizeof(cplx);
+  c->expT = ar.take<char>(nn * tsz);
+  c->expTinv = ar.take<char>(nn * tsz);
+  c->eexp = ar.take<cplx>(5 * (size_t)no);
+  c->myexp = ar.take<cplx>(25 * (size_t)no);
+  c->expalpha = ar.take<cplx>(25 * (size_t)no);
+  c->phasefac = ar.take<cplx>(5 * (size_t)no);
+  c->gama_ratio = ar.take<double>(25 * (size_t)no);
+  c->f = ar.take<int8_t>((size_t)no * c->L);
+  c->G = ar.take<cplx>(nn);
+  c->Gtmp = ar.take<cplx>(nn);
+  c->H = ar.take<cplx>(nn);
+  c->X = ar.take<cplx>((size_t)n * CHAIN_KMAX);
+  c->Y = ar.take<cplx>((size_t)n * CHAIN_KMAX);
+  c->Wk = ar.take<cplx>((size_t)CHAIN_KMAX * CHAIN_KMAX);
+  c->amapk = ar.take<int>(CHAIN_KMAX);
+  c->st = ar.take<ChainState>(1);
+  c->sc = ar.take<DevScalars>(1);
+  c->log = ar.take<double>(3 * (size_t)LOG_CAP);
+  c->dv_d = ar.take<cplx>((size_t)n * c->W);
+  c->dv_dinv = ar.take<cplx>((size_t)n * c->W);
+  c->dv_dconj = ar.take<cplx>((size_t)n * c->W);
+  c->U0 = ar.take<cplx>(nn);
+  c->U1 = ar.take<cplx>(nn);
+  c->TP = ar.take<cplx>(nn);
+  c->Qc = ar.take<cplx>(nn);
+  c->DUP = ar.take<cplx>(n);
+  c->DUPinv = ar.take<cplx>(n);
+  c->Xc = ar.take<cplx>(nn);
+  c->Yc = ar.take<cplx>(nn);
+  c->cscal = ar.take<cplx>(4 * (size_t)n);
+  c->u_arena = ar.take<double>((size_t)std::max<long long>(c->u_cap, 1));
+  qr_work_place(c->qr, n, ar);
+  c->pool.assign(nslot, UDVSlot{});
+  for (auto& sl : c->pool) {
+    sl.U = ar.take<cplx>(nn);
+    sl.V = ar.take<cplx>(nn);
+    sl.D = ar.take<cplx>(n);
+    sl.sc = ar.take<cplx>(2);
+  }
+}
+
+template <class T>
+hipError_t put_chain(alfq_ctx* c, int z, T* dst, const T* src, size_t count) {
+  return hipMemcpy((char*)dst + (long long)z * c->cs, src, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
+  if (!descs || !out || nchain < 1) { alfq_set_error("null argument or empty batch"); return ST_GENERIC; }
+  *out = nullptr;
+  int st;
+  for (int z = 0; z < nchain; ++z)
+    if ((st = check_desc(&descs[z], &descs[0]))) return st;
+  const alfq_model_desc* d = &descs[0];
+  alfq_ctx* c = new alfq_ctx();
+  c->B = nchain;
+  c->n = d->ndim; c->L = d->ltrot; c->W = d->nwrap; c->n_op = d->n_op; c->n_sun = d->n_sun;
+  c->t_real = d->t_is_real != 0;
+  c->nstm = (c->L % c->W == 0) ? c->L / c->W : c->L / c->W + 1;
+  c->stab.assign(c->nstm + 1, 0);
+  for (int k = 1; k < c->nstm; ++k) c->stab[k] = c->W * k;
+  c->stab[c->nstm] = c->L;
+  c->K = std::max(1, std::min(chain_max_delay(d->ndim) > 0 ? chain_max_delay(d->ndim) : 32, std::max(1, d->n_op)));
+  const int n = c->n, no = c->n_op;
+  auto fail = [&](const char* what) {
+    alfq_set_error(std::string("alfq_ctx_create: ") + what);
+    alfq_ctx_destroy(c);
+    return ST_GENERIC;
+  };
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+  // shared operator structure: sites, types, per-site operator lists, uniform offsets
+  std::vector<int> site(no), typ(no), sptr(n + 1, 0), sops(no);
+  for (int i = 0; i < no; ++i) {
+    site[i] = d->op_P[i] - 1;
+    typ[i] = d->op_type[i];
+    sptr[site[i] + 1] += 1;
+  }
   for (int i = 0; i < n; ++i) sptr[i + 1] += sptr[i];
   {
     std::vector<int> fillp(sptr.begin(), sptr.end() - 1);
@@ -785,63 +903,63 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
   c->h_op_type = typ;
   c->per_visit_sum = 0;
   for (int i = 0; i < no; ++i) c->per_visit_sum += (typ[i] == 2) ? 3 : 2;
-  {
-    std::vector<int> up(no + 1, 0), dn(no + 1, 0);
-    for (int v = 0; v < no; ++v) {
-      up[v + 1] = up[v] + ((typ[v] == 2) ? 3 : 2);
-      dn[v + 1] = dn[v] + ((typ[no - 1 - v] == 2) ? 3 : 2);
-    }
-    if (hipMalloc(&c->uoff_up, (no + 1) * sizeof(int)) || hipMalloc(&c->uoff_dn, (no + 1) * sizeof(int)) ||
-        hipMemcpy(c->uoff_up, up.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice) ||
-        hipMemcpy(c->uoff_dn, dn.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice))
-      return fail("uniform offsets");
+  c->u_cap = c->u_arena_cap = 2LL * c->L * c->per_visit_sum;
+  std::vector<int> up(no + 1, 0), dn(no + 1, 0);
+  for (int v = 0; v < no; ++v) {
+    up[v + 1] = up[v] + ((typ[v] == 2) ? 3 : 2);
+    dn[v + 1] = dn[v] + ((typ[no - 1 - v] == 2) ? 3 : 2);
   }
   const int nop1 = std::max(no, 1);
-  if (hipMalloc(&c->op_site, nop1 * sizeof(int)) || hipMalloc(&c->op_type, nop1 * sizeof(int)) ||
-      hipMalloc(&c->site_ptr, (n + 1) * sizeof(int)) || hipMalloc(&c->site_ops, nop1 * sizeof(int)) ||
-      hipMalloc(&c->eexp, 5 * nop1 * sizeof(cplx)) || hipMalloc(&c->myexp, 25 * nop1 * sizeof(cplx)) ||
-      hipMalloc(&c->expalpha, 25 * nop1 * sizeof(cplx)) || hipMalloc(&c->phasefac, 5 * nop1 * sizeof(cplx)) ||
-      hipMalloc(&c->gama_ratio, 25 * nop1 * sizeof(double)))
+  if (hipMalloc(&c->uoff_up, (no + 1) * sizeof(int)) || hipMalloc(&c->uoff_dn, (no + 1) * sizeof(int)) ||
+      hipMalloc(&c->op_site, nop1 * sizeof(int)) || hipMalloc(&c->op_type, nop1 * sizeof(int)) ||
+      hipMalloc(&c->site_ptr, (n + 1) * sizeof(int)) || hipMalloc(&c->site_ops, nop1 * sizeof(int)))
     return fail("hipMalloc tables");
-  if (no > 0) {
-    hipMemcpy(c->op_site, site.data(), no * sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(c->op_type, typ.data(), no * sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(c->site_ops, sops.data(), no * sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(c->eexp, eexp.data(), 5 * no * sizeof(cplx), hipMemcpyHostToDevice);
-    hipMemcpy(c->myexp, myexp.data(), 25 * no * sizeof(cplx), hipMemcpyHostToDevice);
-    hipMemcpy(c->expalpha, expal.data(), 25 * no * sizeof(cplx), hipMemcpyHostToDevice);
-    hipMemcpy(c->phasefac, phf.data(), 5 * no * sizeof(cplx), hipMemcpyHostToDevice);
-    hipMemcpy(c->gama_ratio, gr.data(), 25 * no * sizeof(double), hipMemcpyHostToDevice);
+  if (hipMemcpy(c->uoff_up, up.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->uoff_dn, dn.data(), (no + 1) * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->site_ptr, sptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice))
+    return fail("copy tables");
+  if (no > 0 && (hipMemcpy(c->op_site, site.data(), no * sizeof(int), hipMemcpyHostToDevice) ||
+                 hipMemcpy(c->op_type, typ.data(), no * sizeof(int), hipMemcpyHostToDevice) ||
+                 hipMemcpy(c->site_ops, sops.data(), no * sizeof(int), hipMemcpyHostToDevice)))
+    return fail("copy tables");
+  // the chains' arenas: one allocation, chain z at z * cs bytes
+  const int nslot = c->nstm + 6;
+  ArenaLayout sizing;
+  place_chain(c, sizing, nslot);
+  c->cs = (long long)((sizing.off + 4095) & ~(size_t)4095);
+  if (hipMalloc(&c->arena, (size_t)c->cs * nchain)) return fail("hipMalloc arena");
+  if (hipMemset(c->arena, 0, (size_t)c->cs * nchain)) return fail("clear arena");
+  ArenaLayout ar;
+  ar.base = c->arena;
+  place_chain(c, ar, nslot);
+  c->qr.nchain = nchain;
+  c->qr.cs = c->cs;
+  c->u = c->u_arena;
+  const size_t nn = (size_t)n * n;
+  const size_t tsz = c->t_real ? sizeof(double) : sizeof(cplx);
+  for (int z = 0; z < nchain; ++z) {
+    const alfq_model_desc* dz = &descs[z];
+    std::vector<cplx> eexp, myexp, expal, phf;
+    std::vector<double> gr;
+    coupling_tables(dz, eexp, myexp, expal, phf, gr);
+    ChainState cs{};
+    cs.phase = make_double2(1.0, 0.0);
+    DevScalars ds{};
+    ds.z_cgr = make_double2(1.0, 0.0);
+    ds.opph = make_double2(1.0, 0.0);
+    std::vector<int8_t> f1((size_t)std::max(no, 1) * c->L, 1);
+    if (put_chain(c, z, (char*)c->expT, (const char*)dz->expT, nn * tsz) ||
+        put_chain(c, z, (char*)c->expTinv, (const char*)dz->expT_inv, nn * tsz) ||
+        put_chain(c, z, c->st, &cs, 1) || put_chain(c, z, c->sc, &ds, 1) ||
+        put_chain(c, z, c->f, f1.data(), f1.size()))
+      return fail("copy chain state");
+    if (no > 0 && (put_chain(c, z, c->eexp, eexp.data(), eexp.size()) ||
+                   put_chain(c, z, c->myexp, myexp.data(), myexp.size()) ||
+                   put_chain(c, z, c->expalpha, expal.data(), expal.size()) ||
+                   put_chain(c, z, c->phasefac, phf.data(), phf.size()) ||
+                   put_chain(c, z, c->gama_ratio, gr.data(), gr.size())))
+      return fail("copy coupling tables");
   }
-  hipMemcpy(c->site_ptr, sptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice);
-  // state
-  const size_t fb = (size_t)std::max(no, 1) * c->L;
-  if (hipMalloc(&c->f, fb) || hipMemset(c->f, 1, fb)) return fail("fields");
-  if (hipMalloc(&c->G, nn * sizeof(cplx)) || hipMalloc(&c->Gtmp, nn * sizeof(cplx)) ||
-      hipMalloc(&c->H, nn * sizeof(cplx)) || hipMalloc(&c->X, (size_t)n * CHAIN_KMAX * sizeof(cplx)) ||
-      hipMalloc(&c->Y, (size_t)n * CHAIN_KMAX * sizeof(cplx)) ||
-      hipMalloc(&c->Wk, (size_t)CHAIN_KMAX * CHAIN_KMAX * sizeof(cplx)) ||
-      hipMalloc(&c->amapk, CHAIN_KMAX * sizeof(int)) || hipMemset(c->amapk, 0, CHAIN_KMAX * sizeof(int)))
-    return fail("G buffers");
-  if (hipMalloc(&c->st, sizeof(ChainState)) || hipMalloc(&c->sc, sizeof(DevScalars)) ||
-      hipMalloc(&c->log, 3 * LOG_CAP * sizeof(double)))
-    return fail("scalars");
-  ChainState cs{};
-  cs.phase = make_double2(1.0, 0.0);
-  DevScalars ds{};
-  ds.z_cgr = make_double2(1.0, 0.0);
-  ds.opph = make_double2(1.0, 0.0);
-  hipMemcpy(c->st, &cs, sizeof(cs), hipMemcpyHostToDevice);
-  hipMemcpy(c->sc, &ds, sizeof(ds), hipMemcpyHostToDevice);
-  hipMemset(c->G, 0, nn * sizeof(cplx));
-  if (hipMalloc(&c->dv_d, (size_t)n * c->W * sizeof(cplx)) || hipMalloc(&c->dv_dinv, (size_t)n * c->W * sizeof(cplx)) ||
-      hipMalloc(&c->dv_dconj, (size_t)n * c->W * sizeof(cplx)) || hipMalloc(&c->U0, nn * sizeof(cplx)) ||
-      hipMalloc(&c->U1, nn * sizeof(cplx)) || hipMalloc(&c->TP, nn * sizeof(cplx)) ||
-      hipMalloc(&c->Qc, nn * sizeof(cplx)) || hipMalloc(&c->DUP, n * sizeof(cplx)) ||
-      hipMalloc(&c->DUPinv, n * sizeof(cplx)) || hipMalloc(&c->Xc, nn * sizeof(cplx)) ||
-      hipMalloc(&c->Yc, nn * sizeof(cplx)) || hipMalloc(&c->cscal, 4 * (size_t)n * sizeof(cplx)))
-    return fail("scratch");
-  if (qr_work_alloc(c->qr, n)) return fail("qr workspace");
   c->var_st.assign(c->nstm + 1, -1);
   if (udv_reset_dev(c, &c->var_r, 'R') || udv_reset_dev(c, &c->var_l, 'L')) return fail("udv");
   for (int k = 1; k <= c->nstm; ++k)
@@ -852,18 +970,36 @@ int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) {
   return ST_NONE;
 }
 
+}  // namespace
+
+extern "C" {
+
+int alfq_ctx_create(const alfq_model_desc* d, alfq_ctx** out) { return create_impl(d, 1, out); }
+
+int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx** out) {
+  return create_impl(descs, nchain, out);
+}
+
+int alfq_set_chain(alfq_ctx* c, int32_t chain) {
+  if (!c) return ST_GENERIC;
+  if (chain < 0 || chain >= c->B) { alfq_set_error("chain index outside the batch"); return ST_GENERIC; }
+  c->cur = chain;
+  return ST_NONE;
+}
+
+int alfq_batch_size(alfq_ctx* c, int32_t* nchain) {
+  if (!c || !nchain) return ST_GENERIC;
+  *nchain = c->B;
+  return ST_NONE;
+}
+
 int alfq_ctx_destroy(alfq_ctx* c) {
   if (!c) return ST_NONE;
   if (c->s) hipStreamSynchronize(c->s);
   drop_graphs(c);
-  for (auto& sl : c->pool) { hipFree(sl.U); hipFree(sl.V); hipFree(sl.D); hipFree(sl.sc); }
-  void* ptrs[] = {c->uoff_up, c->uoff_dn, c->expT, c->expTinv, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->eexp, c->myexp,
-                  c->expalpha, c->phasefac, c->gama_ratio, c->f, c->G, c->Gtmp, c->H, c->X, c->Y, c->Wk, c->amapk, c->st, c->sc,
-                  c->log, c->dv_d, c->dv_dinv, c->dv_dconj, c->U0, c->U1, c->TP, c->Qc, c->DUP, c->DUPinv,
-                  c->Xc, c->Yc, c->cscal, c->u};
+  void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  if (c->qr.A) qr_work_free(c->qr);
   for (auto& pr : c->chain_timer.ev) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -893,7 +1029,7 @@ int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) 
     h[i] = (int8_t)v;
   }
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy(c->f, h.data(), h.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(at(c, c->f), h.data(), h.size(), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -902,7 +1038,7 @@ int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
   if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
   std::vector<int8_t> h((size_t)n_op * ltrot);
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy(h.data(), c->f, h.size(), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(h.data(), at(c, c->f), h.size(), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < h.size(); ++i) f[i] = h[i];
   return ST_NONE;
 }
@@ -910,7 +1046,7 @@ int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
 int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy2D(c->G, c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
+  HIP_OK(hipMemcpy2D(at(c, c->G), c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
                      hipMemcpyHostToDevice));
   return ST_NONE;
 }
@@ -918,7 +1054,7 @@ int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
 int alfq_green_get(alfq_ctx* c, double* G, int32_t ld) {
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), c->G, c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
+  HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), at(c, c->G), c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
                      hipMemcpyDeviceToHost));
   return ST_NONE;
 }
@@ -927,7 +1063,7 @@ int alfq_phase_set(alfq_ctx* c, const double* phase) {
   if (!c || !phase) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   const cplx p = make_double2(phase[0], phase[1]);
-  HIP_OK(hipMemcpy(&c->st->phase, &p, sizeof(cplx), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&at(c, c->st)->phase, &p, sizeof(cplx), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -935,7 +1071,7 @@ int alfq_phase_get(alfq_ctx* c, double* phase) {
   if (!c || !phase) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   cplx p;
-  HIP_OK(hipMemcpy(&p, &c->st->phase, sizeof(cplx), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&p, &at(c, c->st)->phase, sizeof(cplx), hipMemcpyDeviceToHost));
   phase[0] = p.x;
   phase[1] = p.y;
   return ST_NONE;
@@ -943,6 +1079,7 @@ int alfq_phase_get(alfq_ctx* c, double* phase) {
 
 int alfq_udv_set(alfq_ctx* c, int32_t var, const double* U, const double* D, const double* V, char side) {
   if (!c) return ST_GENERIC;
+  if (c->B > 1) { alfq_set_error("udv_set: single chains only (a slot binding is shared by the batch)"); return ST_GENERIC; }
   int* v = var_ref(c, var);
   if (!v) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
   const int id = slot_alloc(c);
@@ -991,9 +1128,9 @@ int alfq_udv_get(alfq_ctx* c, int32_t var, double* U, double* D, double* V) {
   const UDVSlot& sl = c->pool[*v];
   const size_t nn = (size_t)c->n * c->n;
   HIP_OK(hipStreamSynchronize(c->s));
-  if (U) HIP_OK(hipMemcpy(U, sl.U, nn * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (V) HIP_OK(hipMemcpy(V, sl.V, nn * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (D) HIP_OK(hipMemcpy(D, sl.D, c->n * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (U) HIP_OK(hipMemcpy(U, at(c, sl.U), nn * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (V) HIP_OK(hipMemcpy(V, at(c, sl.V), nn * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (D) HIP_OK(hipMemcpy(D, at(c, sl.D), c->n * sizeof(cplx), hipMemcpyDeviceToHost));
   return ST_NONE;
 }
 
@@ -1015,11 +1152,14 @@ int alfq_udv_assign(alfq_ctx* c, int32_t dst, int32_t src) {
 
 int alfq_rng_set_tape(alfq_ctx* c, const double* u, int64_t n) {
   if (!c || (!u && n > 0) || n < 0) return ST_GENERIC;
+  if (c->B > 1) { alfq_set_error("random tapes are for single chains; use alfq_rng_set_philox per chain"); return ST_GENERIC; }
   HIP_OK(hipStreamSynchronize(c->s));
-  if (c->u) hipFree(c->u);
-  c->u = nullptr;
-  HIP_OK(hipMalloc(&c->u, std::max<int64_t>(n, 1) * sizeof(double)));
-  if (n > 0) HIP_OK(hipMemcpy(c->u, u, n * sizeof(double), hipMemcpyHostToDevice));
+  drop_graphs(c);
+  if (c->u_tape) hipFree(c->u_tape);
+  c->u_tape = nullptr;
+  HIP_OK(hipMalloc(&c->u_tape, std::max<int64_t>(n, 1) * sizeof(double)));
+  if (n > 0) HIP_OK(hipMemcpy(c->u_tape, u, n * sizeof(double), hipMemcpyHostToDevice));
+  c->u = c->u_tape;
   c->u_cap = n;
   c->u_len = n;
   c->u_base = 0;
@@ -1042,11 +1182,17 @@ int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
   c->rng_next = 0;
   c->u_len = 0;
   c->u_base = 0;
+  c->u = c->u_arena;
+  c->u_cap = c->u_arena_cap;
+  // the chain's key and stream live in its device state
+  ChainState* stc = at(c, c->st);
   const unsigned long long zero = 0;
   const long long len0 = 0;
-  HIP_OK(hipMemcpy(&c->st->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(&c->st->u_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(&c->st->u_len, &len0, sizeof(len0), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->rng_pos, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->u_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->u_len, &len0, sizeof(len0), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->seed, &seed, sizeof(seed), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->stream, &stream, sizeof(stream), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -1054,7 +1200,7 @@ int alfq_rng_position(alfq_ctx* c, int64_t* pos) {
   if (!c || !pos) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   unsigned long long p;
-  HIP_OK(hipMemcpy(&p, &c->st->rng_pos, sizeof(p), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&p, &at(c, c->st)->rng_pos, sizeof(p), hipMemcpyDeviceToHost));
   *pos = (int64_t)p;
   return ST_NONE;
 }
@@ -1115,7 +1261,7 @@ int alfq_cgr(alfq_ctx* c, int32_t nvar, int32_t var_r, int32_t var_l, int32_t ch
   if ((st = check_status(c))) return st;
   if (z) {
     cplx zz;
-    HIP_OK(hipMemcpy(&zz, &c->sc->z_cgr, sizeof(cplx), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&zz, &at(c, c->sc)->z_cgr, sizeof(cplx), hipMemcpyDeviceToHost));
     z[0] = zz.x;
     z[1] = zz.y;
   }
@@ -1195,8 +1341,8 @@ int alfq_control_get(alfq_ctx* c, alfq_control* out) {
   HIP_OK(hipStreamSynchronize(c->s));
   ChainState cs;
   DevScalars ds;
-  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&cs, at(c, c->st), sizeof(cs), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, at(c, c->sc), sizeof(ds), hipMemcpyDeviceToHost));
   out->nc_up = cs.nc;
   out->acc_up = cs.acc;
   out->ncg = ds.ncg;
@@ -1211,15 +1357,15 @@ int alfq_control_reset(alfq_ctx* c) {
   HIP_OK(hipStreamSynchronize(c->s));
   ChainState cs;
   DevScalars ds;
-  HIP_OK(hipMemcpy(&cs, c->st, sizeof(cs), hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&cs, at(c, c->st), sizeof(cs), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, at(c, c->sc), sizeof(ds), hipMemcpyDeviceToHost));
   cs.nc = cs.acc = 0;
   ds.ncg = 0;
   ds.xmeang = ds.xmaxg = ds.xmaxp = 0.0;
   ds.nlog = 0;
   ds.status = 0;
-  HIP_OK(hipMemcpy(c->st, &cs, sizeof(cs), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->sc, &ds, sizeof(ds), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(at(c, c->st), &cs, sizeof(cs), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(at(c, c->sc), &ds, sizeof(ds), hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -1227,11 +1373,11 @@ int alfq_precision_log(alfq_ctx* c, double* out3, int64_t cap, int64_t* count) {
   if (!c || !count) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   DevScalars ds;
-  HIP_OK(hipMemcpy(&ds, c->sc, sizeof(ds), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&ds, at(c, c->sc), sizeof(ds), hipMemcpyDeviceToHost));
   const int64_t avail = std::min<int64_t>(ds.nlog, LOG_CAP);
   *count = ds.nlog;
   const int64_t k = std::min<int64_t>(avail, cap);
-  if (out3 && k > 0) HIP_OK(hipMemcpy(out3, c->log, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
+  if (out3 && k > 0) HIP_OK(hipMemcpy(out3, at(c, c->log), 3 * k * sizeof(double), hipMemcpyDeviceToHost));
   return ST_NONE;
 }
 

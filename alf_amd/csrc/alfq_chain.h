@@ -22,6 +22,9 @@ struct ChainState {
   int max_slice_acc;              // largest slice_acc seen since the host last cleared it
   unsigned long long u_base;      // global index of u[0] in the uniform buffer
   long long u_len;                // uniforms held by the buffer
+  unsigned long long seed;        // Philox key of this chain
+  unsigned stream;                // Philox stream of this chain
+  int pad2;
 };
 
 struct ChainArgs {
@@ -40,6 +43,11 @@ struct ChainArgs {
   ChainState* st;
   // block kernel outputs for the flush G += (G(:, amap) W) Y^T: W (32 x 32, ld 32), site list
   cplx* W; int* amap;
+  // batches: byte stride between the chains' arenas; G, X, Y, f, the coupling tables
+  // (op_myexp, op_expalpha, gama_ratio), u, st, W and amap are per chain, op_site,
+  // op_type and uoff are shared
+  long long cs;
+  int nchain;
 };
 
 // Optional per-launch timing of the chain kernel: start/stop events recorded by the
@@ -53,9 +61,11 @@ struct KTimer {
 int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* timer = nullptr);
 // largest delay depth the submatrix block kernel supports (deeper delays use the generic kernel)
 int chain_max_delay(int n);
-// uniforms of global indices [st->rng_pos, st->rng_pos + count) into u; records the window in st
-int launch_philox_fill(double* u, ChainState* st, long long count, unsigned long long seed, unsigned stream,
-                       hipStream_t s);
+// uniforms of global indices [st->rng_pos, st->rng_pos + count) into u with the chain's
+// Philox key and stream (st->seed, st->stream); records the window in st
+int launch_philox_fill(double* u, ChainState* st, long long count, int nchain, long long cs, hipStream_t s);
+// f, eexp, d, dinv, dconj (and phasefac, out) per chain; site_ptr, site_ops shared
 int launch_dvec(const int8_t* f, int n_op, int s0, int ns, int n, const int* site_ptr, const int* site_ops,
-                const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, hipStream_t s);
-int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, hipStream_t s);
+                const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, int nchain, long long cs, hipStream_t s);
+int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, int nchain,
+                   long long cs, hipStream_t s);

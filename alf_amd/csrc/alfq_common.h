@@ -40,6 +40,28 @@ __host__ __device__ inline cplx cfma(cplx a, cplx b, cplx c) {  // a*b + c
   return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
 }
 
+// Batches of chains: every per-chain device buffer of chain z lives at the same offset of
+// chain z's arena, so a kernel finds its chain's data as (chain 0's pointer) + z * stride
+// bytes, with z = blockIdx.z.  Null pointers (absent optional operands) stay null.
+template <class T>
+__device__ __forceinline__ T* chain_ptr(T* p, long long stride) {
+  return p ? (T*)((char*)p + (long long)blockIdx.z * stride) : p;
+}
+
+// Two-pass layout of a chain's arena: the first pass (base == nullptr) only sizes it, the
+// second assigns pointers at the same 256-byte aligned offsets.
+struct ArenaLayout {
+  char* base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
 // Status codes: the values of the reference's runtime_error_mod enum
 // (Libraries/Modules/runtime_error_mod.F90:56-68), returned by every C entry point.
 enum alfq_status_internal {
@@ -84,8 +106,10 @@ struct GemmArgs {
   const double* rs_real;    // real per-row scale (alternative to rs)
   const int* amap;          // column gather of opA
   const int* bmap;          // row gather of opB
-  int batch;                // number of independent problems (grid z)
+  int batch;                // number of independent problems per chain
   long long strideA, strideB, strideC;  // element strides between problems
+  int nchain;               // chains of the batch (grid z = batch * nchain)
+  long long cstride;        // byte stride between the chains' arenas (applies to every pointer)
 };
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,

@@ -4,6 +4,8 @@
 
 struct QRWork {
   int n = 0, nb = 0, nblk = 0;
+  int nchain = 1;         // chains of the batch; every buffer below is per chain
+  long long cs = 0;       // byte stride between the chains' arenas
   bool lds_panel = false; // panel factorisation in LDS (n <= 2048)
   cplx* A = nullptr;      // n x n factorisation workspace
   cplx* Vw = nullptr;     // explicit Householder panels (n x n)
@@ -22,10 +24,10 @@ struct QRWork {
   cplx* scal = nullptr;   // 8 n-vectors of scale factors
 };
 
-int qr_work_alloc(QRWork& w, int n);
-void qr_work_free(QRWork& w);
-int launch_fill(cplx* X, long long count, cplx val, hipStream_t s);
-int launch_identity(cplx* X, int n, hipStream_t s);
+// places the workspace of one chain in its arena (sizes it when ar.base is null)
+void qr_work_place(QRWork& w, int n, ArenaLayout& ar);
+int launch_fill(cplx* X, long long count, cplx val, int nchain, long long cs, hipStream_t s);
+int launch_identity(cplx* X, int n, int nchain, long long cs, hipStream_t s);
 
 // decompose_UDV_state: (U, D, V) of the propagated state -> new (U, D, V) with
 // U D V = Usrc Dsrc Vsrc (side R) or U D V^H = Usrc Dsrc Vsrc^H (side L).

@@ -48,10 +48,12 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
 // uniforms u[q - base] for global indices q in [base, base + count), base = st->rng_pos: one
 // Philox block per index, 53-bit mantissa from two words, u in [0, 1).  The window is read
 // from and recorded in device memory, so a captured sweep replays with the stream's position.
-__global__ void philox_fill_kernel(double* __restrict__ u, ChainState* st, long long count,
-                                   unsigned long long seed, unsigned stream) {
+__global__ void philox_fill_kernel(double* __restrict__ u, ChainState* st, long long count, long long cs) {
+  u = chain_ptr(u, cs);
+  st = chain_ptr(st, cs);
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned long long base = st->rng_pos;
+  const unsigned long long base = st->rng_pos, seed = st->seed;
+  const unsigned stream = st->stream;
   if (i == 0) {
     st->u_base = base;
     st->u_len = count;
@@ -271,6 +273,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int tid = threadIdx.x;
   const int n = a.n;
   const int ga = tid & (SB_V - 1), gq = tid >> 5;   // g entries (ga, gq + 8 q), q = 0..3
+  a.G = chain_ptr(a.G, a.cs);
+  a.X = chain_ptr(a.X, a.cs);
+  a.Y = chain_ptr(a.Y, a.cs);
+  a.f = chain_ptr(a.f, a.cs);
+  a.op_myexp = chain_ptr(a.op_myexp, a.cs);
+  a.op_expalpha = chain_ptr(a.op_expalpha, a.cs);
+  a.gama_ratio = chain_ptr(a.gama_ratio, a.cs);
+  a.u = chain_ptr(a.u, a.cs);
+  a.st = chain_ptr(a.st, a.cs);
+  a.W = chain_ptr(a.W, a.cs);
+  a.amap = chain_ptr(a.amap, a.cs);
   ChainState* st = a.st;
   if (tid == 0) {
     s_pos0 = a.first_round ? 0 : st->pos;
@@ -679,7 +692,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 __global__ void dvec_kernel(const int8_t* __restrict__ f, int n_op, int s0, int ns, int n,
                             const int* __restrict__ site_ptr, const int* __restrict__ site_ops,
                             const cplx* __restrict__ eexp, cplx* __restrict__ d, cplx* __restrict__ dinv,
-                            cplx* __restrict__ dconj) {
+                            cplx* __restrict__ dconj, long long cs) {
+  f = chain_ptr(f, cs);
+  eexp = chain_ptr(eexp, cs);
+  d = chain_ptr(d, cs);
+  dinv = chain_ptr(dinv, cs);
+  dconj = chain_ptr(dconj, cs);
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * ns) return;
   const int i = idx % n, s = idx / n;
@@ -698,8 +716,11 @@ __global__ void dvec_kernel(const int8_t* __restrict__ f, int n_op, int s0, int 
 
 // Op_phase: prod over (op, slice) of exp(i Im(g alpha phi(f))) as a block product.
 __global__ void opphase_kernel(const int8_t* __restrict__ f, long long count, int n_op,
-                               const cplx* __restrict__ phasefac, cplx* __restrict__ out) {
+                               const cplx* __restrict__ phasefac, cplx* __restrict__ out, long long cs) {
   __shared__ cplx sh[256];
+  f = chain_ptr(f, cs);
+  phasefac = chain_ptr(phasefac, cs);
+  out = chain_ptr(out, cs);
   cplx p = make_double2(1.0, 0.0);
   for (long long i = threadIdx.x; i < count; i += blockDim.x) {
     const int op = (int)(i % n_op);
@@ -717,11 +738,10 @@ __global__ void opphase_kernel(const int8_t* __restrict__ f, long long count, in
 
 }  // namespace
 
-int launch_philox_fill(double* u, ChainState* st, long long count, unsigned long long seed, unsigned stream,
-                       hipStream_t s) {
+int launch_philox_fill(double* u, ChainState* st, long long count, int nchain, long long cs, hipStream_t s) {
   if (count <= 0) return ST_NONE;
-  hipLaunchKernelGGL(philox_fill_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, u, st, count, seed,
-                     stream);
+  hipLaunchKernelGGL(philox_fill_kernel, dim3((unsigned)((count + 255) / 256), 1, nchain), dim3(256), 0, s, u, st,
+                     count, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
@@ -745,12 +765,16 @@ int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
         t->ev.emplace_back(e0, e1);
       }
       auto& pr = t->ev[t->used++];
-      hipExtLaunchKernelGGL(chain_block_kernel, dim3(1), dim3(256), 0, s, pr.first, pr.second, 0, a);
+      hipExtLaunchKernelGGL(chain_block_kernel, dim3(1, 1, a.nchain), dim3(256), 0, s, pr.first, pr.second, 0, a);
     } else {
-      hipLaunchKernelGGL(chain_block_kernel, dim3(1), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(chain_block_kernel, dim3(1, 1, a.nchain), dim3(256), 0, s, a);
     }
     HIP_OK(hipGetLastError());
     return ST_NONE;
+  }
+  if (a.nchain > 1) {
+    alfq_set_error("launch_chain: delays beyond the block kernel's depth are supported for single chains only");
+    return ST_GENERIC;
   }
   hipLaunchKernelGGL(chain_kernel, dim3(1), dim3(256), 0, s, a);
   HIP_OK(hipGetLastError());
@@ -758,16 +782,17 @@ int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
 }
 
 int launch_dvec(const int8_t* f, int n_op, int s0, int ns, int n, const int* site_ptr, const int* site_ops,
-                const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, hipStream_t s) {
+                const cplx* eexp, cplx* d, cplx* dinv, cplx* dconj, int nchain, long long cs, hipStream_t s) {
   const int total = n * ns;
-  hipLaunchKernelGGL(dvec_kernel, dim3((total + 255) / 256), dim3(256), 0, s, f, n_op, s0, ns, n, site_ptr,
-                     site_ops, eexp, d, dinv, dconj);
+  hipLaunchKernelGGL(dvec_kernel, dim3((total + 255) / 256, 1, nchain), dim3(256), 0, s, f, n_op, s0, ns, n,
+                     site_ptr, site_ops, eexp, d, dinv, dconj, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
 
-int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, hipStream_t s) {
-  hipLaunchKernelGGL(opphase_kernel, dim3(1), dim3(256), 0, s, f, count, n_op, phasefac, out);
+int launch_opphase(const int8_t* f, long long count, int n_op, const cplx* phasefac, cplx* out, int nchain,
+                   long long cs, hipStream_t s) {
+  hipLaunchKernelGGL(opphase_kernel, dim3(1, 1, nchain), dim3(256), 0, s, f, count, n_op, phasefac, out, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }

@@ -25,6 +25,16 @@ constexpr int NT = 256;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// moves every operand pointer to the arena of the chain blockIdx.z / batch
+__device__ __forceinline__ void chain_operands(GemmArgs& g) {
+  if (g.nchain <= 1) return;
+  const long long off = (long long)(blockIdx.z / (g.batch > 1 ? g.batch : 1)) * g.cstride;
+  auto mv = [off](auto p) { return p ? (decltype(p))((const char*)p + off) : p; };
+  g.A = mv(g.A); g.B = mv(g.B); g.C = (void*)mv((const char*)g.C);
+  g.rs = mv(g.rs); g.cs = mv(g.cs); g.bs = mv(g.bs); g.rs_real = mv(g.rs_real);
+  g.amap = mv(g.amap); g.bmap = mv(g.bmap); g.kdev = mv(g.kdev);
+}
+
 template <bool A_CPLX, bool B_CPLX, int WM, int WN>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   constexpr int KS = 4 / (WM * WN);
@@ -40,7 +50,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
   const int j0 = blockIdx.y * TN;
-  const int z = blockIdx.z;
+  chain_operands(g);
+  const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
     int kd = *g.kdev;
@@ -209,10 +220,13 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 }
 
 inline long long cdivl(long long a, long long b) { return (a + b - 1) / b; }
+inline unsigned gemm_zdim(const GemmArgs& g) {
+  return (unsigned)((g.batch > 1 ? g.batch : 1) * (g.nchain > 1 ? g.nchain : 1));
+}
 
 template <int WM, int WN>
 void launch_shape(const GemmArgs& g, hipStream_t s) {
-  dim3 grid((unsigned)cdivl(g.M, 16 * WM), (unsigned)cdivl(g.N, 16 * WN), g.batch > 1 ? g.batch : 1);
+  dim3 grid((unsigned)cdivl(g.M, 16 * WM), (unsigned)cdivl(g.N, 16 * WN), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
   if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_kernel<true, true, WM, WN>), grid, dim3(NT), 0, s, g);
   else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false, WM, WN>), grid, dim3(NT), 0, s, g);
@@ -233,7 +247,9 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, z = blockIdx.z;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  chain_operands(g);
+  const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
     const int kd = *g.kdev;
@@ -351,7 +367,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
 
 template <int NW>
 void launch_direct(const GemmArgs& g, hipStream_t s) {
-  dim3 grid((unsigned)cdivl(g.M, 16), (unsigned)cdivl(g.N, 16), g.batch > 1 ? g.batch : 1);
+  dim3 grid((unsigned)cdivl(g.M, 16), (unsigned)cdivl(g.N, 16), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
   if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_direct_kernel<true, true, NW>), grid, dim3(64 * NW), 0, s, g);
   else if (g.a_cplx) hipLaunchKernelGGL((gemm_direct_kernel<true, false, NW>), grid, dim3(64 * NW), 0, s, g);
@@ -370,6 +386,8 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
   g.C = C; g.ldc = ldc;
   g.alpha = alpha; g.beta = beta;
   g.batch = 1;
+  g.nchain = 1;
+  g.cstride = 0;
   return g;
 }
 
@@ -392,8 +410,8 @@ static int gemm_tile_shape(int M, int N, int K, int batch) {
 
 int gemm_launch(const GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return ST_NONE;
-  if (g.batch > 65535) { alfq_set_error("gemm_launch: batch too large"); return ST_GENERIC; }
-  const long long b = g.batch > 1 ? g.batch : 1;
+  if (gemm_zdim(g) > 65535) { alfq_set_error("gemm_launch: batch too large"); return ST_GENERIC; }
+  const long long b = gemm_zdim(g);
   if (cdivl(g.M, 32) * cdivl(g.N, 32) * b < 256) {
     // too few 32 x 32 tiles to fill the device: 16 x 16 tiles, K split over up to 8 waves
     const int ksteps = g.K > 0 ? (g.K + 3) / 4 : 1;
@@ -404,7 +422,7 @@ int gemm_launch(const GemmArgs& g, hipStream_t s) {
     HIP_OK(hipGetLastError());
     return ST_NONE;
   }
-  switch (gemm_tile_shape(g.M, g.N, g.K, g.batch)) {
+  switch (gemm_tile_shape(g.M, g.N, g.K, (int)gemm_zdim(g))) {
     case 0: launch_shape<2, 2>(g, s); break;
     case 1: launch_shape<1, 2>(g, s); break;
     default: launch_shape<1, 1>(g, s); break;

@@ -44,7 +44,10 @@ __device__ double block_sum(double v, double* red) {
 // ---- column norms of the matrix to be factorised -------------------------------
 // src(i, j) = S(i, j) * D(j)   or, with ctrans, conj(S(j, i)).
 __global__ void colnorm_kernel(const cplx* __restrict__ S, int n, int lds, const cplx* __restrict__ D,
-                               int ctrans, double* __restrict__ norms) {
+                               int ctrans, double* __restrict__ norms, long long cs) {
+  S = chain_ptr(S, cs);
+  D = chain_ptr(D, cs);
+  norms = chain_ptr(norms, cs);
   const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= n) return;
@@ -63,7 +66,10 @@ __global__ void colnorm_kernel(const cplx* __restrict__ S, int n, int lds, const
 
 // rank sort, descending, ties broken by index: perm[rank] = j, pinv[j] = rank
 __global__ void sortperm_kernel(const double* __restrict__ norms, int n, int* __restrict__ perm,
-                                int* __restrict__ pinv) {
+                                int* __restrict__ pinv, long long cs) {
+  norms = chain_ptr(norms, cs);
+  perm = chain_ptr(perm, cs);
+  pinv = chain_ptr(pinv, cs);
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const double x = norms[j];
     int r = 0;
@@ -78,7 +84,11 @@ __global__ void sortperm_kernel(const double* __restrict__ norms, int n, int* __
 
 // A(:, i) = src(:, perm[i])
 __global__ void gather_kernel(const cplx* __restrict__ S, int n, int lds, const cplx* __restrict__ D,
-                              int ctrans, const int* __restrict__ perm, cplx* __restrict__ A) {
+                              int ctrans, const int* __restrict__ perm, cplx* __restrict__ A, long long cs) {
+  S = chain_ptr(S, cs);
+  D = chain_ptr(D, cs);
+  perm = chain_ptr(perm, cs);
+  A = chain_ptr(A, cs);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n) return;
   const int r = (int)(idx % n), i = (int)(idx / n);
@@ -93,12 +103,14 @@ __global__ void gather_kernel(const cplx* __restrict__ S, int n, int lds, const 
   A[idx] = v;
 }
 
-__global__ void fill_kernel(cplx* __restrict__ X, long long count, cplx val) {
+__global__ void fill_kernel(cplx* __restrict__ X, long long count, cplx val, long long cs) {
+  X = chain_ptr(X, cs);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < count) X[idx] = val;
 }
 
-__global__ void identity_kernel(cplx* __restrict__ X, int n) {
+__global__ void identity_kernel(cplx* __restrict__ X, int n, long long cs) {
+  X = chain_ptr(X, cs);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n) return;
   const int r = (int)(idx % n), c = (int)(idx / n);
@@ -113,7 +125,14 @@ __global__ void identity_kernel(cplx* __restrict__ X, int n) {
 __global__ __launch_bounds__(PT) void panel_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                    cplx* __restrict__ tau, cplx* __restrict__ Vw,
                                                    cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
-                                                   cplx* __restrict__ Tw, cplx* __restrict__ pdet) {
+                                                   cplx* __restrict__ Tw, cplx* __restrict__ pdet, long long cs) {
+  A = chain_ptr(A, cs);
+  tau = chain_ptr(tau, cs);
+  Vw = chain_ptr(Vw, cs);
+  Y1w = chain_ptr(Y1w, cs);
+  Y2w = chain_ptr(Y2w, cs);
+  Tw = chain_ptr(Tw, cs);
+  pdet = chain_ptr(pdet, cs);
   __shared__ double red[PT / 64];
   __shared__ cplx s_tau[NBMAX];
   __shared__ cplx s_T[NBMAX * NBMAX];
@@ -274,7 +293,14 @@ constexpr int QPW = PB / PLW;   // panel columns per wave
 __global__ __launch_bounds__(PLT) void panel_lds_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                        cplx* __restrict__ tau, cplx* __restrict__ Vw,
                                                        cplx* __restrict__ Y1w, cplx* __restrict__ Y2w,
-                                                       cplx* __restrict__ Tw, cplx* __restrict__ pdet) {
+                                                       cplx* __restrict__ Tw, cplx* __restrict__ pdet, long long cs) {
+  A = chain_ptr(A, cs);
+  tau = chain_ptr(tau, cs);
+  Vw = chain_ptr(Vw, cs);
+  Y1w = chain_ptr(Y1w, cs);
+  Y2w = chain_ptr(Y2w, cs);
+  Tw = chain_ptr(Tw, cs);
+  pdet = chain_ptr(pdet, cs);
   extern __shared__ cplx sp[];                 // rows x ncol, column-major, ld = rows
   __shared__ cplx s_TG[PB * PB];               // upper incl. diagonal: T; strict lower: Gram
   __shared__ cplx s_tau[PB], s_sc[PB];
@@ -479,7 +505,13 @@ __device__ __forceinline__ double hsum32(double v) {
 __global__ __launch_bounds__(1024) void panel_reg_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                          cplx* __restrict__ tau, cplx* __restrict__ Vw,
                                                          cplx* __restrict__ Y1w, cplx* __restrict__ Tw,
-                                                         cplx* __restrict__ pdet) {
+                                                         cplx* __restrict__ pdet, long long cs) {
+  A = chain_ptr(A, cs);
+  tau = chain_ptr(tau, cs);
+  Vw = chain_ptr(Vw, cs);
+  Y1w = chain_ptr(Y1w, cs);
+  Tw = chain_ptr(Tw, cs);
+  pdet = chain_ptr(pdet, cs);
   extern __shared__ cplx smem[];      // loop: vbuf[2][RP]; afterwards: V[RP x 32], column-major
   __shared__ cplx s_S[32 * 32];       // Gram above the diagonal, then T
   __shared__ cplx s_tau[32];
@@ -653,7 +685,16 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
                                                        const cplx* __restrict__ detUl,
                                                        cplx* __restrict__ Dout, cplx* __restrict__ Rt,
                                                        cplx* __restrict__ phase_out,
-                                                       cplx* __restrict__ detU_out) {
+                                                       cplx* __restrict__ detU_out, long long cs) {
+  A = chain_ptr(A, cs);
+  perm = chain_ptr(perm, cs);
+  pdet = chain_ptr(pdet, cs);
+  detUr = chain_ptr(detUr, cs);
+  detUl = chain_ptr(detUl, cs);
+  Dout = chain_ptr(Dout, cs);
+  Rt = chain_ptr(Rt, cs);
+  phase_out = chain_ptr(phase_out, cs);
+  detU_out = chain_ptr(detU_out, cs);
   __shared__ cplx sh[PT];
   const int tid = threadIdx.x;
   // product of normalised diagonal entries
@@ -709,7 +750,10 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
 // Rt = R with row i divided by |R_ii| (zero below the diagonal); for the UDV decompose
 // (mode 0/1) row 0 is further divided by the decompose phase.
 __global__ void qr_rt_kernel(const cplx* __restrict__ A, int n, int mode, const cplx* __restrict__ phase,
-                             cplx* __restrict__ Rt) {
+                             cplx* __restrict__ Rt, long long cs) {
+  A = chain_ptr(A, cs);
+  phase = chain_ptr(phase, cs);
+  Rt = chain_ptr(Rt, cs);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n) return;
   const int i = (int)(idx % n), j = (int)(idx / n);
@@ -722,14 +766,18 @@ __global__ void qr_rt_kernel(const cplx* __restrict__ A, int n, int mode, const 
 }
 
 // X(:, 0) *= s
-__global__ void scale_col0_kernel(cplx* __restrict__ X, int n, const cplx* __restrict__ s) {
+__global__ void scale_col0_kernel(cplx* __restrict__ X, int n, const cplx* __restrict__ s, long long cs) {
+  X = chain_ptr(X, cs);
+  s = chain_ptr(s, cs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) X[i] = cmul(X[i], *s);
 }
 
 // Invert the nb x nb diagonal blocks of the upper-triangular Rt into Rinv (one block per WG).
 __global__ __launch_bounds__(PT) void trinv_diag_kernel(const cplx* __restrict__ Rt, int n, int nb,
-                                                        cplx* __restrict__ Rinv) {
+                                                        cplx* __restrict__ Rinv, long long cs) {
+  Rt = chain_ptr(Rt, cs);
+  Rinv = chain_ptr(Rinv, cs);
   __shared__ cplx s[NBMAX * NBMAX];
   __shared__ cplx x[NBMAX * NBMAX];
   const int b = blockIdx.x;
@@ -760,7 +808,7 @@ inline unsigned nblocks(long long count, int per) { return (unsigned)((count + p
 
 }  // namespace
 
-int qr_work_alloc(QRWork& w, int n) {
+void qr_work_place(QRWork& w, int n, ArenaLayout& ar) {
   w.n = n;
   // panel width: the widest whose rows x nb panel fits the LDS kernel's 128 KiB
   if (n <= 256) w.nb = 32;
@@ -771,41 +819,38 @@ int qr_work_alloc(QRWork& w, int n) {
   w.lds_panel = n <= 2048;
   w.nblk = (n + w.nb - 1) / w.nb;
   const size_t nn = (size_t)n * n;
-  HIP_OK(hipMalloc(&w.A, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.Vw, nn * sizeof(cplx)));
-  HIP_OK(hipMemset(w.Vw, 0, nn * sizeof(cplx)));   // the panels never write above their diagonal block
-  HIP_OK(hipMalloc(&w.Y1, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.Y2, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.Rt, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.Rinv, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.Tmp, nn * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.T, (size_t)w.nblk * w.nb * w.nb * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.W, (size_t)w.nb * n * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.tau, n * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.pdet, w.nblk * sizeof(cplx)));
-  HIP_OK(hipMalloc(&w.norms, n * sizeof(double)));
-  HIP_OK(hipMalloc(&w.perm, n * sizeof(int)));
-  HIP_OK(hipMalloc(&w.pinv, n * sizeof(int)));
-  HIP_OK(hipMalloc(&w.scal, 8 * n * sizeof(cplx)));
-  return ST_NONE;
+  w.A = ar.take<cplx>(nn);
+  w.Vw = ar.take<cplx>(nn);   // the panels never write above their diagonal block: zero from creation
+  w.Y1 = ar.take<cplx>(nn);
+  w.Y2 = ar.take<cplx>(nn);
+  w.Rt = ar.take<cplx>(nn);
+  w.Rinv = ar.take<cplx>(nn);
+  w.Tmp = ar.take<cplx>(nn);
+  w.T = ar.take<cplx>((size_t)w.nblk * w.nb * w.nb);
+  w.W = ar.take<cplx>((size_t)w.nb * n);
+  w.tau = ar.take<cplx>(n);
+  w.pdet = ar.take<cplx>(w.nblk);
+  w.norms = ar.take<double>(n);
+  w.perm = ar.take<int>(n);
+  w.pinv = ar.take<int>(n);
+  w.scal = ar.take<cplx>(8 * (size_t)n);
 }
 
-void qr_work_free(QRWork& w) {
-  hipFree(w.A); hipFree(w.Vw); hipFree(w.Y1); hipFree(w.Y2); hipFree(w.Rt); hipFree(w.Rinv);
-  hipFree(w.Tmp); hipFree(w.T); hipFree(w.W); hipFree(w.tau); hipFree(w.pdet); hipFree(w.norms);
-  hipFree(w.perm); hipFree(w.pinv); hipFree(w.scal);
-  w = QRWork{};
+static int gemm_w(const QRWork& w, GemmArgs g, hipStream_t s) {
+  g.nchain = w.nchain;
+  g.cstride = w.cs;
+  return gemm_launch(g, s);
 }
 
-int launch_fill(cplx* X, long long count, cplx val, hipStream_t s) {
+int launch_fill(cplx* X, long long count, cplx val, int nchain, long long cs, hipStream_t s) {
   if (count <= 0) return ST_NONE;
-  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(count, 256)), dim3(256), 0, s, X, count, val);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(count, 256), 1, nchain), dim3(256), 0, s, X, count, val, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
 
-int launch_identity(cplx* X, int n, hipStream_t s) {
-  hipLaunchKernelGGL(identity_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, X, n);
+int launch_identity(cplx* X, int n, int nchain, long long cs, hipStream_t s) {
+  hipLaunchKernelGGL(identity_kernel, dim3(nblocks((long long)n * n, 256), 1, nchain), dim3(256), 0, s, X, n, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
@@ -814,10 +859,11 @@ int launch_identity(cplx* X, int n, hipStream_t s) {
 static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctrans, hipStream_t s) {
   const int n = w.n, nb = w.nb;
   int st;
-  hipLaunchKernelGGL(colnorm_kernel, dim3(nblocks(n, 4)), dim3(256), 0, s, S, n, lds, D, ctrans, w.norms);
-  hipLaunchKernelGGL(sortperm_kernel, dim3(1), dim3(256), 0, s, w.norms, n, w.perm, w.pinv);
-  hipLaunchKernelGGL(gather_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, S, n, lds, D,
-                     ctrans, w.perm, w.A);
+  hipLaunchKernelGGL(colnorm_kernel, dim3(nblocks(n, 4), 1, w.nchain), dim3(256), 0, s, S, n, lds, D, ctrans,
+                     w.norms, w.cs);
+  hipLaunchKernelGGL(sortperm_kernel, dim3(1, 1, w.nchain), dim3(256), 0, s, w.norms, n, w.perm, w.pinv, w.cs);
+  hipLaunchKernelGGL(gather_kernel, dim3(nblocks((long long)n * n, 256), 1, w.nchain), dim3(256), 0, s, S, n, lds, D,
+                     ctrans, w.perm, w.A, w.cs);
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   for (int k = 0; k < n; k += nb) {
@@ -829,8 +875,8 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
                                      (int)(RP * 32 * sizeof(cplx)));
       });
       HIP_OK(attr_r);
-      hipLaunchKernelGGL(panel_reg_kernel, dim3(1), dim3(1024), RP * 32 * sizeof(cplx), s, w.A, n, k, nb, w.tau, w.Vw,
-                         w.Y1, w.T, w.pdet);
+      hipLaunchKernelGGL(panel_reg_kernel, dim3(1, 1, w.nchain), dim3(1024), RP * 32 * sizeof(cplx), s, w.A, n, k, nb,
+                         w.tau, w.Vw, w.Y1, w.T, w.pdet, w.cs);
     } else if (w.lds_panel) {
       const size_t lds = (size_t)(n - k) * std::min(nb, n - k) * sizeof(cplx);
       static std::once_flag once;
@@ -840,11 +886,11 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
                                        128 * 1024);
       });
       HIP_OK(attr_err);
-      hipLaunchKernelGGL(panel_lds_kernel, dim3(1), dim3(PLT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1,
-                         (cplx*)nullptr, w.T, w.pdet);
+      hipLaunchKernelGGL(panel_lds_kernel, dim3(1, 1, w.nchain), dim3(PLT), lds, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1,
+                         (cplx*)nullptr, w.T, w.pdet, w.cs);
     } else {
-      hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1, (cplx*)nullptr,
-                         w.T, w.pdet);
+      hipLaunchKernelGGL(panel_kernel, dim3(1, 1, w.nchain), dim3(PT), 0, s, w.A, n, k, nb, w.tau, w.Vw, w.Y1,
+                         (cplx*)nullptr, w.T, w.pdet, w.cs);
     }
     HIP_OK(hipGetLastError());
     const int kc = k + nb;
@@ -853,11 +899,11 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
     // W = V^H A(k:, kc:)
     GemmArgs g1 = gemm_args(nb, ntr, rows, w.Vw + k + (long long)k * n, n, OP_C, true,
                             w.A + k + (long long)kc * n, n, OP_N, true, w.W, nb, one, zero);
-    if ((st = gemm_launch(g1, s))) return st;
+    if ((st = gemm_w(w, g1, s))) return st;
     // A(k:, kc:) -= Y1 W
     GemmArgs g2 = gemm_args(rows, ntr, nb, w.Y1 + k + (long long)k * n, n, OP_N, true, w.W, nb, OP_N, true,
                             w.A + k + (long long)kc * n, n, mone, one);
-    if ((st = gemm_launch(g2, s))) return st;
+    if ((st = gemm_w(w, g2, s))) return st;
   }
   return ST_NONE;
 }
@@ -866,7 +912,7 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
 static int qr_form_q(QRWork& w, cplx* Q, hipStream_t s) {
   const int n = w.n, nb = w.nb;
   int st;
-  if ((st = launch_identity(Q, n, s))) return st;
+  if ((st = launch_identity(Q, n, w.nchain, w.cs, s))) return st;
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   // Y2 = V T for every panel (V is zero above each panel's diagonal block)
   if (n % nb == 0) {
@@ -875,14 +921,14 @@ static int qr_form_q(QRWork& w, cplx* Q, hipStream_t s) {
     gy.strideA = (long long)n * nb;
     gy.strideB = (long long)nb * nb;
     gy.strideC = (long long)n * nb;
-    if ((st = gemm_launch(gy, s))) return st;
+    if ((st = gemm_w(w, gy, s))) return st;
   } else {
     for (int b = 0; b < w.nblk; ++b) {
       const int k = b * nb;
       const int ncol = (k + nb <= n) ? nb : n - k;
       GemmArgs gy = gemm_args(n - k, ncol, ncol, w.Vw + k + (long long)k * n, n, OP_N, true,
                               w.T + (long long)b * nb * nb, nb, OP_N, true, w.Y2 + k + (long long)k * n, n, one, zero);
-      if ((st = gemm_launch(gy, s))) return st;
+      if ((st = gemm_w(w, gy, s))) return st;
     }
   }
   for (int b = w.nblk - 1; b >= 0; --b) {
@@ -892,11 +938,11 @@ static int qr_form_q(QRWork& w, cplx* Q, hipStream_t s) {
     // W = V^H Q(k:, k:)
     GemmArgs g1 = gemm_args(ncol, rows, rows, w.Vw + k + (long long)k * n, n, OP_C, true,
                             Q + k + (long long)k * n, n, OP_N, true, w.W, nb, one, zero);
-    if ((st = gemm_launch(g1, s))) return st;
+    if ((st = gemm_w(w, g1, s))) return st;
     // Q(k:, k:) -= Y2 W
     GemmArgs g2 = gemm_args(rows, rows, ncol, w.Y2 + k + (long long)k * n, n, OP_N, true, w.W, nb, OP_N, true,
                             Q + k + (long long)k * n, n, mone, one);
-    if ((st = gemm_launch(g2, s))) return st;
+    if ((st = gemm_w(w, g2, s))) return st;
   }
   return ST_NONE;
 }
@@ -906,25 +952,26 @@ int udv_decompose_dev(QRWork& w, const cplx* Usrc, const cplx* Dsrc, const cplx*
   const int n = w.n;
   int st;
   if ((st = qr_factor(w, Usrc, n, Dsrc, 0, s))) return st;
-  hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet, side_l ? 1 : 0,
-                     (const cplx*)nullptr, (const cplx*)nullptr, Dout, w.Rt, phase_out, detU_out);
-  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, w.A, n, side_l ? 1 : 0,
-                     (const cplx*)phase_out, w.Rt);
+  hipLaunchKernelGGL(qr_finish_kernel, dim3(1, 1, w.nchain), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet,
+                     side_l ? 1 : 0, (const cplx*)nullptr, (const cplx*)nullptr, Dout, w.Rt, phase_out, detU_out, w.cs);
+  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256), 1, w.nchain), dim3(256), 0, s, w.A, n,
+                     side_l ? 1 : 0, (const cplx*)phase_out, w.Rt, w.cs);
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   if (!side_l) {
     // V = Rt * P^T * V   (row i of P^T V is row perm[i] of V)
     GemmArgs g = gemm_args(n, n, n, w.Rt, n, OP_N, true, Vsrc, n, OP_N, true, Vout, n, one, zero);
     g.bmap = w.perm;
-    if ((st = gemm_launch(g, s))) return st;
+    if ((st = gemm_w(w, g, s))) return st;
   } else {
     // V = V P * Rt^H    (column i of V P is column perm[i] of V)
     GemmArgs g = gemm_args(n, n, n, Vsrc, n, OP_N, true, w.Rt, n, OP_C, true, Vout, n, one, zero);
     g.amap = w.perm;
-    if ((st = gemm_launch(g, s))) return st;
+    if ((st = gemm_w(w, g, s))) return st;
   }
   if ((st = qr_form_q(w, Uout, s))) return st;
-  hipLaunchKernelGGL(scale_col0_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, Uout, n, (const cplx*)phase_out);
+  hipLaunchKernelGGL(scale_col0_kernel, dim3(nblocks(n, 256), 1, w.nchain), dim3(256), 0, s, Uout, n,
+                     (const cplx*)phase_out, w.cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }
@@ -934,10 +981,10 @@ int qr_factor_cgr(QRWork& w, const cplx* TPUP, int nvar, const cplx* detUr, cons
   const int n = w.n;
   int st;
   if ((st = qr_factor(w, TPUP, n, nullptr, nvar == 1 ? 0 : 1, s))) return st;
-  hipLaunchKernelGGL(qr_finish_kernel, dim3(1), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet,
-                     nvar == 1 ? 2 : 3, detUr, detUl, DUP, w.Rt, z_out, (cplx*)nullptr);
-  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, s, w.A, n, nvar == 1 ? 2 : 3,
-                     (const cplx*)z_out, w.Rt);
+  hipLaunchKernelGGL(qr_finish_kernel, dim3(1, 1, w.nchain), dim3(PT), 0, s, w.A, n, w.nblk, w.perm, w.pdet,
+                     nvar == 1 ? 2 : 3, detUr, detUl, DUP, w.Rt, z_out, (cplx*)nullptr, w.cs);
+  hipLaunchKernelGGL(qr_rt_kernel, dim3(nblocks((long long)n * n, 256), 1, w.nchain), dim3(256), 0, s, w.A, n,
+                     nvar == 1 ? 2 : 3, (const cplx*)z_out, w.Rt, w.cs);
   HIP_OK(hipGetLastError());
   if ((st = qr_form_q(w, Qout, s))) return st;
   return ST_NONE;
@@ -946,8 +993,8 @@ int qr_factor_cgr(QRWork& w, const cplx* TPUP, int nvar, const cplx* detUr, cons
 int tri_inverse_dev(QRWork& w, const cplx* Rt, cplx* Rinv, hipStream_t s) {
   const int n = w.n, nb = w.nb;
   int st;
-  if ((st = launch_fill(Rinv, (long long)n * n, make_double2(0.0, 0.0), s))) return st;
-  hipLaunchKernelGGL(trinv_diag_kernel, dim3(w.nblk), dim3(PT), 0, s, Rt, n, nb, Rinv);
+  if ((st = launch_fill(Rinv, (long long)n * n, make_double2(0.0, 0.0), w.nchain, w.cs, s))) return st;
+  hipLaunchKernelGGL(trinv_diag_kernel, dim3(w.nblk, 1, w.nchain), dim3(PT), 0, s, Rt, n, nb, Rinv, w.cs);
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   for (int b = 1; b < w.nblk; ++b) {
@@ -956,11 +1003,11 @@ int tri_inverse_dev(QRWork& w, const cplx* Rt, cplx* Rinv, hipStream_t s) {
     // T1 = Rt(0:k, k:k+m) Rinv(k:k+m, k:k+m)
     GemmArgs g1 = gemm_args(k, m, m, Rt + (long long)k * n, n, OP_N, true, Rinv + k + (long long)k * n, n, OP_N,
                             true, w.Tmp, n, one, zero);
-    if ((st = gemm_launch(g1, s))) return st;
+    if ((st = gemm_w(w, g1, s))) return st;
     // Rinv(0:k, k:k+m) = -Rinv(0:k, 0:k) T1
     GemmArgs g2 = gemm_args(k, m, k, Rinv, n, OP_N, true, w.Tmp, n, OP_N, true, Rinv + (long long)k * n, n, mone,
                             zero);
-    if ((st = gemm_launch(g2, s))) return st;
+    if ((st = gemm_w(w, g2, s))) return st;
   }
   return ST_NONE;
 }

@@ -146,6 +146,17 @@ module alfq_mod
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsweep
     end function
+    integer(c_int) function alfq_ctx_create_batch(descs, nchain, ctx) bind(C, name="alfq_ctx_create_batch")
+      import :: c_int, c_ptr, c_int32_t, alfq_model_desc
+      type(alfq_model_desc), intent(in) :: descs(*)
+      integer(c_int32_t), value :: nchain
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function alfq_set_chain(ctx, chain) bind(C, name="alfq_set_chain")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: chain
+    end function
     integer(c_int) function alfq_sweep_graphs(ctx, enable) bind(C, name="alfq_sweep_graphs")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -31,7 +31,7 @@ EXPORTS = [
     "alfq_init", "alfq_sweep", "alfq_sweep_timed", "alfq_sweep_many",
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
-    "alfq_sweep_graphs",
+    "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
 ]
 
 
@@ -102,6 +102,9 @@ def load_library(path: str = LIB_PATH):
         "alfq_precision_log": [P, dp, C.c_int64, C.POINTER(C.c_int64)],
         "alfq_kernel_timing": [P, C.c_int32],
         "alfq_sweep_graphs": [P, C.c_int32],
+        "alfq_ctx_create_batch": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(P)],
+        "alfq_set_chain": [P, C.c_int32],
+        "alfq_batch_size": [P, C.POINTER(C.c_int32)],
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
                       C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32],
@@ -131,6 +134,30 @@ def check(lib, code):
         raise AlfqError(code, lib.alfq_last_error().decode())
 
 
+def model_desc(model):
+    """alfq_model_desc of a host model, with the arrays it points to (keep them alive)."""
+    ops = model.Op_V
+    t_real = bool(model.Op_T.is_real)
+    if t_real:
+        expT = np.asfortranarray(np.real(model.Op_T.mat), dtype=np.float64)
+        expTi = np.asfortranarray(np.real(model.Op_T.invmat), dtype=np.float64)
+    else:
+        expT = np.asfortranarray(model.Op_T.mat, dtype=np.complex128)
+        expTi = np.asfortranarray(model.Op_T.invmat, dtype=np.complex128)
+    if any(o.N != 1 for o in ops):
+        raise NotImplementedError("the device sweep handles 1x1 interaction operators")
+    keep = dict(expT=expT, expTi=expTi,
+                P=np.array([int(o.P[0]) + 1 for o in ops], dtype=np.int32),
+                type=np.array([o.type for o in ops], dtype=np.int32),
+                E=np.array([float(o.E[0]) for o in ops], dtype=np.float64),
+                g=np.array([o.g for o in ops], dtype=np.complex128),
+                alpha=np.array([o.alpha for o in ops], dtype=np.complex128))
+    d = ModelDesc(model.Ndim, model.Ltrot, model.Nwrap, model.N_SUN, model.N_FL, len(ops), int(t_real),
+                  _dptr(keep["expT"]), _dptr(keep["expTi"]), _iptr(keep["P"]), _iptr(keep["type"]),
+                  _dptr(keep["E"]), _dptr(keep["g"]), _dptr(keep["alpha"]))
+    return d, keep
+
+
 class Engine:
     """One Markov chain on the GPU behind the C ABI.
 
@@ -140,30 +167,13 @@ class Engine:
     def __init__(self, model, delay: int | None = None):
         self.lib = load_library()
         self.model = model
-        ops = model.Op_V
-        n = model.Ndim
-        t_real = bool(model.Op_T.is_real)
-        if t_real:
-            self._expT = np.asfortranarray(np.real(model.Op_T.mat), dtype=np.float64)
-            self._expTi = np.asfortranarray(np.real(model.Op_T.invmat), dtype=np.float64)
-        else:
-            self._expT = np.asfortranarray(model.Op_T.mat, dtype=np.complex128)
-            self._expTi = np.asfortranarray(model.Op_T.invmat, dtype=np.complex128)
-        self._P = np.array([int(o.P[0]) + 1 for o in ops], dtype=np.int32)
-        self._type = np.array([o.type for o in ops], dtype=np.int32)
-        self._E = np.array([float(o.E[0]) for o in ops], dtype=np.float64)
-        self._g = np.array([o.g for o in ops], dtype=np.complex128)
-        self._alpha = np.array([o.alpha for o in ops], dtype=np.complex128)
-        if any(o.N != 1 for o in ops):
-            raise NotImplementedError("the device sweep handles 1x1 interaction operators")
-        d = ModelDesc(n, model.Ltrot, model.Nwrap, model.N_SUN, model.N_FL, len(ops), int(t_real),
-                      _dptr(self._expT), _dptr(self._expTi), _iptr(self._P), _iptr(self._type),
-                      _dptr(self._E), _dptr(self._g), _dptr(self._alpha))
+        d, self._keep = model_desc(model)
         self._desc = d
         h = C.c_void_p()
         check(self.lib, self.lib.alfq_ctx_create(C.byref(d), C.byref(h)))
         self.ctx = h
-        self.n = n
+        self.n = model.Ndim
+        self.nchain = 1
         if delay is not None:
             self.set_delay(delay)
 
@@ -331,6 +341,37 @@ def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None):
     check(lib, lib.alfq_gemm(M, N, K, opa.encode(), int(a_real), _dptr(Aa), max(Aa.shape[0], 1), opb.encode(),
                              int(b_real), _dptr(Ba), max(Ba.shape[0], 1), _dptr(al), _dptr(be), _dptr(Cc), max(M, 1)))
     return Cc
+
+
+class BatchEngine(Engine):
+    """Several independent Markov chains advancing in lockstep in one context: every
+    kernel launch covers all of them.  The per-chain accessors inherited from Engine
+    (fields, G, phase, random stream, control, ...) act on the chain selected with
+    ``chain(z)``; sweeps act on every chain."""
+
+    def __init__(self, models, delay: int | None = None):
+        self.lib = load_library()
+        self.models = list(models)
+        self.model = self.models[0]
+        descs, self._keep = [], []
+        for m in self.models:
+            d, k = model_desc(m)
+            descs.append(d)
+            self._keep.append(k)
+        arr = (ModelDesc * len(descs))(*descs)
+        self._descs = arr
+        h = C.c_void_p()
+        check(self.lib, self.lib.alfq_ctx_create_batch(arr, len(descs), C.byref(h)))
+        self.ctx = h
+        self.n = self.model.Ndim
+        self.nchain = len(descs)
+        if delay is not None:
+            self.set_delay(delay)
+
+    def chain(self, z):
+        """Select chain z for the per-chain accessors; returns self for chaining calls."""
+        self._c(self.lib.alfq_set_chain(self.ctx, int(z)))
+        return self
 
 
 def sweep_many(engines, nsweep=1):

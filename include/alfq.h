@@ -68,6 +68,17 @@ typedef struct alfq_control {
  * main.F90:588-605. */
 int alfq_ctx_create(const alfq_model_desc* desc, alfq_ctx** out);
 int alfq_ctx_destroy(alfq_ctx* ctx);
+/* Batches (no counterpart in the reference, which runs one chain per MPI rank): nchain
+ * independent Markov chains that advance in lockstep, every kernel launch covering all
+ * of them.  descs[z] describes chain z; Ndim, Ltrot, Nwrap, N_SUN, the operators' sites
+ * and field types and the type of T must agree, the couplings (U, ...) and exp(-dtau T)
+ * may differ.  alfq_ctx_create(desc) is the batch of one.  Sweeps, wraps and CGR act on
+ * every chain; the accessors (fields, G, PHASE, random stream, UDV reads, control,
+ * precision log) act on the chain chosen with alfq_set_chain (chain 0 initially).
+ * Random tapes and alfq_udv_set are single-chain only. */
+int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx** out);
+int alfq_set_chain(alfq_ctx* ctx, int32_t chain);
+int alfq_batch_size(alfq_ctx* ctx, int32_t* nchain);
 /* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */
 int alfq_set_delay(alfq_ctx* ctx, int32_t k);
 
