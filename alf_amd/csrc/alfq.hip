@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -966,6 +967,9 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
     if (udv_reset_dev(c, &c->var_st[k], 'L')) return fail("udv");
   if (hipEventCreate(&c->ev0) || hipEventCreate(&c->ev1)) return fail("events");
   if (hipStreamSynchronize(c->s)) return fail("sync");
+  // ALFQ_SWEEP_GRAPHS=0 launches every kernel from the host (kernel tracers that do not
+  // follow graph replays need this)
+  if (const char* e = std::getenv("ALFQ_SWEEP_GRAPHS")) c->graphs_enabled = std::atoi(e) != 0;
   *out = c;
   return ST_NONE;
 }

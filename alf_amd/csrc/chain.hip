@@ -760,8 +760,10 @@ int launch_chain(const ChainArgs& a, hipStream_t s, KTimer* t) {
     if (t && t->on) {
       if (t->used == t->ev.size()) {
         hipEvent_t e0, e1;
-        HIP_OK(hipEventCreate(&e0));
-        HIP_OK(hipEventCreate(&e1));
+        // no system-scope release when the timing events are recorded: they bracket one
+        // kernel on the device and nothing on the host reads its results in between
+        HIP_OK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+        HIP_OK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
         t->ev.emplace_back(e0, e1);
       }
       auto& pr = t->ev[t->used++];
