@@ -35,6 +35,120 @@ __device__ __forceinline__ void chain_operands(GemmArgs& g) {
   g.amap = mv(g.amap); g.bmap = mv(g.bmap); g.kdev = mv(g.kdev);
 }
 
+// One K stage of an LDS-tiled kernel's operands, held in registers from its global loads to
+// its LDS stores.  Gather indices and row scales are read first and every data load is then
+// issued back to back: no wait on one load sits between the issue of two others (a wait at
+// the join of an optional map lookup would otherwise serialise the whole stage).
+// Conjugation and scaling are applied when the stage is stored.
+template <int TM, int TN, int BK, bool A_CPLX, bool B_CPLX>
+struct Stage {
+  static constexpr int LA = TM * BK / NT, LB = BK * TN / NT;   // elements per thread
+  static_assert(LA >= 1 && LB >= 1, "a stage covers the workgroup");
+  double a_re[LA], a_im[LA], b_re[LB], b_im[LB];
+  cplx sc[LB];
+
+  __device__ __forceinline__ static void a_index(const GemmArgs& g, int tid, int p, int& i, int& kk) {
+    const int idx = tid + NT * p;
+    if (g.opA == OP_N) { i = idx % TM; kk = idx / TM; } else { kk = idx % BK; i = idx / BK; }
+  }
+  __device__ __forceinline__ static void b_index(const GemmArgs& g, int tid, int p, int& kk, int& j) {
+    const int idx = tid + NT * p;
+    if (g.opB == OP_N) { kk = idx % BK; j = idx / BK; } else { j = idx % TN; kk = idx / TN; }
+  }
+
+  __device__ __forceinline__ void load(const GemmArgs& g, const void* A, const void* B, int i0, int j0,
+                                       int K, int k0, int tid) {
+    int ka[LA], kb[LB];
+#pragma unroll
+    for (int p = 0; p < LA; ++p) {
+      int i, kk;
+      a_index(g, tid, p, i, kk);
+      ka[p] = (i0 + i < g.M && k0 + kk < K) ? k0 + kk : -1;
+    }
+#pragma unroll
+    for (int p = 0; p < LB; ++p) {
+      int kk, j;
+      b_index(g, tid, p, kk, j);
+      kb[p] = (k0 + kk < K && j0 + j < g.N) ? k0 + kk : -1;
+    }
+    if (g.bs) {
+#pragma unroll
+      for (int p = 0; p < LB; ++p) sc[p] = kb[p] >= 0 ? g.bs[kb[p]] : make_double2(0.0, 0.0);
+    }
+    if (g.amap) {
+#pragma unroll
+      for (int p = 0; p < LA; ++p) if (ka[p] >= 0) ka[p] = g.amap[ka[p]];
+    }
+    if (g.bmap) {
+#pragma unroll
+      for (int p = 0; p < LB; ++p) if (kb[p] >= 0) kb[p] = g.bmap[kb[p]];
+    }
+#pragma unroll
+    for (int p = 0; p < LA; ++p) {
+      int i, kk;
+      a_index(g, tid, p, i, kk);
+      const long long gi = i0 + i;
+      double re = 0.0, im = 0.0;
+      if (ka[p] >= 0) {
+        const long long off = (g.opA == OP_N) ? gi + (long long)ka[p] * g.lda : ka[p] + gi * g.lda;
+        if (A_CPLX) {
+          const cplx v = ((const cplx*)A)[off];
+          re = v.x;
+          im = v.y;
+        } else {
+          re = ((const double*)A)[off];
+        }
+      }
+      a_re[p] = re;
+      a_im[p] = im;
+    }
+#pragma unroll
+    for (int p = 0; p < LB; ++p) {
+      int kk, j;
+      b_index(g, tid, p, kk, j);
+      const long long gj = j0 + j;
+      double re = 0.0, im = 0.0;
+      if (kb[p] >= 0) {
+        const long long off = (g.opB == OP_N) ? kb[p] + gj * g.ldb : gj + (long long)kb[p] * g.ldb;
+        if (g.b_cplx) {
+          const cplx v = ((const cplx*)B)[off];
+          re = v.x;
+          im = v.y;
+        } else {
+          re = ((const double*)B)[off];
+        }
+      }
+      b_re[p] = re;
+      b_im[p] = im;
+    }
+  }
+
+  __device__ __forceinline__ void store(const GemmArgs& g, int tid, double (*As_re)[TM + 1],
+                                        double (*As_im)[TM + 1], double (*Bs_re)[TN + 1],
+                                        double (*Bs_im)[TN + 1]) const {
+#pragma unroll
+    for (int p = 0; p < LA; ++p) {
+      int i, kk;
+      a_index(g, tid, p, i, kk);
+      As_re[kk][i] = a_re[p];
+      if (A_CPLX) As_im[kk][i] = (g.opA == OP_C) ? -a_im[p] : a_im[p];
+    }
+#pragma unroll
+    for (int p = 0; p < LB; ++p) {
+      int kk, j;
+      b_index(g, tid, p, kk, j);
+      double re = b_re[p], im = (g.opB == OP_C) ? -b_im[p] : b_im[p];
+      if (g.bs) {
+        const double r2 = re * sc[p].x - im * sc[p].y;
+        im = re * sc[p].y + im * sc[p].x;
+        re = r2;
+      }
+      Bs_re[kk][j] = re;
+      if (B_CPLX) Bs_im[kk][j] = im;
+    }
+  }
+};
+
 template <bool A_CPLX, bool B_CPLX, int WM, int WN>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   constexpr int KS = 4 / (WM * WN);
@@ -59,14 +173,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   }
   if (K <= 0 && g.beta.x == 1.0 && g.beta.y == 0.0) return;  // nothing to add
 
-  const double* Ar = (const double*)g.A;
-  const cplx* Ac = (const cplx*)g.A;
-  const double* Br = (const double*)g.B;
-  const cplx* Bc = (const cplx*)g.B;
-  if (g.batch > 1) {
-    Ar += z * g.strideA; Ac += z * g.strideA;
-    Br += z * g.strideB; Bc += z * g.strideB;
-  }
+  const long long za = g.batch > 1 ? z * g.strideA : 0, zb = g.batch > 1 ? z * g.strideB : 0;
+  const void* A = A_CPLX ? (const void*)((const cplx*)g.A + za) : (const void*)((const double*)g.A + za);
+  const void* B = g.b_cplx ? (const void*)((const cplx*)g.B + zb) : (const void*)((const double*)g.B + zb);
   cplx* C = (cplx*)g.C + (g.batch > 1 ? z * g.strideC : 0);
 
   const int wave = tid >> 6, lane = tid & 63;
@@ -77,87 +186,13 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   v4d acc_re = {0, 0, 0, 0};
   v4d acc_im = {0, 0, 0, 0};
 
-  // global -> register loads of one K stage; consecutive threads walk the contiguous
-  // index of each operand's storage
-  double ra_re[LA], ra_im[LA], rb_re[LB], rb_im[LB];
-  auto a_index = [&](int p, int& i, int& kk) {
-    const int idx = tid + NT * p;
-    if (g.opA == OP_N) { i = idx % TM; kk = idx / TM; } else { kk = idx % BK; i = idx / BK; }
-  };
-  auto b_index = [&](int p, int& kk, int& j) {
-    const int idx = tid + NT * p;
-    if (g.opB == OP_N) { kk = idx % BK; j = idx / BK; } else { j = idx % TN; kk = idx / TN; }
-  };
-  auto load_tile = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < LA; ++p) {
-      int i, kk;
-      a_index(p, i, kk);
-      const int gi = i0 + i, gk = k0 + kk;
-      double re = 0.0, im = 0.0;
-      if (gi < g.M && gk < K) {
-        const int kcol = g.amap ? g.amap[gk] : gk;
-        const long long off = (g.opA == OP_N) ? gi + (long long)kcol * g.lda : kcol + (long long)gi * g.lda;
-        if (A_CPLX) {
-          const cplx v = Ac[off];
-          re = v.x;
-          im = (g.opA == OP_C) ? -v.y : v.y;
-        } else {
-          re = Ar[off];
-        }
-      }
-      ra_re[p] = re;
-      ra_im[p] = im;
-    }
-#pragma unroll
-    for (int p = 0; p < LB; ++p) {
-      int kk, j;
-      b_index(p, kk, j);
-      const int gk = k0 + kk, gj = j0 + j;
-      double re = 0.0, im = 0.0;
-      if (gk < K && gj < g.N) {
-        const int krow = g.bmap ? g.bmap[gk] : gk;
-        const long long off = (g.opB == OP_N) ? krow + (long long)gj * g.ldb : gj + (long long)krow * g.ldb;
-        if (g.b_cplx) {
-          const cplx v = Bc[off];
-          re = v.x;
-          im = (g.opB == OP_C) ? -v.y : v.y;
-        } else {
-          re = Br[off];
-        }
-        if (g.bs) {
-          const cplx sc = g.bs[gk];
-          const double r2 = re * sc.x - im * sc.y;
-          im = re * sc.y + im * sc.x;
-          re = r2;
-        }
-      }
-      rb_re[p] = re;
-      rb_im[p] = im;
-    }
-  };
-  auto store_tile = [&]() {
-#pragma unroll
-    for (int p = 0; p < LA; ++p) {
-      int i, kk;
-      a_index(p, i, kk);
-      As_re[kk][i] = ra_re[p];
-      if (A_CPLX) As_im[kk][i] = ra_im[p];
-    }
-#pragma unroll
-    for (int p = 0; p < LB; ++p) {
-      int kk, j;
-      b_index(p, kk, j);
-      Bs_re[kk][j] = rb_re[p];
-      if (B_CPLX) Bs_im[kk][j] = rb_im[p];
-    }
-  };
+  Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
 
-  if (K > 0) load_tile(0);
+  if (K > 0) st.load(g, A, B, i0, j0, K, 0, tid);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    store_tile();
+    st.store(g, tid, As_re, As_im, Bs_re, Bs_im);
     __syncthreads();
-    if (k0 + BK < K) load_tile(k0 + BK);   // next stage's loads overlap this stage's MFMAs
+    if (k0 + BK < K) st.load(g, A, B, i0, j0, K, k0 + BK, tid);   // in flight during this stage's MFMAs
 #pragma unroll
     for (int kq = 0; kq < 16; kq += 4) {
       const int kk = 16 * ks + kq;
@@ -219,6 +254,117 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   }
 }
 
+// Register-blocked 64 x 64 tile for grids with several tiles per CU (batched chains): 4
+// waves in a 2 x 2 arrangement, each wave a 32 x 32 sub-tile held as 2 x 2 MFMA blocks, so
+// every operand fragment read from LDS feeds two blocks and a K stage of 16 carries 4x the
+// MFMA work of the 32 x 32 tile per barrier.  The next stage's global loads are in flight
+// (in registers) while the current stage is multiplied out of LDS.
+template <bool A_CPLX, bool B_CPLX>
+__global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
+  constexpr int TM = 64, TN = 64, BK = 16;
+  __shared__ double As_re[BK][TM + 1];
+  __shared__ double As_im[A_CPLX ? BK : 1][TM + 1];
+  __shared__ double Bs_re[BK][TN + 1];
+  __shared__ double Bs_im[B_CPLX ? BK : 1][TN + 1];
+
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * TM;
+  const int j0 = blockIdx.y * TN;
+  chain_operands(g);
+  const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
+  int K = g.K;
+  if (g.kdev) {
+    const int kd = *g.kdev;
+    K = kd < K ? kd : K;
+  }
+  if (K <= 0 && g.beta.x == 1.0 && g.beta.y == 0.0) return;  // nothing to add
+
+  const long long za = g.batch > 1 ? z * g.strideA : 0, zb = g.batch > 1 ? z * g.strideB : 0;
+  const void* A = A_CPLX ? (const void*)((const cplx*)g.A + za) : (const void*)((const double*)g.A + za);
+  const void* B = g.b_cplx ? (const void*)((const cplx*)g.B + zb) : (const void*)((const double*)g.B + zb);
+  cplx* C = (cplx*)g.C + (g.batch > 1 ? z * g.strideC : 0);
+
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4d acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc_re[a][b] = v4d{0, 0, 0, 0};
+      acc_im[a][b] = v4d{0, 0, 0, 0};
+    }
+
+  Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
+
+  if (K > 0) st.load(g, A, B, i0, j0, K, 0, tid);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    st.store(g, tid, As_re, As_im, Bs_re, Bs_im);
+    __syncthreads();
+    if (k0 + BK < K) st.load(g, A, B, i0, j0, K, k0 + BK, tid);
+#pragma unroll
+    for (int kq = 0; kq < BK; kq += 4) {
+      const int kk = kq + lk;
+      double a_re[2], a_im[2], b_re[2], b_im[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        a_re[a] = As_re[kk][wr + 16 * a + lr];
+        if (A_CPLX) a_im[a] = As_im[kk][wr + 16 * a + lr];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        b_re[b] = Bs_re[kk][wc + 16 * b + lr];
+        if (B_CPLX) b_im[b] = Bs_im[kk][wc + 16 * b + lr];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re[a], b_re[b], acc_re[a][b], 0, 0, 0);
+          if (B_CPLX) acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re[a], b_im[b], acc_im[a][b], 0, 0, 0);
+        }
+      if (A_CPLX) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const double na = -a_im[a];
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            if (B_CPLX) acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(na, b_im[b], acc_re[a][b], 0, 0, 0);
+            acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im[a], b_re[b], acc_im[a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // D(i, j) of a 16 x 16 block in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int gj = j0 + wc + 16 * b + lr;
+    if (gj >= g.N) continue;
+    const cplx csj = g.cs ? g.cs[gj] : make_double2(1.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * a + 4 * r + lk;
+        if (gi >= g.M) continue;
+        cplx v = make_double2(acc_re[a][b][r], (A_CPLX || B_CPLX) ? acc_im[a][b][r] : 0.0);
+        v = cmul(g.alpha, v);
+        if (g.rs) v = cmul(g.rs[gi], v);
+        if (g.rs_real) v = cscale(v, g.rs_real[gi]);
+        if (g.cs) v = cmul(v, csj);
+        const long long off = gi + (long long)gj * g.ldc;
+        if (g.beta.x != 0.0 || g.beta.y != 0.0) v = cadd(v, cmul(g.beta, C[off]));
+        C[off] = v;
+      }
+    }
+  }
+}
+
 inline long long cdivl(long long a, long long b) { return (a + b - 1) / b; }
 inline unsigned gemm_zdim(const GemmArgs& g) {
   return (unsigned)((g.batch > 1 ? g.batch : 1) * (g.nchain > 1 ? g.nchain : 1));
@@ -232,6 +378,15 @@ void launch_shape(const GemmArgs& g, hipStream_t s) {
   else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false, WM, WN>), grid, dim3(NT), 0, s, g);
   else if (bc) hipLaunchKernelGGL((gemm_kernel<false, true, WM, WN>), grid, dim3(NT), 0, s, g);
   else hipLaunchKernelGGL((gemm_kernel<false, false, WM, WN>), grid, dim3(NT), 0, s, g);
+}
+
+void launch_big(const GemmArgs& g, hipStream_t s) {
+  dim3 grid((unsigned)cdivl(g.M, 64), (unsigned)cdivl(g.N, 64), gemm_zdim(g));
+  const bool bc = g.b_cplx || g.bs != nullptr;
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true>), grid, dim3(NT), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false>), grid, dim3(NT), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gemm_big_kernel<false, false>), grid, dim3(NT), 0, s, g);
 }
 
 
@@ -273,51 +428,65 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   const int gi = i0 + lr, gj = j0 + lr;
   const bool row_ok = gi < g.M, col_ok = gj < g.N;
 
-  auto load_a = [&](int k) -> cplx {
-    cplx v = make_double2(0.0, 0.0);
-    if (row_ok && k < ke) {
-      const int kc = g.amap ? g.amap[k] : k;
-      const long long off = (g.opA == OP_N) ? gi + (long long)kc * g.lda : kc + (long long)gi * g.lda;
-      if (A_CPLX) {
-        v = Ac[off];
-        if (g.opA == OP_C) v.y = -v.y;
-      } else {
-        v.x = Ar[off];
+  // One batch of U k-steps in the MFMA operand layout.  Gather indices and row scales are
+  // read first, then every data load is issued back to back; conjugation and scaling are
+  // applied when the batch moves into the MFMA operands.
+  constexpr int U = NW >= 4 ? 8 : 4;   // k-steps per batch: one batch covers a wave's K at Ndim = 256
+  auto load_batch = [&](int kbase, cplx* a, cplx* b, cplx* sc) {
+    int ia[U], ib[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kbase + 4 * u + lk;
+      ia[u] = (row_ok && k < ke) ? k : -1;
+      ib[u] = (col_ok && k < ke) ? k : -1;
+    }
+    if (g.bs) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) sc[u] = ib[u] >= 0 ? g.bs[ib[u]] : make_double2(0.0, 0.0);
+    }
+    if (g.amap) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) if (ia[u] >= 0) ia[u] = g.amap[ia[u]];
+    }
+    if (g.bmap) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) if (ib[u] >= 0) ib[u] = g.bmap[ib[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = make_double2(0.0, 0.0);
+      if (ia[u] >= 0) {
+        const long long off = (g.opA == OP_N) ? gi + (long long)ia[u] * g.lda : ia[u] + (long long)gi * g.lda;
+        if (A_CPLX) a[u] = Ac[off];
+        else a[u].x = Ar[off];
       }
     }
-    return v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      b[u] = make_double2(0.0, 0.0);
+      if (ib[u] >= 0) {
+        const long long off = (g.opB == OP_N) ? ib[u] + (long long)gj * g.ldb : gj + (long long)ib[u] * g.ldb;
+        if (g.b_cplx) b[u] = Bc[off];
+        else b[u].x = Br[off];
+      }
+    }
   };
-  auto load_b = [&](int k) -> cplx {
-    cplx v = make_double2(0.0, 0.0);
-    if (col_ok && k < ke) {
-      const int kr = g.bmap ? g.bmap[k] : k;
-      const long long off = (g.opB == OP_N) ? kr + (long long)gj * g.ldb : gj + (long long)kr * g.ldb;
-      if (g.b_cplx) {
-        v = Bc[off];
-        if (g.opB == OP_C) v.y = -v.y;
-      } else {
-        v.x = Br[off];
-      }
-      if (g.bs) v = cmul(v, g.bs[k]);
+  auto finish = [&](cplx* a, cplx* b, const cplx* sc) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (A_CPLX && g.opA == OP_C) a[u].y = -a[u].y;
+      if (g.opB == OP_C) b[u].y = -b[u].y;
+      if (g.bs) b[u] = cmul(b[u], sc[u]);
     }
-    return v;
   };
 
   v4d acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
-  constexpr int U = NW >= 4 ? 8 : 4;   // k-steps per batch: one batch covers a wave's K at Ndim = 256
-  cplx av[U], bv[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    av[u] = load_a(kb + 4 * u + lk);
-    bv[u] = load_b(kb + 4 * u + lk);
-  }
+  cplx av[U], bv[U], sv[U];
+  load_batch(kb, av, bv, sv);
+  finish(av, bv, sv);
   for (int k0 = kb; k0 < ke; k0 += 4 * U) {
-    cplx an[U], bn[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      an[u] = load_a(k0 + 4 * U + 4 * u + lk);
-      bn[u] = load_b(k0 + 4 * U + 4 * u + lk);
-    }
+    cplx an[U], bn[U], sn[U];
+    load_batch(k0 + 4 * U, an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc_re, 0, 0, 0);
@@ -327,6 +496,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
         acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, acc_im, 0, 0, 0);
       }
     }
+    finish(an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) { av[u] = an[u]; bv[u] = bn[u]; }
   }
@@ -421,6 +591,15 @@ int gemm_launch(const GemmArgs& g, hipStream_t s) {
     else launch_direct<1>(g, s);
     HIP_OK(hipGetLastError());
     return ST_NONE;
+  }
+  {
+    // two or more 64 x 64 tiles per CU, at least 3/4 of each tile's area inside the product
+    const long long tm = cdivl(g.M, 64), tn = cdivl(g.N, 64);
+    if (tm * tn * b >= 512 && 4LL * g.M * g.N >= 3LL * (64 * tm) * (64 * tn)) {
+      launch_big(g, s);
+      HIP_OK(hipGetLastError());
+      return ST_NONE;
+    }
   }
   switch (gemm_tile_shape(g.M, g.N, g.K, (int)gemm_zdim(g))) {
     case 0: launch_shape<2, 2>(g, s); break;

@@ -34,7 +34,9 @@ def engine(rt, ham, f, u, delay=8):
 # the shapes reach all three tilings: 32x32 (16x16x16, 300x17x5), 16x32 with K split in 2
 # (32x32x32, 37x50x29, 256x224x32) and 16x16 with K split in 4 (the others)
 @pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (37, 50, 29), (100, 36, 64), (256, 256, 256),
-                                   (256, 224, 32), (32, 224, 256), (16, 200, 300), (300, 17, 5)])
+                                   (256, 224, 32), (32, 224, 256), (16, 200, 300), (300, 17, 5),
+                                   # grids of >= 512 tiles of 64 x 64 (the batched-chain tiling), one ragged
+                                   (2048, 1024, 48), (1100, 1900, 37)])
 def test_gemm_against_numpy(rt, shape):
     M, N, K = shape
     r = np.random.default_rng(M * 7 + N)
@@ -311,6 +313,37 @@ def test_batched_chains_match_single_chains(rt):
         e.set_philox(100 + z, z)
         e.init()
         e.sweep(2)
+        b.chain(z)
+        assert np.array_equal(b.get_fields(), e.get_fields())
+        assert rel(b.get_green(), e.get_green()) < TOL
+        assert abs(b.get_phase() - e.get_phase()) < TOL
+        cb, ce = b.control(), e.control()
+        assert cb["ACC_up"] == ce["ACC_up"] and cb["NC_up"] == ce["NC_up"]
+        assert cb["XMAXG"] < 1e-8
+        e.close()
+    b.close()
+
+
+def test_batched_chains_at_full_tile_grids(rt):
+    """32 chains of Ndim = 256: every full-size product of the batch runs on the 64 x 64
+    tiling (>= 512 tiles per launch).  Chains 0, 13 and 31 follow the trajectories of the
+    same chains run one per context (16 x 16 and 32 x 32 tilings there)."""
+    h_a = hubbard(L1=16, L2=16, beta=1.0, nwrap=10, U=4.0)
+    h_b = hubbard(L1=16, L2=16, beta=1.0, nwrap=10, U=3.0)
+    hams = [h_a if z % 2 == 0 else h_b for z in range(32)]
+    fs = [fields_for(h, 200 + z) for z, h in enumerate(hams)]
+    b = rt.BatchEngine(hams, delay=32)
+    for z, f in enumerate(fs):
+        b.chain(z).set_fields(f)
+        b.chain(z).set_philox(7, z)
+    b.init()
+    b.sweep(1)
+    for z in (0, 13, 31):
+        e = rt.Engine(hams[z], delay=32)
+        e.set_fields(fs[z])
+        e.set_philox(7, z)
+        e.init()
+        e.sweep(1)
         b.chain(z)
         assert np.array_equal(b.get_fields(), e.get_fields())
         assert rel(b.get_green(), e.get_green()) < TOL
