@@ -157,6 +157,7 @@ struct alfq_ctx {
   int n = 0, L = 0, W = 0, nstm = 0, n_op = 0, n_sun = 0;
   bool t_real = true;
   int K = 32;
+  int prec = ALFQ_PREC_FP64;             // arithmetic of the hopping wraps of G
   hipStream_t s = nullptr;
   // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
   int B = 1;
@@ -229,11 +230,11 @@ struct alfq_ctx {
   // sweeps replayed as HIP graphs, keyed by the host state they start from
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
-    int var_r, var_l, rounds, K, rng_mode;
+    int var_r, var_l, rounds, K, rng_mode, prec;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
-             K == o.K && rng_mode == o.rng_mode && var_st == o.var_st && refs == o.refs;
+             K == o.K && rng_mode == o.rng_mode && prec == o.prec && var_st == o.var_st && refs == o.refs;
     }
   };
   struct SweepGraph {
@@ -400,12 +401,14 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
   // H = exp(-dtau T) G
   GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
                           ONE, ZERO);
+  g1.prec = c->prec;
   if ((st = gemm_c(c, g1))) return st;
   // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
   GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
                           c->n, ONE, ZERO);
   g2.rs = c->dv_d;
   g2.cs = c->dv_dinv;
+  g2.prec = c->prec;
   if ((st = gemm_c(c, g2))) return st;
   std::swap(c->G, c->Gtmp);
   if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
@@ -425,10 +428,12 @@ int wrapgrdo_dev(alfq_ctx* c, int ntau) {
                           ONE, ZERO);
   g1.rs = c->dv_dinv;
   g1.bs = c->dv_d;
+  g1.prec = c->prec;
   if ((st = gemm_c(c, g1))) return st;
   // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
   GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
                           c->n, ONE, ZERO);
+  g2.prec = c->prec;
   if ((st = gemm_c(c, g2))) return st;
   std::swap(c->G, c->Gtmp);
   return ST_NONE;
@@ -608,6 +613,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   alfq_ctx::HostState h;
   h.G = c->G; h.Gtmp = c->Gtmp; h.u = c->u;
   h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
+  h.prec = c->prec;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -1019,6 +1025,16 @@ int alfq_set_delay(alfq_ctx* c, int32_t k) {
   return ST_NONE;
 }
 
+int alfq_set_precision(alfq_ctx* c, int32_t mode) {
+  if (!c) return ST_GENERIC;
+  if (mode != ALFQ_PREC_FP64 && mode != ALFQ_PREC_FP32_WRAPS) {
+    alfq_set_error("precision: unknown mode");
+    return ST_GENERIC;
+  }
+  c->prec = mode;
+  return ST_NONE;
+}
+
 int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) {
   if (!c || !f) return ST_GENERIC;
   if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
@@ -1417,7 +1433,7 @@ int alfq_kernel_timing_get(alfq_ctx* c, double* total_ms, int64_t* launches) {
 
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha, const double* beta,
-              double* Cm, int32_t ldc) {
+              double* Cm, int32_t ldc, int32_t prec) {
   if (M < 0 || N < 0 || K < 0 || !A || !B || !Cm || !alpha || !beta) { alfq_set_error("gemm: bad arguments"); return ST_GENERIC; }
   auto op = [](char o) { return (o == 'T' || o == 't') ? OP_T : ((o == 'C' || o == 'c') ? OP_C : OP_N); };
   const int oa = op(opa), ob = op(opb);
@@ -1434,6 +1450,7 @@ int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, cons
   HIP_OK(hipMemcpy(dC, Cm, csz, hipMemcpyHostToDevice));
   GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
                          make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
+  g.prec = prec == ALFQ_PREC_FP32_WRAPS ? 1 : 0;
   int st = gemm_launch(g, 0);
   if (!st) {
     HIP_OK(hipDeviceSynchronize());

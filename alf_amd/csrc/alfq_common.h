@@ -110,6 +110,7 @@ struct GemmArgs {
   long long strideA, strideB, strideC;  // element strides between problems
   int nchain;               // chains of the batch (grid z = batch * nchain)
   long long cstride;        // byte stride between the chains' arenas (applies to every pointer)
+  int prec;                 // 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA and accumulation
 };
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,

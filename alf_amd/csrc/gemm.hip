@@ -24,6 +24,28 @@ namespace {
 constexpr int NT = 256;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// The two MFMA arithmetics of the kernels below (GemmArgs::prec 0 and 1).  Both take A as
+// lane = row + 16 k and B as lane = column + 16 k (k = 0..3); they differ in where the
+// 16 x 16 result lands:
+//   FP64 v_mfma_f64_16x16x4f64: D(i, j) in lane j + 16 (i % 4), register i / 4
+//   FP32 v_mfma_f32_16x16x4f32: D(i, j) in lane j + 16 (i / 4), register i % 4
+// With FP32 the operands are rounded to float when staged and the products accumulate in
+// float; scalings, alpha and beta are applied to the result in double.
+template <bool F32> struct Mma;
+template <> struct Mma<false> {
+  typedef double E;
+  typedef v4d Acc;
+  __device__ __forceinline__ static Acc mac(E a, E b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ __forceinline__ static int row(int r, int lk) { return 4 * r + lk; }
+};
+template <> struct Mma<true> {
+  typedef float E;
+  typedef v4f Acc;
+  __device__ __forceinline__ static Acc mac(E a, E b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ __forceinline__ static int row(int r, int lk) { return 4 * lk + r; }
+};
 
 // moves every operand pointer to the arena of the chain blockIdx.z / batch
 __device__ __forceinline__ void chain_operands(GemmArgs& g) {
@@ -123,15 +145,15 @@ struct Stage {
     }
   }
 
-  __device__ __forceinline__ void store(const GemmArgs& g, int tid, double (*As_re)[TM + 1],
-                                        double (*As_im)[TM + 1], double (*Bs_re)[TN + 1],
-                                        double (*Bs_im)[TN + 1]) const {
+  template <typename E>
+  __device__ __forceinline__ void store(const GemmArgs& g, int tid, E (*As_re)[TM + 1], E (*As_im)[TM + 1],
+                                        E (*Bs_re)[TN + 1], E (*Bs_im)[TN + 1]) const {
 #pragma unroll
     for (int p = 0; p < LA; ++p) {
       int i, kk;
       a_index(g, tid, p, i, kk);
-      As_re[kk][i] = a_re[p];
-      if (A_CPLX) As_im[kk][i] = (g.opA == OP_C) ? -a_im[p] : a_im[p];
+      As_re[kk][i] = (E)a_re[p];
+      if (A_CPLX) As_im[kk][i] = (E)((g.opA == OP_C) ? -a_im[p] : a_im[p]);
     }
 #pragma unroll
     for (int p = 0; p < LB; ++p) {
@@ -143,22 +165,24 @@ struct Stage {
         im = re * sc[p].y + im * sc[p].x;
         re = r2;
       }
-      Bs_re[kk][j] = re;
-      if (B_CPLX) Bs_im[kk][j] = im;
+      Bs_re[kk][j] = (E)re;
+      if (B_CPLX) Bs_im[kk][j] = (E)im;
     }
   }
 };
 
-template <bool A_CPLX, bool B_CPLX, int WM, int WN>
+template <bool A_CPLX, bool B_CPLX, int WM, int WN, bool F32>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
+  typedef Mma<F32> M;
+  typedef typename M::E E;
   constexpr int KS = 4 / (WM * WN);
   constexpr int TM = 16 * WM, TN = 16 * WN, BK = 16 * KS;
   constexpr int LA = TM * BK / NT, LB = BK * TN / NT;   // elements per thread per stage
   static_assert(WM * WN * KS == 4 && LA >= 1 && LB >= 1, "4 waves per tile");
-  __shared__ double As_re[BK][TM + 1];
-  __shared__ double As_im[A_CPLX ? BK : 1][TM + 1];
-  __shared__ double Bs_re[BK][TN + 1];
-  __shared__ double Bs_im[B_CPLX ? BK : 1][TN + 1];
+  __shared__ E As_re[BK][TM + 1];
+  __shared__ E As_im[A_CPLX ? BK : 1][TM + 1];
+  __shared__ E Bs_re[BK][TN + 1];
+  __shared__ E Bs_im[B_CPLX ? BK : 1][TN + 1];
   __shared__ cplx red[KS > 1 ? KS - 1 : 1][KS > 1 ? TM * TN : 1];
 
   const int tid = threadIdx.x;
@@ -183,8 +207,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   const int wr = 16 * wm, wc = 16 * wn;
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4d acc_re = {0, 0, 0, 0};
-  v4d acc_im = {0, 0, 0, 0};
+  typename M::Acc acc_re = {0, 0, 0, 0};
+  typename M::Acc acc_im = {0, 0, 0, 0};
 
   Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
 
@@ -196,33 +220,33 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int kq = 0; kq < 16; kq += 4) {
       const int kk = 16 * ks + kq;
-      const double a_re = As_re[kk + lk][wr + lr];
-      const double b_re = Bs_re[kk + lk][wc + lr];
-      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re, b_re, acc_re, 0, 0, 0);
+      const E a_re = As_re[kk + lk][wr + lr];
+      const E b_re = Bs_re[kk + lk][wc + lr];
+      acc_re = M::mac(a_re, b_re, acc_re);
       if (B_CPLX) {
-        const double b_im = Bs_im[kk + lk][wc + lr];
-        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re, b_im, acc_im, 0, 0, 0);
+        const E b_im = Bs_im[kk + lk][wc + lr];
+        acc_im = M::mac(a_re, b_im, acc_im);
         if (A_CPLX) {
-          const double a_im = As_im[kk + lk][wr + lr];
-          acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_im, b_im, acc_re, 0, 0, 0);
-          acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im, b_re, acc_im, 0, 0, 0);
+          const E a_im = As_im[kk + lk][wr + lr];
+          acc_re = M::mac(-a_im, b_im, acc_re);
+          acc_im = M::mac(a_im, b_re, acc_im);
         }
       } else if (A_CPLX) {
-        const double a_im = As_im[kk + lk][wr + lr];
-        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im, b_re, acc_im, 0, 0, 0);
+        const E a_im = As_im[kk + lk][wr + lr];
+        acc_im = M::mac(a_im, b_re, acc_im);
       }
     }
     __syncthreads();
   }
 
-  // ---- epilogue: v_mfma_f64_16x16x4f64 leaves D(i, j) in lane j + 16*(i%4), register i/4,
-  //      so lane holds rows wr + 4*r + lane/16 (r = 0..3) of column wc + lane%16 ----
+  // ---- epilogue: register r of a lane holds row wr + M::row(r, lane / 16) of column
+  //      wc + lane % 16 ----
   if (KS > 1) {
     // waves ks >= 1 park their partial tiles; the ks = 0 waves add them in order ks = 1, 2, ...
     if (ks > 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        red[ks - 1][(wr + 4 * r + lk) + TM * (wc + lr)] = make_double2(acc_re[r], acc_im[r]);
+        red[ks - 1][(wr + M::row(r, lk)) + TM * (wc + lr)] = make_double2(acc_re[r], acc_im[r]);
     }
     __syncthreads();
     if (ks > 0) return;
@@ -230,7 +254,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
     for (int q = 0; q < KS - 1; ++q) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const cplx p = red[q][(wr + 4 * r + lk) + TM * (wc + lr)];
+        const cplx p = red[q][(wr + M::row(r, lk)) + TM * (wc + lr)];
         acc_re[r] += p.x;
         acc_im[r] += p.y;
       }
@@ -241,7 +265,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   cplx csj = g.cs ? g.cs[gj] : make_double2(1.0, 0.0);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int gi = i0 + wr + 4 * r + lk;
+    const int gi = i0 + wr + M::row(r, lk);
     if (gi >= g.M) continue;
     cplx v = make_double2(acc_re[r], (A_CPLX || B_CPLX) ? acc_im[r] : 0.0);
     v = cmul(g.alpha, v);
@@ -259,13 +283,15 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 // every operand fragment read from LDS feeds two blocks and a K stage of 16 carries 4x the
 // MFMA work of the 32 x 32 tile per barrier.  The next stage's global loads are in flight
 // (in registers) while the current stage is multiplied out of LDS.
-template <bool A_CPLX, bool B_CPLX>
+template <bool A_CPLX, bool B_CPLX, bool F32>
 __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
+  typedef Mma<F32> M;
+  typedef typename M::E E;
   constexpr int TM = 64, TN = 64, BK = 16;
-  __shared__ double As_re[BK][TM + 1];
-  __shared__ double As_im[A_CPLX ? BK : 1][TM + 1];
-  __shared__ double Bs_re[BK][TN + 1];
-  __shared__ double Bs_im[B_CPLX ? BK : 1][TN + 1];
+  __shared__ E As_re[BK][TM + 1];
+  __shared__ E As_im[A_CPLX ? BK : 1][TM + 1];
+  __shared__ E Bs_re[BK][TN + 1];
+  __shared__ E Bs_im[B_CPLX ? BK : 1][TN + 1];
 
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
@@ -288,13 +314,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4d acc_re[2][2], acc_im[2][2];
+  typename M::Acc acc_re[2][2], acc_im[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      acc_re[a][b] = v4d{0, 0, 0, 0};
-      acc_im[a][b] = v4d{0, 0, 0, 0};
+      acc_re[a][b] = typename M::Acc{0, 0, 0, 0};
+      acc_im[a][b] = typename M::Acc{0, 0, 0, 0};
     }
 
   Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
@@ -307,7 +333,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
 #pragma unroll
     for (int kq = 0; kq < BK; kq += 4) {
       const int kk = kq + lk;
-      double a_re[2], a_im[2], b_re[2], b_im[2];
+      E a_re[2], a_im[2], b_re[2], b_im[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         a_re[a] = As_re[kk][wr + 16 * a + lr];
@@ -322,17 +348,17 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re[a], b_re[b], acc_re[a][b], 0, 0, 0);
-          if (B_CPLX) acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re[a], b_im[b], acc_im[a][b], 0, 0, 0);
+          acc_re[a][b] = M::mac(a_re[a], b_re[b], acc_re[a][b]);
+          if (B_CPLX) acc_im[a][b] = M::mac(a_re[a], b_im[b], acc_im[a][b]);
         }
       if (A_CPLX) {
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          const double na = -a_im[a];
+          const E na = -a_im[a];
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            if (B_CPLX) acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(na, b_im[b], acc_re[a][b], 0, 0, 0);
-            acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im[a], b_re[b], acc_im[a][b], 0, 0, 0);
+            if (B_CPLX) acc_re[a][b] = M::mac(na, b_im[b], acc_re[a][b]);
+            acc_im[a][b] = M::mac(a_im[a], b_re[b], acc_im[a][b]);
           }
         }
       }
@@ -340,7 +366,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
     __syncthreads();
   }
 
-  // D(i, j) of a 16 x 16 block in lane j + 16 (i % 4), register i / 4
+  // register r of a lane holds row M::row(r, lane / 16) of its 16 x 16 block
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     const int gj = j0 + wc + 16 * b + lr;
@@ -350,7 +376,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
     for (int a = 0; a < 2; ++a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + wr + 16 * a + 4 * r + lk;
+        const int gi = i0 + wr + 16 * a + M::row(r, lk);
         if (gi >= g.M) continue;
         cplx v = make_double2(acc_re[a][b][r], (A_CPLX || B_CPLX) ? acc_im[a][b][r] : 0.0);
         v = cmul(g.alpha, v);
@@ -370,23 +396,33 @@ inline unsigned gemm_zdim(const GemmArgs& g) {
   return (unsigned)((g.batch > 1 ? g.batch : 1) * (g.nchain > 1 ? g.nchain : 1));
 }
 
-template <int WM, int WN>
-void launch_shape(const GemmArgs& g, hipStream_t s) {
+template <int WM, int WN, bool F32>
+void launch_shape_p(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)cdivl(g.M, 16 * WM), (unsigned)cdivl(g.N, 16 * WN), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_kernel<true, true, WM, WN>), grid, dim3(NT), 0, s, g);
-  else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false, WM, WN>), grid, dim3(NT), 0, s, g);
-  else if (bc) hipLaunchKernelGGL((gemm_kernel<false, true, WM, WN>), grid, dim3(NT), 0, s, g);
-  else hipLaunchKernelGGL((gemm_kernel<false, false, WM, WN>), grid, dim3(NT), 0, s, g);
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_kernel<true, true, WM, WN, F32>), grid, dim3(NT), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_kernel<true, false, WM, WN, F32>), grid, dim3(NT), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_kernel<false, true, WM, WN, F32>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gemm_kernel<false, false, WM, WN, F32>), grid, dim3(NT), 0, s, g);
+}
+template <int WM, int WN>
+void launch_shape(const GemmArgs& g, hipStream_t s) {
+  if (g.prec) launch_shape_p<WM, WN, true>(g, s);
+  else launch_shape_p<WM, WN, false>(g, s);
 }
 
-void launch_big(const GemmArgs& g, hipStream_t s) {
+template <bool F32>
+void launch_big_p(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)cdivl(g.M, 64), (unsigned)cdivl(g.N, 64), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true>), grid, dim3(NT), 0, s, g);
-  else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false>), grid, dim3(NT), 0, s, g);
-  else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true>), grid, dim3(NT), 0, s, g);
-  else hipLaunchKernelGGL((gemm_big_kernel<false, false>), grid, dim3(NT), 0, s, g);
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true, F32>), grid, dim3(NT), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false, F32>), grid, dim3(NT), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true, F32>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gemm_big_kernel<false, false, F32>), grid, dim3(NT), 0, s, g);
+}
+void launch_big(const GemmArgs& g, hipStream_t s) {
+  if (g.prec) launch_big_p<true>(g, s);
+  else launch_big_p<false>(g, s);
 }
 
 
@@ -396,8 +432,10 @@ void launch_big(const GemmArgs& g, hipStream_t s) {
 // v_mfma_f64_16x16x4f64 operand layout (lane = row + 16 k for A, column + 16 k for B),
 // four or eight k-steps per batch with the next batch in flight.  The NW partial tiles are
 // summed through LDS in wave order.
-template <bool A_CPLX, bool B_CPLX, int NW>
+template <bool A_CPLX, bool B_CPLX, int NW, bool F32>
 __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
+  typedef Mma<F32> M;
+  typedef typename M::E E;
   __shared__ cplx red[NW > 1 ? NW - 1 : 1][256];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -480,7 +518,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     }
   };
 
-  v4d acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
+  typename M::Acc acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
   cplx av[U], bv[U], sv[U];
   load_batch(kb, av, bv, sv);
   finish(av, bv, sv);
@@ -489,22 +527,22 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     load_batch(k0 + 4 * U, an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc_re, 0, 0, 0);
-      if (B_CPLX) acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].y, acc_im, 0, 0, 0);
+      acc_re = M::mac((E)av[u].x, (E)bv[u].x, acc_re);
+      if (B_CPLX) acc_im = M::mac((E)av[u].x, (E)bv[u].y, acc_im);
       if (A_CPLX) {
-        if (B_CPLX) acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[u].y, bv[u].y, acc_re, 0, 0, 0);
-        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, acc_im, 0, 0, 0);
+        if (B_CPLX) acc_re = M::mac((E)-av[u].y, (E)bv[u].y, acc_re);
+        acc_im = M::mac((E)av[u].y, (E)bv[u].x, acc_im);
       }
     }
     finish(an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) { av[u] = an[u]; bv[u] = bn[u]; }
   }
-  // D(i, j) in lane j + 16 (i % 4), register i / 4
+  // register r of a lane holds row M::row(r, lane / 16) of the tile
   if (NW > 1) {
     if (wave > 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) red[wave - 1][(4 * r + lk) + 16 * lr] = make_double2(acc_re[r], acc_im[r]);
+      for (int r = 0; r < 4; ++r) red[wave - 1][M::row(r, lk) + 16 * lr] = make_double2(acc_re[r], acc_im[r]);
     }
     __syncthreads();
     if (wave > 0) return;
@@ -512,7 +550,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     for (int w = 0; w < NW - 1; ++w) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const cplx p = red[w][(4 * r + lk) + 16 * lr];
+        const cplx p = red[w][M::row(r, lk) + 16 * lr];
         acc_re[r] += p.x;
         acc_im[r] += p.y;
       }
@@ -522,7 +560,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   const cplx csj = g.cs ? g.cs[gj] : make_double2(1.0, 0.0);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int ri = i0 + 4 * r + lk;
+    const int ri = i0 + M::row(r, lk);
     if (ri >= g.M) continue;
     cplx v = make_double2(acc_re[r], (A_CPLX || B_CPLX) ? acc_im[r] : 0.0);
     v = cmul(g.alpha, v);
@@ -535,14 +573,19 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   }
 }
 
-template <int NW>
-void launch_direct(const GemmArgs& g, hipStream_t s) {
+template <int NW, bool F32>
+void launch_direct_p(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)cdivl(g.M, 16), (unsigned)cdivl(g.N, 16), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_direct_kernel<true, true, NW>), grid, dim3(64 * NW), 0, s, g);
-  else if (g.a_cplx) hipLaunchKernelGGL((gemm_direct_kernel<true, false, NW>), grid, dim3(64 * NW), 0, s, g);
-  else if (bc) hipLaunchKernelGGL((gemm_direct_kernel<false, true, NW>), grid, dim3(64 * NW), 0, s, g);
-  else hipLaunchKernelGGL((gemm_direct_kernel<false, false, NW>), grid, dim3(64 * NW), 0, s, g);
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_direct_kernel<true, true, NW, F32>), grid, dim3(64 * NW), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_direct_kernel<true, false, NW, F32>), grid, dim3(64 * NW), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_direct_kernel<false, true, NW, F32>), grid, dim3(64 * NW), 0, s, g);
+  else hipLaunchKernelGGL((gemm_direct_kernel<false, false, NW, F32>), grid, dim3(64 * NW), 0, s, g);
+}
+template <int NW>
+void launch_direct(const GemmArgs& g, hipStream_t s) {
+  if (g.prec) launch_direct_p<NW, true>(g, s);
+  else launch_direct_p<NW, false>(g, s);
 }
 }  // namespace
 
@@ -558,6 +601,7 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
   g.batch = 1;
   g.nchain = 1;
   g.cstride = 0;
+  g.prec = 0;
   return g;
 }
 

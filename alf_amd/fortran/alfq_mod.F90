@@ -30,7 +30,7 @@ module alfq_mod
     real(c_double) :: xmean_g_sum, xmax_g, xmax_p
   end type alfq_control
 
-  public :: alfq_ctx_create, alfq_ctx_destroy, alfq_set_delay, alfq_fields_set, alfq_fields_get, &
+  public :: alfq_ctx_create, alfq_ctx_destroy, alfq_set_delay, alfq_set_precision, alfq_fields_set, alfq_fields_get, &
             alfq_green_set, alfq_green_get, alfq_phase_set, alfq_phase_get, alfq_udv_reset, &
             alfq_udv_assign, alfq_rng_set_tape, alfq_rng_set_philox, alfq_wrapgrup, alfq_wrapgrdo, &
             alfq_wrapur, alfq_wrapul, alfq_cgr, alfq_init, alfq_sweep, alfq_sync, alfq_control_get, &
@@ -51,6 +51,11 @@ module alfq_mod
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: k
+    end function
+    integer(c_int) function alfq_set_precision(ctx, mode) bind(C, name="alfq_set_precision")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
     end function
     integer(c_int) function alfq_fields_set(ctx, f, n_op, ltrot) bind(C, name="alfq_fields_set")
       import :: c_int, c_ptr, c_int32_t

@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(_HERE, "libalfq.so")
 ALFQ_UDVR = -1
 ALFQ_UDVL = -2
 
+PREC_FP64, PREC_FP32_WRAPS = 0, 1     # alfq_set_precision modes
+
 ERROR_NAMES = {0: "ERROR_NONE", 1: "ERROR_GENERIC", 2: "ERROR_FILE_NOT_FOUND", 3: "ERROR_UNSTABLE_MATRIX",
                4: "ERROR_MISSING_OBS", 5: "ERROR_FIELDS", 6: "ERROR_HAMILTONIAN"}
 
@@ -32,6 +34,7 @@ EXPORTS = [
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
+    "alfq_set_precision",
 ]
 
 
@@ -74,6 +77,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_ctx_create": [C.POINTER(ModelDesc), C.POINTER(P)],
         "alfq_ctx_destroy": [P],
         "alfq_set_delay": [P, C.c_int32],
+        "alfq_set_precision": [P, C.c_int32],
         "alfq_fields_set": [P, ip, C.c_int32, C.c_int32],
         "alfq_fields_get": [P, ip, C.c_int32, C.c_int32],
         "alfq_green_set": [P, dp, C.c_int32],
@@ -107,7 +111,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_batch_size": [P, C.POINTER(C.c_int32)],
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
-                      C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32],
+                      C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -194,6 +198,10 @@ class Engine:
     # ---- state ----
     def set_delay(self, k):
         self._c(self.lib.alfq_set_delay(self.ctx, int(k)))
+
+    def set_precision(self, mode):
+        """PREC_FP64 (drop-in) or PREC_FP32_WRAPS (hopping wraps of G on FP32 MFMA)."""
+        self._c(self.lib.alfq_set_precision(self.ctx, int(mode)))
 
     def set_fields(self, f):
         a = np.asfortranarray(f, dtype=np.int32)
@@ -325,8 +333,8 @@ class Engine:
         return out[:3 * k].reshape(k, 3)
 
 
-def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None):
-    """C = alpha op(A) op(B) + beta C through the engine's MFMA kernel (alfq_gemm)."""
+def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None, prec=PREC_FP64):
+    """C = alpha op(A) op(B) + beta C through the engine's MFMA kernels (alfq_gemm)."""
     lib = load_library()
     a_real = not np.iscomplexobj(A)
     b_real = not np.iscomplexobj(B)
@@ -339,7 +347,8 @@ def gemm(A, B, opa="N", opb="N", alpha=1.0, beta=0.0, Cin=None):
     al = np.array([complex(alpha).real, complex(alpha).imag])
     be = np.array([complex(beta).real, complex(beta).imag])
     check(lib, lib.alfq_gemm(M, N, K, opa.encode(), int(a_real), _dptr(Aa), max(Aa.shape[0], 1), opb.encode(),
-                             int(b_real), _dptr(Ba), max(Ba.shape[0], 1), _dptr(al), _dptr(be), _dptr(Cc), max(M, 1)))
+                             int(b_real), _dptr(Ba), max(Ba.shape[0], 1), _dptr(al), _dptr(be), _dptr(Cc), max(M, 1),
+                             int(prec)))
     return Cc
 
 

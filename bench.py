@@ -53,23 +53,47 @@ def cpu_baseline(ham, fields, seconds_budget=30.0):
                       f"(ZGEQP3 pivoting), one thread, {dt:.1f} s"}
 
 
-def batched_rate(chains, steps):
+def batched_rate(chains, steps, prec=0):
     """Aggregate sweeps/s of `chains` independent chains of the workload in one batched context."""
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import BatchEngine
-    hams = [hubbard_ham_set(HubbardParams(**WORKLOAD)) for _ in range(chains)]
-    b = BatchEngine(hams, delay=32)
-    for z, h in enumerate(hams):
-        h.nsigma.set_uniform_four_state(z)
-        b.chain(z).set_fields(h.nsigma.f.copy())
+    ham = hubbard_ham_set(HubbardParams(**WORKLOAD))
+    b = BatchEngine([ham] * chains, delay=32)
+    b.set_precision(prec)
+    for z in range(chains):
+        ham.nsigma.set_uniform_four_state(z)
+        b.chain(z).set_fields(ham.nsigma.f.copy())
         b.chain(z).set_philox(seed=z, stream=0)
     b.init()
     b.sweep(3)            # the first sweep runs eagerly, the next two capture both graph parities
     ms = b.sweep_timed(steps)
-    xmax = max(b.chain(z).control()["XMAXG"] for z in range(chains))
+    ctl = [b.chain(z).control() for z in range(chains)]
     b.close()
     return {"chains": chains, "steps": steps, "ms_per_batched_sweep": ms / steps,
-            "aggregate_sweeps_per_s": chains * steps / (ms / 1e3), "precision_green_max": xmax}
+            "aggregate_sweeps_per_s": chains * steps / (ms / 1e3),
+            "precision_green_max": max(c["XMAXG"] for c in ctl),
+            "precision_green_mean": float(np.mean([c["precision_green_mean"] for c in ctl])),
+            "precision_phase_max": max(c["XMAXP"] for c in ctl)}
+
+
+def single_rate(ham, fields, delay, steps, warmup, prec):
+    """Sweeps/s of one chain of the workload and its Precision Green/Phase statistics."""
+    from alf_amd.runtime import Engine
+    eng = Engine(ham, delay=delay)
+    eng.set_precision(prec)
+    eng.set_fields(fields)
+    eng.set_philox(seed=0, stream=0)
+    eng.init()
+    for _ in range(warmup):
+        eng.sweep(1)
+    eng.sync()
+    eng.control_reset()
+    ms = eng.sweep_timed(steps)
+    ctl = eng.control()
+    eng.close()
+    return {"sweeps_per_s": steps / (ms / 1e3), "ms_per_sweep": ms / steps,
+            "precision_green_mean": ctl["precision_green_mean"], "precision_green_max": ctl["XMAXG"],
+            "precision_phase_max": ctl["XMAXP"]}
 
 
 def main():
@@ -78,7 +102,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--delay", type=int, default=32)
-    ap.add_argument("--batch", type=int, default=64, help="chains of the batched line (0: skip it)")
+    ap.add_argument("--batch", type=int, default=256, help="chains of the batched line (0: skip it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -139,6 +163,17 @@ def main():
     eng.close()
     if args.batch > 0:
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
+    # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64
+    from alf_amd.runtime import PREC_FP32_WRAPS
+    mixed = {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) with FP32 operands, "
+                     "FP32 MFMA and FP32 accumulation; UDV chain, QR, CGR, Metropolis ratio and delayed "
+                     "updates FP64",
+             "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_FP32_WRAPS)}
+    if args.batch > 0:
+        mixed["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_FP32_WRAPS)
+    mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
+                                     mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
+    result["mixed"] = mixed
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ham, fields)
     print(json.dumps(result), flush=True)

@@ -81,6 +81,16 @@ int alfq_set_chain(alfq_ctx* ctx, int32_t chain);
 int alfq_batch_size(alfq_ctx* ctx, int32_t* nchain);
 /* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */
 int alfq_set_delay(alfq_ctx* ctx, int32_t k);
+/* Arithmetic of the hopping wraps of G in WRAPGRUP/WRAPGRDO (Hop_mod_mmthr/mmthl and
+ * their inverses, Prog/Hop_mod.F90:143-250).  ALFQ_PREC_FP64 (the default) is the drop-in;
+ * ALFQ_PREC_FP32_WRAPS rounds both operands of those products to FP32 and multiplies and
+ * accumulates in FP32 on the matrix cores (no counterpart in the reference).  The UDV
+ * propagation, QR, CGR, the Metropolis ratio and the delayed updates stay FP64 in both
+ * modes, so every stabilisation point rebuilds G in FP64 and Precision Green measures the
+ * drift the FP32 wraps accumulated since the last one. */
+#define ALFQ_PREC_FP64 0
+#define ALFQ_PREC_FP32_WRAPS 1
+int alfq_set_precision(alfq_ctx* ctx, int32_t mode);
 
 /* ---- state movement ------------------------------------------------------ */
 /* nsigma%f (Fields_mod.F90:79-99): n_op x ltrot integers, column-major. */
@@ -146,12 +156,13 @@ int alfq_control_reset(alfq_ctx* ctx);
 /* Per-stabilisation record: (XMAX, XMEAN, |Z - PHASE|) of each checked CGR call. */
 int alfq_precision_log(alfq_ctx* ctx, double* out3, int64_t cap, int64_t* count);
 const char* alfq_last_error(void);
-/* The engine's FP64 MFMA product on host arrays (copied in and out), for
- * checking the kernel in isolation: C = alpha op(A) op(B) + beta C with op in
- * 'N', 'T', 'C'; A and/or B may be real (a_is_real/b_is_real), C is complex. */
+/* The engine's MFMA product on host arrays (copied in and out), for checking the
+ * kernels in isolation: C = alpha op(A) op(B) + beta C with op in 'N', 'T', 'C'; A
+ * and/or B may be real (a_is_real/b_is_real), C is complex; prec is an ALFQ_PREC_*
+ * arithmetic (ALFQ_PREC_FP32_WRAPS: FP32 operands, products and sums). */
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
-              const double* beta, double* C, int32_t ldc);
+              const double* beta, double* C, int32_t ldc, int32_t prec);
 /* Per-launch timing of the Metropolis chain kernel (start/stop events recorded
  * with each launch on the context's stream).  Enabling resets the record;
  * *total_ms is the summed kernel time of the recorded launches. */

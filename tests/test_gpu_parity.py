@@ -353,3 +353,45 @@ def test_batched_chains_at_full_tile_grids(rt):
         assert cb["XMAXG"] < 1e-8
         e.close()
     b.close()
+
+
+# ---------------------------------------------------------------- FP32 hopping wraps
+@pytest.mark.parametrize("shape", [(16, 16, 16), (37, 50, 29), (256, 256, 256), (256, 224, 32), (32, 224, 256),
+                                   (300, 17, 5), (2048, 1024, 48)])
+def test_gemm_fp32_arithmetic(rt, shape):
+    """FP32 MFMA products (every tiling: direct split-K, 32x32 / 16x32 / 16x16, 64x64) equal
+    the product of the operands rounded to float32, to float32 accumulation accuracy; the
+    result layout of v_mfma_f32_16x16x4f32 differs from the FP64 instruction's, so a wrong
+    mapping would show as O(1) errors."""
+    M, N, K = shape
+    r = np.random.default_rng(M * 5 + N)
+    A = r.standard_normal((M, K)) + 1j * r.standard_normal((M, K))
+    B = r.standard_normal((K, N)) + 1j * r.standard_normal((K, N))
+    Ar = r.standard_normal((M, K))
+    f32 = lambda X: X.astype(np.complex64).astype(np.complex128) if np.iscomplexobj(X) \
+        else X.astype(np.float32).astype(np.float64)
+    Br = r.standard_normal((K, N))
+    for a, b in ((A, B), (Ar, B), (A, Br)):
+        want = f32(a) @ f32(b)
+        got = rt.gemm(a, b, prec=rt.PREC_FP32_WRAPS)
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err < 2e-6 * np.sqrt(K / 16), err
+        assert np.linalg.norm(got - a @ b) / np.linalg.norm(a @ b) > 1e-10   # FP32 arithmetic did run
+
+
+def test_fp32_wraps_sweep_precision(rt):
+    """With the hopping wraps of G on FP32 MFMA, every stabilisation point still rebuilds G in
+    FP64 from the UDV chain (FP64 in both modes), so Precision Green measures the FP32 drift
+    over Nwrap slices: well above FP64 roundoff, and bounded."""
+    ham = hubbard(L1=6, L2=6, beta=4.0, nwrap=10)
+    f = fields_for(ham, 11)
+    e = rt.Engine(ham, delay=16)
+    e.set_fields(f)
+    e.set_philox(5, 0)
+    e.set_precision(rt.PREC_FP32_WRAPS)
+    e.init()
+    e.sweep(2)
+    c = e.control()
+    assert 1e-9 < c["XMAXG"] < 1e-3, c["XMAXG"]
+    assert c["XMAXP"] < 1e-3
+    e.close()
