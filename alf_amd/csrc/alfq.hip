@@ -360,9 +360,15 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
       if ((st = gemm_c(c, gx))) return st;
     }
     // G += X Y^T with the rank read from the chain state
-    GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
-    g.kdev = &c->st->m;
-    if ((st = gemm_c(c, g))) return st;
+    const long long ftiles = (long long)((c->n + 63) / 64) * ((c->n + 63) / 64) * c->B;
+    if (reg_kernel && ftiles >= 256) {
+      // a batch with enough 64 x 64 tiles to fill the device: one memory round trip per tile
+      if ((st = flush_launch(c->G, c->X, c->Y, &c->st->m, c->n, c->K, c->B, c->cs, c->s))) return st;
+    } else {
+      GemmArgs g = gemm_args(c->n, c->n, c->K, c->X, c->n, OP_N, true, c->Y, c->n, OP_T, true, c->G, c->n, ONE, ONE);
+      g.kdev = &c->st->m;
+      if ((st = gemm_c(c, g))) return st;
+    }
   }
   c->rng_next += c->per_visit_sum;
   return ST_NONE;

@@ -117,3 +117,7 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
                    const void* B, int ldb, int opB, bool b_cplx, void* C, int ldc,
                    cplx alpha, cplx beta);
 int gemm_launch(const GemmArgs& g, hipStream_t s);
+// G += X Y^T (all n x n / n x m, column-major, ld n) with the rank m read from *mdev and
+// clamped to kmax <= 32, for the nchain chains of a batch (pointers advance by cs bytes).
+int flush_launch(cplx* G, const cplx* X, const cplx* Y, const int* mdev, int n, int kmax, int nchain,
+                 long long cs, hipStream_t s);

@@ -61,13 +61,14 @@ __device__ __forceinline__ void chain_operands(GemmArgs& g) {
 // its LDS stores.  Gather indices and row scales are read first and every data load is then
 // issued back to back: no wait on one load sits between the issue of two others (a wait at
 // the join of an optional map lookup would otherwise serialise the whole stage).
-// Conjugation and scaling are applied when the stage is stored.
+// Conjugation is applied when the stage is stored; the row scales of opB (one per k) go to
+// an LDS vector of the stage and are applied to the B fragments at multiplication time.
 template <int TM, int TN, int BK, bool A_CPLX, bool B_CPLX>
 struct Stage {
   static constexpr int LA = TM * BK / NT, LB = BK * TN / NT;   // elements per thread
-  static_assert(LA >= 1 && LB >= 1, "a stage covers the workgroup");
+  static_assert(LA >= 1 && LB >= 1 && BK <= NT, "a stage covers the workgroup");
   double a_re[LA], a_im[LA], b_re[LB], b_im[LB];
-  cplx sc[LB];
+  cplx scl;                 // bs(k0 + tid) for tid < BK
 
   __device__ __forceinline__ static void a_index(const GemmArgs& g, int tid, int p, int& i, int& kk) {
     const int idx = tid + NT * p;
@@ -93,10 +94,7 @@ struct Stage {
       b_index(g, tid, p, kk, j);
       kb[p] = (k0 + kk < K && j0 + j < g.N) ? k0 + kk : -1;
     }
-    if (g.bs) {
-#pragma unroll
-      for (int p = 0; p < LB; ++p) sc[p] = kb[p] >= 0 ? g.bs[kb[p]] : make_double2(0.0, 0.0);
-    }
+    if (g.bs) scl = (tid < BK && k0 + tid < K) ? g.bs[k0 + tid] : make_double2(0.0, 0.0);
     if (g.amap) {
 #pragma unroll
       for (int p = 0; p < LA; ++p) if (ka[p] >= 0) ka[p] = g.amap[ka[p]];
@@ -147,7 +145,7 @@ struct Stage {
 
   template <typename E>
   __device__ __forceinline__ void store(const GemmArgs& g, int tid, E (*As_re)[TM + 1], E (*As_im)[TM + 1],
-                                        E (*Bs_re)[TN + 1], E (*Bs_im)[TN + 1]) const {
+                                        E (*Bs_re)[TN + 1], E (*Bs_im)[TN + 1], cplx* bsl) const {
 #pragma unroll
     for (int p = 0; p < LA; ++p) {
       int i, kk;
@@ -159,17 +157,20 @@ struct Stage {
     for (int p = 0; p < LB; ++p) {
       int kk, j;
       b_index(g, tid, p, kk, j);
-      double re = b_re[p], im = (g.opB == OP_C) ? -b_im[p] : b_im[p];
-      if (g.bs) {
-        const double r2 = re * sc[p].x - im * sc[p].y;
-        im = re * sc[p].y + im * sc[p].x;
-        re = r2;
-      }
-      Bs_re[kk][j] = (E)re;
-      if (B_CPLX) Bs_im[kk][j] = (E)im;
+      Bs_re[kk][j] = (E)b_re[p];
+      if (B_CPLX) Bs_im[kk][j] = (E)((g.opB == OP_C) ? -b_im[p] : b_im[p]);
     }
+    if (g.bs && tid < BK) bsl[tid] = scl;
   }
 };
+
+// B fragment times the row scale of its k (the scales of a stage sit in LDS)
+template <typename E>
+__device__ __forceinline__ void scale_b(const cplx s, E& re, E& im) {
+  const E r2 = re * (E)s.x - im * (E)s.y;
+  im = re * (E)s.y + im * (E)s.x;
+  re = r2;
+}
 
 template <bool A_CPLX, bool B_CPLX, int WM, int WN, bool F32>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   __shared__ E Bs_re[BK][TN + 1];
   __shared__ E Bs_im[B_CPLX ? BK : 1][TN + 1];
   __shared__ cplx red[KS > 1 ? KS - 1 : 1][KS > 1 ? TM * TN : 1];
+  __shared__ cplx bsl[BK];
 
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
@@ -214,17 +216,18 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 
   if (K > 0) st.load(g, A, B, i0, j0, K, 0, tid);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    st.store(g, tid, As_re, As_im, Bs_re, Bs_im);
+    st.store(g, tid, As_re, As_im, Bs_re, Bs_im, bsl);
     __syncthreads();
     if (k0 + BK < K) st.load(g, A, B, i0, j0, K, k0 + BK, tid);   // in flight during this stage's MFMAs
 #pragma unroll
     for (int kq = 0; kq < 16; kq += 4) {
       const int kk = 16 * ks + kq;
       const E a_re = As_re[kk + lk][wr + lr];
-      const E b_re = Bs_re[kk + lk][wc + lr];
+      E b_re = Bs_re[kk + lk][wc + lr];
+      E b_im = B_CPLX ? Bs_im[kk + lk][wc + lr] : (E)0;
+      if (B_CPLX && g.bs) scale_b(bsl[kk + lk], b_re, b_im);
       acc_re = M::mac(a_re, b_re, acc_re);
       if (B_CPLX) {
-        const E b_im = Bs_im[kk + lk][wc + lr];
         acc_im = M::mac(a_re, b_im, acc_im);
         if (A_CPLX) {
           const E a_im = As_im[kk + lk][wr + lr];
@@ -283,15 +286,16 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 // every operand fragment read from LDS feeds two blocks and a K stage of 16 carries 4x the
 // MFMA work of the 32 x 32 tile per barrier.  The next stage's global loads are in flight
 // (in registers) while the current stage is multiplied out of LDS.
-template <bool A_CPLX, bool B_CPLX, bool F32>
+template <bool A_CPLX, bool B_CPLX, bool F32, int BK>
 __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   typedef Mma<F32> M;
   typedef typename M::E E;
-  constexpr int TM = 64, TN = 64, BK = 16;
+  constexpr int TM = 64, TN = 64;
   __shared__ E As_re[BK][TM + 1];
   __shared__ E As_im[A_CPLX ? BK : 1][TM + 1];
   __shared__ E Bs_re[BK][TN + 1];
   __shared__ E Bs_im[B_CPLX ? BK : 1][TN + 1];
+  __shared__ cplx bsl[BK];
 
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
 
   if (K > 0) st.load(g, A, B, i0, j0, K, 0, tid);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    st.store(g, tid, As_re, As_im, Bs_re, Bs_im);
+    st.store(g, tid, As_re, As_im, Bs_re, Bs_im, bsl);
     __syncthreads();
     if (k0 + BK < K) st.load(g, A, B, i0, j0, K, k0 + BK, tid);
 #pragma unroll
@@ -343,6 +347,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
       for (int b = 0; b < 2; ++b) {
         b_re[b] = Bs_re[kk][wc + 16 * b + lr];
         if (B_CPLX) b_im[b] = Bs_im[kk][wc + 16 * b + lr];
+      }
+      if (B_CPLX && g.bs) {
+        const cplx sk = bsl[kk];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) scale_b(sk, b_re[b], b_im[b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -391,6 +400,104 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   }
 }
 
+// Delayed-update flush G += X Y^T of rank m <= 32 (the block kernel's round).  A workgroup
+// owns a 64 x 64 tile of one chain's G; its operands (the m columns of X for the tile's rows
+// and of Y for its columns) and the tile of G itself, read directly in the MFMA result
+// layout, are all requested at the start, so a tile costs one memory round trip; the
+// product takes ceil(m / 4) k-steps of 2 x 2 MFMA blocks per wave and G is written once.
+constexpr int FL_T = 64, FL_K = 32;
+__global__ __launch_bounds__(NT, 2) void flush_kernel(cplx* G, const cplx* X, const cplx* Y, const int* mdev,
+                                                      int n, int kmax, long long cs) {
+  typedef Mma<false> M;
+  __shared__ double Xr[FL_K][FL_T + 1], Xi[FL_K][FL_T + 1], Yr[FL_K][FL_T + 1], Yi[FL_K][FL_T + 1];
+  G = chain_ptr(G, cs);
+  X = chain_ptr(X, cs);
+  Y = chain_ptr(Y, cs);
+  mdev = chain_ptr(mdev, cs);
+  int m = *mdev;
+  m = m < kmax ? m : kmax;
+  if (m <= 0) return;
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * FL_T, j0 = blockIdx.y * FL_T;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+  const int li = tid & 63, lkb = tid >> 6;   // operand rows: index li, columns k = lkb + 4 p
+
+  cplx xv[FL_K / 4], yv[FL_K / 4];
+#pragma unroll
+  for (int p = 0; p < FL_K / 4; ++p) {
+    const int k = lkb + 4 * p;
+    xv[p] = make_double2(0.0, 0.0);
+    yv[p] = make_double2(0.0, 0.0);
+    if (k < m && i0 + li < n) xv[p] = X[i0 + li + (long long)k * n];
+    if (k < m && j0 + li < n) yv[p] = Y[j0 + li + (long long)k * n];
+  }
+  cplx gv[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * a + M::row(r, lk), gj = j0 + wc + 16 * b + lr;
+        gv[a][b][r] = make_double2(0.0, 0.0);
+        if (gi < n && gj < n) gv[a][b][r] = G[gi + (long long)gj * n];
+      }
+#pragma unroll
+  for (int p = 0; p < FL_K / 4; ++p) {
+    const int k = lkb + 4 * p;
+    Xr[k][li] = xv[p].x;
+    Xi[k][li] = xv[p].y;
+    Yr[k][li] = yv[p].x;
+    Yi[k][li] = yv[p].y;
+  }
+  __syncthreads();
+
+  v4d acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc_re[a][b] = v4d{0, 0, 0, 0};
+      acc_im[a][b] = v4d{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < m; k0 += 4) {
+    const int kk = k0 + lk;
+    double a_re[2], a_im[2], b_re[2], b_im[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      a_re[a] = Xr[kk][wr + 16 * a + lr];
+      a_im[a] = Xi[kk][wr + 16 * a + lr];
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      b_re[b] = Yr[kk][wc + 16 * b + lr];
+      b_im[b] = Yi[kk][wc + 16 * b + lr];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const double na = -a_im[a];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = M::mac(a_re[a], b_re[b], acc_re[a][b]);
+        acc_im[a][b] = M::mac(a_re[a], b_im[b], acc_im[a][b]);
+        acc_re[a][b] = M::mac(na, b_im[b], acc_re[a][b]);
+        acc_im[a][b] = M::mac(a_im[a], b_re[b], acc_im[a][b]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * a + M::row(r, lk), gj = j0 + wc + 16 * b + lr;
+        if (gi < n && gj < n)
+          G[gi + (long long)gj * n] = make_double2(gv[a][b][r].x + acc_re[a][b][r], gv[a][b][r].y + acc_im[a][b][r]);
+      }
+}
+
 inline long long cdivl(long long a, long long b) { return (a + b - 1) / b; }
 inline unsigned gemm_zdim(const GemmArgs& g) {
   return (unsigned)((g.batch > 1 ? g.batch : 1) * (g.nchain > 1 ? g.nchain : 1));
@@ -411,18 +518,22 @@ void launch_shape(const GemmArgs& g, hipStream_t s) {
   else launch_shape_p<WM, WN, false>(g, s);
 }
 
-template <bool F32>
+template <bool F32, int BK>
 void launch_big_p(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)cdivl(g.M, 64), (unsigned)cdivl(g.N, 64), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true, F32>), grid, dim3(NT), 0, s, g);
-  else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false, F32>), grid, dim3(NT), 0, s, g);
-  else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true, F32>), grid, dim3(NT), 0, s, g);
-  else hipLaunchKernelGGL((gemm_big_kernel<false, false, F32>), grid, dim3(NT), 0, s, g);
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true, F32, BK>), grid, dim3(NT), 0, s, g);
+  else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false, F32, BK>), grid, dim3(NT), 0, s, g);
+  else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true, F32, BK>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gemm_big_kernel<false, false, F32, BK>), grid, dim3(NT), 0, s, g);
 }
+// K stages of 32 for long products with a real operand (the hopping and UDV wraps): twice
+// the MFMA work per barrier and per load latency; complex x complex stays at 16, where 32
+// would not fit two waves per SIMD without spilling
 void launch_big(const GemmArgs& g, hipStream_t s) {
-  if (g.prec) launch_big_p<true>(g, s);
-  else launch_big_p<false>(g, s);
+  const bool deep = g.K >= 64 && !(g.a_cplx && (g.b_cplx || g.bs != nullptr));
+  if (g.prec) deep ? launch_big_p<true, 32>(g, s) : launch_big_p<true, 16>(g, s);
+  else deep ? launch_big_p<false, 32>(g, s) : launch_big_p<false, 16>(g, s);
 }
 
 
@@ -588,6 +699,16 @@ void launch_direct(const GemmArgs& g, hipStream_t s) {
   else launch_direct_p<NW, false>(g, s);
 }
 }  // namespace
+
+int flush_launch(cplx* G, const cplx* X, const cplx* Y, const int* mdev, int n, int kmax, int nchain,
+                 long long cs, hipStream_t s) {
+  if (kmax < 1 || kmax > FL_K) { alfq_set_error("flush: rank outside [1, 32]"); return ST_GENERIC; }
+  if (nchain < 1 || nchain > 65535) { alfq_set_error("flush: bad batch"); return ST_GENERIC; }
+  const unsigned t = (unsigned)cdivl(n, FL_T);
+  hipLaunchKernelGGL(flush_kernel, dim3(t, t, (unsigned)nchain), dim3(NT), 0, s, G, X, Y, mdev, n, kmax, cs);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,
                    const void* B, int ldb, int opB, bool b_cplx, void* C, int ldc,

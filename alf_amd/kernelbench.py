@@ -9,10 +9,35 @@ writes G once: 32 n^2 bytes per accepted update.  achieved = accepted updates pe
 launch x 32 n^2 B / average launch duration; bound = HBM.  The kernel itself keeps
 a 32 x 32 block of G in LDS and leaves the full-size products to the flush GEMMs,
 so the figure is the rate at which it retires the reference's rank-1 updates.
+
+``traffic`` (HBM bytes per launch) comes from hardware counters, which cannot be
+read in the timed run: two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of the
+benchmark, each scaled by what the same counter reads for a device-to-device copy
+of known size (``python3 -m alf_amd.kernelbench copy``).  The summary of those runs
+is committed beside the other profiles and read here.
+
+    python3 -m alf_amd.kernelbench copy [--mib N]   # the calibration copy
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
+import re
+import sys
+
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                               "chain_block_kernel_16x16_U4_beta10_hbm_traffic.md")
+
+
+def measured_traffic(path: str = TRAFFIC_SUMMARY):
+    """HBM bytes per launch of chain_block_kernel from the committed counter summary, or None."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def measure_dominant(eng, sweeps: int = 1) -> dict:
@@ -34,7 +59,7 @@ def measure_dominant(eng, sweeps: int = 1) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(),
         "kernel": "chain_block_kernel (Metropolis decisions + delayed rank-1 updates)",
         "avg_launch_us": avg_s * 1e6,
         "launches": launches,
@@ -42,3 +67,35 @@ def measure_dominant(eng, sweeps: int = 1) -> dict:
         "accepted": acc,
         "algorithmic_bytes_per_launch": bytes_per_launch,
     }
+
+
+def copy_calibration(nbytes: int, repeats: int = 3) -> None:
+    """Device-to-device copies of nbytes (the HIP runtime's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: each dispatch reads and writes exactly nbytes."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, nbytes) != 0 or hip.hipMemset(dst, 0, nbytes) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(repeats):
+        if hip.hipMemcpy(dst, src, nbytes, 3) != 0:     # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"copied {nbytes} bytes device to device x {repeats}", flush=True)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) >= 2 and sys.argv[1] == "copy":
+        mib = int(sys.argv[sys.argv.index("--mib") + 1]) if "--mib" in sys.argv else 256
+        copy_calibration(mib << 20)
+    else:
+        print(__doc__)

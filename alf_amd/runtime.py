@@ -34,7 +34,7 @@ EXPORTS = [
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
-    "alfq_set_precision",
+    "alfq_set_precision", "alfq_mfma_rate",
 ]
 
 
@@ -78,6 +78,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_ctx_destroy": [P],
         "alfq_set_delay": [P, C.c_int32],
         "alfq_set_precision": [P, C.c_int32],
+        "alfq_mfma_rate": [C.c_int32, dp],
         "alfq_fields_set": [P, ip, C.c_int32, C.c_int32],
         "alfq_fields_get": [P, ip, C.c_int32, C.c_int32],
         "alfq_green_set": [P, dp, C.c_int32],
@@ -390,3 +391,11 @@ def sweep_many(engines, nsweep=1):
     ms = C.c_double()
     check(lib, lib.alfq_sweep_many(arr, len(engines), nsweep, C.byref(ms)))
     return ms.value
+
+
+def mfma_rate(kind):
+    """Measured TFLOP/s ceiling: 0 FP64 MFMA, 1 FP32 MFMA, 2 vector FP64 FMA (alfq_mfma_rate)."""
+    lib = load_library()
+    v = C.c_double()
+    check(lib, lib.alfq_mfma_rate(int(kind), C.byref(v)))
+    return v.value

@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--delay", type=int, default=32)
     ap.add_argument("--batch", type=int, default=256, help="chains of the batched line (0: skip it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
@@ -165,15 +166,20 @@ def main():
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
     # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64
     from alf_amd.runtime import PREC_FP32_WRAPS
-    mixed = {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) with FP32 operands, "
+    mixed = None if args.no_mixed else {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) with FP32 operands, "
                      "FP32 MFMA and FP32 accumulation; UDV chain, QR, CGR, Metropolis ratio and delayed "
                      "updates FP64",
              "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_FP32_WRAPS)}
-    if args.batch > 0:
-        mixed["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_FP32_WRAPS)
-    mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
-                                     mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
-    result["mixed"] = mixed
+    if mixed is not None:
+        if args.batch > 0:
+            mixed["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_FP32_WRAPS)
+        mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
+                                         mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
+        result["mixed"] = mixed
+    # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
+    from alf_amd.runtime import mfma_rate
+    result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
+                                        "fp64_vector_fma": mfma_rate(2)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ham, fields)
     print(json.dumps(result), flush=True)

@@ -173,6 +173,10 @@ int alfq_kernel_timing_get(alfq_ctx* ctx, double* total_ms, int64_t* launches);
  * sweep into a HIP graph the first time its starting state is seen and replay it with
  * one launch afterwards; enable = 0 launches every kernel from the host. */
 int alfq_sweep_graphs(alfq_ctx* ctx, int32_t enable);
+/* Measured issue-rate ceiling (no counterpart in the reference; measurement only) of
+ * kind 0: FP64 MFMA v_mfma_f64_16x16x4f64, 1: FP32 MFMA v_mfma_f32_16x16x4f32,
+ * 2: vector FP64 FMA, in TFLOP/s over all CUs of the current device. */
+int alfq_mfma_rate(int32_t kind, double* tflops);
 /* Library build identification. */
 const char* alfq_version(void);
 

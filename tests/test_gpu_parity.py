@@ -395,3 +395,11 @@ def test_fp32_wraps_sweep_precision(rt):
     assert 1e-9 < c["XMAXG"] < 1e-3, c["XMAXG"]
     assert c["XMAXP"] < 1e-3
     e.close()
+
+
+def test_arithmetic_rate_probe(rt):
+    """The measured issue-rate ceilings are positive and below the published dense peaks."""
+    f64, f32, v64 = rt.mfma_rate(0), rt.mfma_rate(1), rt.mfma_rate(2)
+    assert 5.0 < f64 < 78.6 * 1.05
+    assert 10.0 < f32 < 157.3 * 1.05
+    assert 5.0 < v64 < 78.6 * 1.05
