@@ -26,6 +26,11 @@
 #include "alfq_qr.h"
 
 static thread_local std::string g_last_error;
+std::recursive_mutex& null_stream_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 void alfq_set_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {
@@ -663,6 +668,7 @@ int sweep_graph(alfq_ctx* c) {
   }
   if (c->graphs.size() >= 8) drop_graphs(c);
   const unsigned long long rn0 = c->rng_next;
+  std::unique_lock<std::recursive_mutex> capture_lock(null_stream_mutex());
   HIP_OK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
   c->capturing = true;
   const int st = sweep_dev(c);
@@ -672,6 +678,7 @@ int sweep_graph(alfq_ctx* c) {
   hipGraphExec_t exec = nullptr;
   bool ok = st == ST_NONE && ee == hipSuccess && graph != nullptr;
   if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  capture_lock.unlock();
   if (!ok) {
     if (graph) hipGraphDestroy(graph);
     (void)hipGetLastError();
@@ -878,7 +885,22 @@ hipError_t put_chain(alfq_ctx* c, int z, T* dst, const T* src, size_t count) {
   return hipMemcpy((char*)dst + (long long)z * c->cs, src, count * sizeof(T), hipMemcpyHostToDevice);
 }
 
+// Bytes of one chain's arena for the model d (what a batch allocates per chain).
+size_t chain_arena_bytes(const alfq_model_desc* d) {
+  alfq_ctx t;
+  t.n = d->ndim; t.L = d->ltrot; t.W = d->nwrap; t.n_op = d->n_op; t.n_sun = d->n_sun;
+  t.t_real = d->t_is_real != 0;
+  t.nstm = (t.L % t.W == 0) ? t.L / t.W : t.L / t.W + 1;
+  long long per = 0;
+  for (int i = 0; i < d->n_op; ++i) per += (d->op_type[i] == 2) ? 3 : 2;
+  t.u_cap = 2LL * t.L * per;
+  ArenaLayout sizing;
+  place_chain(&t, sizing, t.nstm + 6);
+  return (sizing.off + 4095) & ~(size_t)4095;
+}
+
 int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
+  NULL_STREAM_LOCK;
   if (!descs || !out || nchain < 1) { alfq_set_error("null argument or empty batch"); return ST_GENERIC; }
   *out = nullptr;
   int st;
@@ -996,6 +1018,14 @@ int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx
   return create_impl(descs, nchain, out);
 }
 
+int alfq_chain_bytes(const alfq_model_desc* d, int64_t* bytes) {
+  if (!d || !bytes) return ST_GENERIC;
+  int st;
+  if ((st = check_desc(d, d))) return st;
+  *bytes = (int64_t)chain_arena_bytes(d);
+  return ST_NONE;
+}
+
 int alfq_set_chain(alfq_ctx* c, int32_t chain) {
   if (!c) return ST_GENERIC;
   if (chain < 0 || chain >= c->B) { alfq_set_error("chain index outside the batch"); return ST_GENERIC; }
@@ -1010,6 +1040,7 @@ int alfq_batch_size(alfq_ctx* c, int32_t* nchain) {
 }
 
 int alfq_ctx_destroy(alfq_ctx* c) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_NONE;
   if (c->s) hipStreamSynchronize(c->s);
   drop_graphs(c);
@@ -1042,6 +1073,7 @@ int alfq_set_precision(alfq_ctx* c, int32_t mode) {
 }
 
 int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) {
+  NULL_STREAM_LOCK;
   if (!c || !f) return ST_GENERIC;
   if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
   std::vector<int8_t> h((size_t)n_op * ltrot);
@@ -1060,6 +1092,7 @@ int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) 
 }
 
 int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
+  NULL_STREAM_LOCK;
   if (!c || !f) return ST_GENERIC;
   if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
   std::vector<int8_t> h((size_t)n_op * ltrot);
@@ -1070,6 +1103,7 @@ int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
 }
 
 int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
+  NULL_STREAM_LOCK;
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   HIP_OK(hipMemcpy2D(at(c, c->G), c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
@@ -1078,6 +1112,7 @@ int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
 }
 
 int alfq_green_get(alfq_ctx* c, double* G, int32_t ld) {
+  NULL_STREAM_LOCK;
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), at(c, c->G), c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
@@ -1086,6 +1121,7 @@ int alfq_green_get(alfq_ctx* c, double* G, int32_t ld) {
 }
 
 int alfq_phase_set(alfq_ctx* c, const double* phase) {
+  NULL_STREAM_LOCK;
   if (!c || !phase) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   const cplx p = make_double2(phase[0], phase[1]);
@@ -1094,6 +1130,7 @@ int alfq_phase_set(alfq_ctx* c, const double* phase) {
 }
 
 int alfq_phase_get(alfq_ctx* c, double* phase) {
+  NULL_STREAM_LOCK;
   if (!c || !phase) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   cplx p;
@@ -1104,6 +1141,7 @@ int alfq_phase_get(alfq_ctx* c, double* phase) {
 }
 
 int alfq_udv_set(alfq_ctx* c, int32_t var, const double* U, const double* D, const double* V, char side) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_GENERIC;
   if (c->B > 1) { alfq_set_error("udv_set: single chains only (a slot binding is shared by the batch)"); return ST_GENERIC; }
   int* v = var_ref(c, var);
@@ -1148,6 +1186,7 @@ int alfq_udv_set(alfq_ctx* c, int32_t var, const double* U, const double* D, con
 }
 
 int alfq_udv_get(alfq_ctx* c, int32_t var, double* U, double* D, double* V) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_GENERIC;
   int* v = var_ref(c, var);
   if (!v) { alfq_set_error("udv: no such variable"); return ST_GENERIC; }
@@ -1177,6 +1216,7 @@ int alfq_udv_assign(alfq_ctx* c, int32_t dst, int32_t src) {
 }
 
 int alfq_rng_set_tape(alfq_ctx* c, const double* u, int64_t n) {
+  NULL_STREAM_LOCK;
   if (!c || (!u && n > 0) || n < 0) return ST_GENERIC;
   if (c->B > 1) { alfq_set_error("random tapes are for single chains; use alfq_rng_set_philox per chain"); return ST_GENERIC; }
   HIP_OK(hipStreamSynchronize(c->s));
@@ -1200,6 +1240,7 @@ int alfq_rng_set_tape(alfq_ctx* c, const double* u, int64_t n) {
 }
 
 int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   c->rng_mode = 2;
@@ -1223,6 +1264,7 @@ int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
 }
 
 int alfq_rng_position(alfq_ctx* c, int64_t* pos) {
+  NULL_STREAM_LOCK;
   if (!c || !pos) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   unsigned long long p;
@@ -1277,6 +1319,7 @@ int alfq_wrapul(alfq_ctx* c, int32_t ntau1, int32_t ntau) {
 }
 
 int alfq_cgr(alfq_ctx* c, int32_t nvar, int32_t var_r, int32_t var_l, int32_t check, double* z) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_GENERIC;
   if (nvar != 1 && nvar != 2) { alfq_set_error("CGR: NVAR must be 1 or 2"); return ST_GENERIC; }
   int* r = var_ref(c, var_r);
@@ -1363,6 +1406,7 @@ int alfq_sync(alfq_ctx* c) {
 }
 
 int alfq_control_get(alfq_ctx* c, alfq_control* out) {
+  NULL_STREAM_LOCK;
   if (!c || !out) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   ChainState cs;
@@ -1379,6 +1423,7 @@ int alfq_control_get(alfq_ctx* c, alfq_control* out) {
 }
 
 int alfq_control_reset(alfq_ctx* c) {
+  NULL_STREAM_LOCK;
   if (!c) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   ChainState cs;
@@ -1396,6 +1441,7 @@ int alfq_control_reset(alfq_ctx* c) {
 }
 
 int alfq_precision_log(alfq_ctx* c, double* out3, int64_t cap, int64_t* count) {
+  NULL_STREAM_LOCK;
   if (!c || !count) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
   DevScalars ds;
@@ -1440,6 +1486,7 @@ int alfq_kernel_timing_get(alfq_ctx* c, double* total_ms, int64_t* launches) {
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha, const double* beta,
               double* Cm, int32_t ldc, int32_t prec) {
+  NULL_STREAM_LOCK;
   if (M < 0 || N < 0 || K < 0 || !A || !B || !Cm || !alpha || !beta) { alfq_set_error("gemm: bad arguments"); return ST_GENERIC; }
   auto op = [](char o) { return (o == 'T' || o == 't') ? OP_T : ((o == 'C' || o == 'c') ? OP_C : OP_N); };
   const int oa = op(opa), ob = op(opb);

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 typedef double2 cplx;   // interleaved (re, im), the layout of Fortran complex(8)
@@ -75,6 +76,13 @@ enum alfq_status_internal {
 };
 
 void alfq_set_error(const std::string& msg);
+
+// Synchronous copies, sets, allocations and frees order themselves against the other
+// streams' work, which invalidates a HIP-graph capture running in another host thread.
+// Entry points that make such calls and the sweep captures hold this process-wide lock
+// (recursive: a failing create releases its context through alfq_ctx_destroy).
+std::recursive_mutex& null_stream_mutex();
+#define NULL_STREAM_LOCK std::lock_guard<std::recursive_mutex> null_stream_lock_(null_stream_mutex())
 
 #define HIP_OK(expr)                                                              \
   do {                                                                            \

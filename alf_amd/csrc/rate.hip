@@ -58,6 +58,7 @@ __global__ __launch_bounds__(256) void rate_vfma64(double* out, int iters) {
 
 extern "C" int alfq_mfma_rate(int32_t kind, double* tflops) {
   if (!tflops || kind < 0 || kind > 2) { alfq_set_error("mfma_rate: bad arguments"); return ST_GENERIC; }
+  NULL_STREAM_LOCK;
   int dev = 0, cus = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

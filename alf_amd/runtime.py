@@ -34,7 +34,7 @@ EXPORTS = [
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
-    "alfq_set_precision", "alfq_mfma_rate",
+    "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
 ]
 
 
@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_set_delay": [P, C.c_int32],
         "alfq_set_precision": [P, C.c_int32],
         "alfq_mfma_rate": [C.c_int32, dp],
+        "alfq_chain_bytes": [C.POINTER(ModelDesc), C.POINTER(C.c_int64)],
         "alfq_fields_set": [P, ip, C.c_int32, C.c_int32],
         "alfq_fields_get": [P, ip, C.c_int32, C.c_int32],
         "alfq_green_set": [P, dp, C.c_int32],
@@ -399,3 +400,13 @@ def mfma_rate(kind):
     v = C.c_double()
     check(lib, lib.alfq_mfma_rate(int(kind), C.byref(v)))
     return v.value
+
+
+def chain_bytes(model):
+    """Device bytes of one chain of `model` in a batch (alfq_chain_bytes)."""
+    lib = load_library()
+    d, keep = model_desc(model)
+    v = C.c_int64()
+    check(lib, lib.alfq_chain_bytes(C.byref(d), C.byref(v)))
+    del keep
+    return int(v.value)

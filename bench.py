@@ -76,6 +76,23 @@ def batched_rate(chains, steps, prec=0):
             "precision_phase_max": max(c["XMAXP"] for c in ctl)}
 
 
+def serving_rate(nreq, nsweeps, rate, seed=0):
+    """p50/p99 time-to-result and aggregate sweeps/s of a Poisson stream of Hubbard requests
+    served by the continuous-batching loop (alf_amd.serving)."""
+    from alf_amd.serving import Server, latency_summary, poisson_requests
+    reqs = poisson_requests(nreq, rate, seed=seed, nsweeps=nsweeps)
+    srv = Server()
+    res = srv.run(reqs)
+    out = latency_summary(res, srv.wall)
+    out.update({"sample": f"{nreq} requests x {nsweeps} sweeps (the serving configuration asks 500 sweeps each), "
+                          f"Poisson arrivals at {rate}/s, square L in {{8,12,16,24,32}} weighted 4:3:2:1:1 "
+                          "(Ndim 64..1024), beta in {4,8,16} (Ltrot 40..160), U in {2,4,6}, Nwrap 10, FP64",
+                "peak_concurrent_batches": srv.peak_batches,
+                "largest_batch": max(r.batch_size for r in res),
+                "precision_green_max": max(r.control["XMAXG"] for r in res)})
+    return out
+
+
 def single_rate(ham, fields, delay, steps, warmup, prec):
     """Sweeps/s of one chain of the workload and its Precision Green/Phase statistics."""
     from alf_amd.runtime import Engine
@@ -105,6 +122,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="chains of the batched line (0: skip it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
+    ap.add_argument("--serve-sweeps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
@@ -176,6 +195,8 @@ def main():
         mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
                                          mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
         result["mixed"] = mixed
+    if args.serve_requests > 0:
+        result["serving"] = serving_rate(args.serve_requests, args.serve_sweeps, 50.0)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
     from alf_amd.runtime import mfma_rate
     result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),

@@ -77,6 +77,9 @@ int alfq_ctx_destroy(alfq_ctx* ctx);
  * precision log) act on the chain chosen with alfq_set_chain (chain 0 initially).
  * Random tapes and alfq_udv_set are single-chain only. */
 int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx** out);
+/* Device bytes one chain of the model takes in a batch (its arena), for admission by
+ * HBM footprint (no counterpart in the reference). */
+int alfq_chain_bytes(const alfq_model_desc* desc, int64_t* bytes);
 int alfq_set_chain(alfq_ctx* ctx, int32_t chain);
 int alfq_batch_size(alfq_ctx* ctx, int32_t* nchain);
 /* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */

@@ -403,3 +403,76 @@ def test_arithmetic_rate_probe(rt):
     assert 5.0 < f64 < 78.6 * 1.05
     assert 10.0 < f32 < 157.3 * 1.05
     assert 5.0 < v64 < 78.6 * 1.05
+
+
+@pytest.mark.parametrize("lattice,L,U,beta,nwrap,n_phi", [("Honeycomb", 12, 3.0, 1.2, 8, 18), ("Square", 24, 4.0, 1.0, 10, 0)])
+def test_large_lattice_sweep_matches_oracle(rt, lattice, L, U, beta, nwrap, n_phi):
+    """Real-size lattices beyond Ndim = 256 (the QR runs its LDS-panel path): the 12 x 12
+    honeycomb with a flux of pi/4 per plaquette (Ndim = 288, complex exp(-dtau T)) and the
+    24 x 24 square lattice (Ndim = 576), at reduced beta so the oracle stays fast."""
+    ham = hubbard(L1=L, L2=L, beta=beta, U=U, nwrap=nwrap, lattice=lattice, n_phi=n_phi)
+    assert ham.Ndim > 256
+    f = fields_for(ham, 21)
+    u = tape(4 * ham.Ndim * ham.Ltrot * 3)
+    s = oracle(ham, f, u)
+    e = engine(rt, ham, f, u, delay=32)
+    s.init()
+    e.init()
+    assert rel(e.get_green(), s.G) < TOL
+    s.sweep()
+    e.sweep(1)
+    assert np.array_equal(e.get_fields(), s.f)
+    assert rel(e.get_green(), s.G) < TOL
+    assert abs(e.get_phase() - s.phase) < TOL
+    c = e.control()
+    assert c["ACC_up"] == s.ctrl.ACC_up and c["XMAXG"] < 1e-9
+    e.close()
+
+
+def test_chain_bytes_matches_batch_stride(rt):
+    """alfq_chain_bytes is the per-chain arena a batch allocates: positive, linear in nothing
+    else, and growing with Ndim and Ltrot."""
+    a = rt.chain_bytes(hubbard(L1=6, L2=6, beta=2.0, nwrap=10))
+    b = rt.chain_bytes(hubbard(L1=8, L2=8, beta=2.0, nwrap=10))
+    c = rt.chain_bytes(hubbard(L1=8, L2=8, beta=4.0, nwrap=10))
+    assert 0 < a < b < c and a % 4096 == 0
+
+
+def test_ndim_1024_sweep_is_self_consistent(rt):
+    """32 x 32 lattice (Ndim = 1024, QR panels of width 8 in LDS): no oracle at this size in
+    test time, so the check is the sweep's own Precision Green / Phase, i.e. the propagated G
+    against the G rebuilt in FP64 from the UDV stack at every stabilisation point."""
+    ham = hubbard(L1=32, L2=32, beta=0.8, U=4.0, nwrap=4)
+    e = rt.Engine(ham, delay=32)
+    e.set_fields(fields_for(ham, 2))
+    e.set_philox(11, 0)
+    e.init()
+    e.sweep(1)
+    c = e.control()
+    assert c["NCG"] >= 4
+    assert c["XMAXG"] < 1e-9 and c["XMAXP"] < 1e-9
+    assert 0.1 < c["ACC_up"] / c["NC_up"] < 0.9
+    e.close()
+
+
+def test_serving_results_same_alone_and_in_mix(rt):
+    """Requests served in a mix (shared batches, batches overlapping on the device, a late
+    arrival admitted while others run) give the same result as each request served alone."""
+    from alf_amd.serving import Request, Server
+    reqs = [Request(0, 6, 2.0, 4.0, seed=3, nsweeps=3, arrival=0.0),
+            Request(1, 6, 2.0, 2.5, seed=4, nsweeps=3, arrival=0.0),
+            Request(2, 4, 1.0, 4.0, seed=5, nsweeps=3, arrival=0.0),
+            Request(3, 6, 2.0, 6.0, seed=6, nsweeps=3, arrival=0.05),
+            Request(4, 8, 1.5, 3.0, seed=7, nsweeps=2, arrival=0.1)]
+    srv = Server(memory_budget=8 << 30)
+    mix = srv.run(reqs)
+    assert [r.rid for r in mix] == [0, 1, 2, 3, 4]
+    assert mix[0].batch_size == 2 and srv.peak_batches >= 2
+    for r in (reqs[1], reqs[3], reqs[4]):
+        solo = Server(memory_budget=8 << 30).run([Request(**{**r.__dict__, "arrival": 0.0})])[0]
+        m = mix[r.rid]
+        assert solo.batch_size == 1
+        assert np.array_equal(solo.fields, m.fields)
+        assert rel(solo.green, m.green) < TOL
+        assert abs(solo.phase - m.phase) < TOL
+        assert m.control["XMAXG"] < 1e-8

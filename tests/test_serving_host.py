@@ -1,0 +1,106 @@
+"""Host logic of the serving loop (admission by memory, batching by shape, overlap of
+batches, latency accounting) with a stand-in backend that sleeps per sweep: no device is
+touched."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from alf_amd.serving import Request, Server, latency_summary, poisson_requests
+
+SWEEP_S = 0.04      # one sweep of the stand-in backend
+
+
+class FakeEngine:
+    def __init__(self, reqs):
+        self.reqs = reqs
+        self.closed = False
+
+    def close(self):
+        self.closed = True
+
+
+class FakeBackend:
+    def __init__(self, bytes_per_chain, fail_rid=None):
+        self.bpc = bytes_per_chain
+        self.fail_rid = fail_rid
+        self.started = []
+        self.lock = threading.Lock()
+
+    def chain_bytes(self, r):
+        return self.bpc[r.L]
+
+    def start(self, reqs):
+        e = FakeEngine(list(reqs))
+        with self.lock:
+            self.started.append(e)
+        if self.fail_rid is not None and any(r.rid == self.fail_rid for r in reqs):
+            raise RuntimeError("engine failure")
+        return e
+
+    def run(self, eng, nsweeps):
+        time.sleep(SWEEP_S * nsweeps)
+
+    def collect(self, eng, z):
+        r = eng.reqs[z]
+        return np.full((2, 2), r.rid), np.eye(2), 1 + 0j, {"XMAXG": 0.0}
+
+
+def run(reqs, budget, bpc=None, max_batch=64, fail_rid=None):
+    be = FakeBackend(bpc or {4: 10, 6: 20}, fail_rid)
+    srv = Server(memory_budget=budget, max_batch=max_batch, backend=be)
+    return srv, be, srv.run(reqs)
+
+
+def test_same_shape_requests_share_a_batch():
+    reqs = [Request(i, 4, 1.0, 4.0 + i, seed=i, nsweeps=3, arrival=0.0) for i in range(3)]
+    srv, be, res = run(reqs, budget=1000)
+    assert len(be.started) == 1 and all(r.batch_size == 3 for r in res)
+    assert all(e.closed for e in be.started)
+    assert [r.rid for r in res] == [0, 1, 2]
+    assert all(int(r.fields[0, 0]) == r.rid for r in res)
+    assert all(3 * SWEEP_S <= r.latency < 3 * SWEEP_S + 0.5 for r in res)
+
+
+def test_memory_budget_serialises_batches_in_arrival_order():
+    a = Request(0, 4, 1.0, 4.0, seed=0, nsweeps=2, arrival=0.0)
+    b = Request(1, 6, 1.0, 4.0, seed=1, nsweeps=2, arrival=0.0)
+    srv, be, res = run([a, b], budget=25)           # 10 + 20 B do not fit together
+    ra, rb = res
+    assert srv.peak_batches == 1
+    assert rb.admitted >= ra.done - 1e-3
+    assert rb.latency > ra.latency
+
+
+def test_batches_run_concurrently_and_short_ones_finish_first():
+    a = Request(0, 4, 1.0, 4.0, seed=0, nsweeps=10, arrival=0.0)
+    b = Request(1, 6, 1.0, 4.0, seed=1, nsweeps=2, arrival=0.05)
+    srv, be, res = run([a, b], budget=1000)
+    ra, rb = res
+    assert srv.peak_batches == 2
+    assert 0.05 <= rb.admitted < ra.done              # admitted while a runs
+    assert rb.done < ra.done                          # and finished before it
+    assert ra.done < 10 * SWEEP_S + 2 * SWEEP_S       # the two overlapped
+
+
+def test_batch_size_cap_oversize_request_and_failures():
+    reqs = [Request(i, 4, 1.0, 4.0, seed=i, nsweeps=1, arrival=0.0) for i in range(5)]
+    srv, be, res = run(reqs, budget=1000, max_batch=2)
+    assert sorted(len(e.reqs) for e in be.started) == [1, 2, 2]
+    with pytest.raises(RuntimeError, match="above the budget"):
+        run([Request(9, 6, 1.0, 4.0, seed=0, nsweeps=1)], budget=10)
+    with pytest.raises(RuntimeError, match="engine failure"):
+        run(reqs[:2], budget=1000, fail_rid=1)
+
+
+def test_poisson_stream_and_summary():
+    reqs = poisson_requests(40, rate=50.0, seed=3, nsweeps=2)
+    gaps = np.diff([r.arrival for r in reqs])
+    assert np.all(gaps > 0) and 0.005 < gaps.mean() < 0.05
+    assert {r.L for r in reqs} <= {8, 12, 16, 24, 32} and {r.ltrot for r in reqs} <= {40, 80, 160}
+    srv, be, res = run(reqs[:6], budget=10 ** 9, bpc={L: 1 for L in (8, 12, 16, 24, 32)})
+    s = latency_summary(res, srv.wall)
+    assert s["requests"] == 6 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
+    assert abs(s["aggregate_sweeps_per_s"] - 12 / srv.wall) < 1e-9
+    assert all(r.latency >= 2 * SWEEP_S for r in res)
