@@ -23,6 +23,7 @@
 
 #include "../../include/alfq.h"
 #include "alfq_chain.h"
+#include "alfq_obs.h"
 #include "alfq_qr.h"
 
 static thread_local std::string g_last_error;
@@ -163,6 +164,13 @@ struct alfq_ctx {
   bool t_real = true;
   int K = 32;
   int prec = ALFQ_PREC_FP64;             // arithmetic of the hopping wraps of G
+  // equal-time measurements
+  ObsDev obs;
+  long long obs_cs = 0;                  // bytes per chain of the measurement blocks
+  int* obs_tables = nullptr;             // site_of, jm, oi, oj (shared)
+  std::vector<int> obs_key;              // host copy of the shared tables (batch consistency)
+  bool obs_on = false;
+  int lobs_st = 1, lobs_en = 0;
   hipStream_t s = nullptr;
   // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
   int B = 1;
@@ -236,10 +244,12 @@ struct alfq_ctx {
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
     int var_r, var_l, rounds, K, rng_mode, prec;
+    bool obs_on;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
-             K == o.K && rng_mode == o.rng_mode && prec == o.prec && var_st == o.var_st && refs == o.refs;
+             K == o.K && rng_mode == o.rng_mode && prec == o.prec && obs_on == o.obs_on && var_st == o.var_st &&
+             refs == o.refs;
     }
   };
   struct SweepGraph {
@@ -581,6 +591,12 @@ int sweep_dev(alfq_ctx* c) {
   return st;
 }
 
+// ham%Obser on slice NTAU1 when measurements are on and NTAU1 is in LOBS_ST..LOBS_EN
+int measure_dev(alfq_ctx* c, int ntau1) {
+  if (!c->obs_on || ntau1 < c->lobs_st || ntau1 > c->lobs_en) return ST_NONE;
+  return obs_measure(c->obs, c->G, c->st, c->B, c->cs, c->obs_cs, c->s);
+}
+
 int sweep_body(alfq_ctx* c) {
   int st;
   const auto& s = c->stab;
@@ -598,12 +614,14 @@ int sweep_body(alfq_ctx* c) {
       if ((st = cgr_dev(c, nvar, c->var_r, c->var_l, 1))) return st;
       ++nst;
     }
+    if ((st = measure_dev(c, ntau1))) return st;          // main.F90:757-772
   }
   if ((st = udv_reset_dev(c, &c->var_l, 'L'))) return st;
   nst = c->nstm - 1;
   for (int ntau = L; ntau >= 1; --ntau) {
     const int ntau1 = ntau - 1;
     if ((st = wrapgrdo_dev(c, ntau))) return st;
+    if ((st = measure_dev(c, ntau1))) return st;          // main.F90:789-802, before stabilising
     if (nst >= 0 && s[nst] == ntau1 && ntau1 != 0) {
       if ((st = wrap_udv_dev(c, &c->var_l, ntau1, s[nst + 1], true))) return st;
       var_bind(c, &c->var_r, c->var_st[nst]);
@@ -625,6 +643,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   h.G = c->G; h.Gtmp = c->Gtmp; h.u = c->u;
   h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
   h.prec = c->prec;
+  h.obs_on = c->obs_on;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -1044,6 +1063,8 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (!c) return ST_NONE;
   if (c->s) hipStreamSynchronize(c->s);
   drop_graphs(c);
+  if (c->obs.buf) hipFree(c->obs.buf);
+  if (c->obs_tables) hipFree(c->obs_tables);
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -1069,6 +1090,131 @@ int alfq_set_precision(alfq_ctx* c, int32_t mode) {
     return ST_GENERIC;
   }
   c->prec = mode;
+  return ST_NONE;
+}
+
+int alfq_obs_setup(alfq_ctx* c, const alfq_obs_desc* d) {
+  NULL_STREAM_LOCK;
+  if (!c || !d || !d->invlist || !d->imj || d->nnz < 0 || (d->nnz > 0 && (!d->o_row || !d->o_col || !d->o_val))) {
+    alfq_set_error("obs_setup: null argument");
+    return ST_GENERIC;
+  }
+  const int nc = d->ncell, no = d->norb, n = c->n, nnz = d->nnz;
+  if (nc < 1 || no < 1 || (long long)nc * no != n) {
+    alfq_set_error("obs_setup: ncell x norb must equal Ndim");
+    return ST_HAMILTONIAN;
+  }
+  if (d->lobs_st < 1 || d->lobs_en > c->L || d->lobs_st > d->lobs_en) {
+    alfq_set_error("obs_setup: measuring interval outside 1..Ltrot");
+    return ST_GENERIC;
+  }
+  // shared tables: site of (cell, orbital), partner cell of (cell, distance class), O's pattern
+  std::vector<int> key;
+  key.reserve(3 + (size_t)nc * no + (size_t)nc * nc + 2 * (size_t)nnz);
+  key.push_back(nc); key.push_back(no); key.push_back(nnz);
+  std::vector<int> seen(n, 0);
+  for (int I = 0; I < nc; ++I)
+    for (int a = 0; a < no; ++a) {
+      const int v = d->invlist[I + (long long)nc * a] - 1;
+      if (v < 0 || v >= n || seen[v]++) { alfq_set_error("obs_setup: Invlist is not a permutation of the sites"); return ST_HAMILTONIAN; }
+      key.push_back(v);
+    }
+  std::vector<int> jm((size_t)nc * nc, -1);
+  for (int I = 0; I < nc; ++I)
+    for (int J = 0; J < nc; ++J) {
+      const int dc = d->imj[I + (long long)nc * J] - 1;
+      if (dc < 0 || dc >= nc || jm[(size_t)I * nc + dc] >= 0) {
+        alfq_set_error("obs_setup: imj(I, :) is not a permutation of the cells");
+        return ST_HAMILTONIAN;
+      }
+      jm[(size_t)I * nc + dc] = J;
+    }
+  key.insert(key.end(), jm.begin(), jm.end());
+  for (int e = 0; e < nnz; ++e) {
+    const int i = d->o_row[e] - 1, j = d->o_col[e] - 1;
+    if (i < 0 || i >= n || j < 0 || j >= n) { alfq_set_error("obs_setup: hopping entry outside the sites"); return ST_HAMILTONIAN; }
+    key.push_back(i);
+  }
+  for (int e = 0; e < nnz; ++e) key.push_back(d->o_col[e] - 1);
+  auto r256 = [](long long b) { return (b + 255) & ~255LL; };
+  if (!c->obs.buf) {
+    ObsDev o;
+    o.n = n; o.ncell = nc; o.norb = no; o.nnz = nnz; o.n_sun = c->n_sun;
+    o.off_u = 0;
+    o.off_ov = 256;
+    o.off_head = r256(o.off_ov + (long long)nnz * (long long)sizeof(cplx));
+    o.off_den0 = o.off_head + r256(sizeof(ObsHead));
+    o.off_latt = o.off_den0 + r256((long long)no * sizeof(cplx));
+    const long long end = o.off_latt + 3LL * nc * no * no * (long long)sizeof(cplx);
+    const long long ocs = (end + 4095) & ~4095LL;
+    const size_t ntab = (size_t)nc * no + (size_t)nc * nc + 2 * (size_t)nnz;
+    if (hipMalloc(&o.buf, (size_t)ocs * c->B) != hipSuccess) { alfq_set_error("obs_setup: hipMalloc"); return ST_GENERIC; }
+    if (hipMalloc(&c->obs_tables, std::max<size_t>(ntab, 1) * sizeof(int)) != hipSuccess) {
+      hipFree(o.buf);
+      alfq_set_error("obs_setup: hipMalloc");
+      return ST_GENERIC;
+    }
+    HIP_OK(hipMemset(o.buf, 0, (size_t)ocs * c->B));
+    HIP_OK(hipMemcpy(c->obs_tables, key.data() + 3, ntab * sizeof(int), hipMemcpyHostToDevice));
+    o.site_of = c->obs_tables;
+    o.jm = o.site_of + (size_t)nc * no;
+    o.oi = o.jm + (size_t)nc * nc;
+    o.oj = o.oi + nnz;
+    c->obs = o;
+    c->obs_cs = ocs;
+    c->obs_key = key;
+  } else if (key != c->obs_key) {
+    alfq_set_error("obs_setup: the lattice tables differ from the batch's");
+    return ST_HAMILTONIAN;
+  }
+  char* blk = c->obs.buf + (long long)c->cur * c->obs_cs;
+  const double u = d->ham_u;
+  HIP_OK(hipMemcpy(blk + c->obs.off_u, &u, sizeof(double), hipMemcpyHostToDevice));
+  if (nnz > 0) HIP_OK(hipMemcpy(blk + c->obs.off_ov, d->o_val, (size_t)nnz * sizeof(cplx), hipMemcpyHostToDevice));
+  c->lobs_st = d->lobs_st;
+  c->lobs_en = d->lobs_en;
+  return ST_NONE;
+}
+
+int alfq_obs_enable(alfq_ctx* c, int32_t on) {
+  if (!c) return ST_GENERIC;
+  if (on && !c->obs.buf) { alfq_set_error("obs_enable: call alfq_obs_setup first"); return ST_GENERIC; }
+  c->obs_on = on != 0;
+  return ST_NONE;
+}
+
+int alfq_obs_reset(alfq_ctx* c) {
+  NULL_STREAM_LOCK;
+  if (!c) return ST_GENERIC;
+  if (!c->obs.buf) return ST_NONE;
+  HIP_OK(hipStreamSynchronize(c->s));
+  for (int z = 0; z < c->B; ++z)
+    HIP_OK(hipMemsetAsync(c->obs.buf + (long long)z * c->obs_cs + c->obs.off_head, 0,
+                          (size_t)(c->obs_cs - c->obs.off_head), c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  return ST_NONE;
+}
+
+int alfq_obs_get(alfq_ctx* c, alfq_obs_sums* sums, double* latt, double* den0) {
+  NULL_STREAM_LOCK;
+  if (!c || !sums) return ST_GENERIC;
+  if (!c->obs.buf) { alfq_set_error("obs_get: call alfq_obs_setup first"); return ST_GENERIC; }
+  HIP_OK(hipStreamSynchronize(c->s));
+  const char* blk = c->obs.buf + (long long)c->cur * c->obs_cs;
+  ObsHead h;
+  HIP_OK(hipMemcpy(&h, blk + c->obs.off_head, sizeof(h), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 4; ++q) {
+    sums->scal[2 * q] = h.scal[q].x;
+    sums->scal[2 * q + 1] = h.scal[q].y;
+  }
+  sums->sign_scal = h.sign_scal;
+  sums->n_scal = h.n_scal;
+  sums->sign_eq = h.sign_eq;
+  sums->n_eq = h.n_eq;
+  const int nc = c->obs.ncell, no = c->obs.norb;
+  if (latt)
+    HIP_OK(hipMemcpy(latt, blk + c->obs.off_latt, 3ULL * nc * no * no * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (den0) HIP_OK(hipMemcpy(den0, blk + c->obs.off_den0, (size_t)no * sizeof(cplx), hipMemcpyDeviceToHost));
   return ST_NONE;
 }
 

@@ -174,3 +174,55 @@ def predefined_latt(lattice_type: str, L1: int, L2: int):
             Invlist[I, no] = nc
             nc += 1
     return ndim, List, Invlist, latt, unit
+
+
+def momentum_list(latt: Lattice) -> np.ndarray:
+    """Listk (lattices_v3_mod.F90:243-275): integer coordinates (m, n) of the N momenta
+    m b1 + n b2 in the Wigner-Seitz cell of the reciprocal lattice spanned by BZ1, BZ2
+    (m outer, n inner)."""
+    L = latt.L
+    out = []
+    for m in range(-L, L + 1):
+        for n in range(-L, L + 1):
+            xk = m * latt.b1_p + n * latt.b2_p
+            ok = True
+            for b_p in (latt.BZ2_p, latt.BZ1_p, latt.BZ2_p - latt.BZ1_p, latt.BZ2_p + latt.BZ1_p):
+                s = float(xk @ b_p)
+                h = float(b_p @ b_p) / 2.0
+                if not (s <= h + EPS_SMALL and s >= -h + EPS_SMALL):
+                    ok = False
+                    break
+            if ok:
+                out.append((m, n))
+    if len(out) != latt.N:
+        raise ValueError(f"momentum_list: {len(out)} momenta for {latt.N} cells")
+    return np.array(out, dtype=np.int64)
+
+
+def distance_classes(latt: Lattice) -> np.ndarray:
+    """imj (lattices_v3_mod.F90:316-330): imj[I, J] is the cell whose position equals
+    R_I - R_J folded back into the cluster (0-based)."""
+    N = latt.N
+    imj = np.zeros((N, N), dtype=np.int64)
+    for nr in range(N):
+        x_p = latt.list[nr, 0] * latt.a1_p + latt.list[nr, 1] * latt.a2_p
+        for nr1 in range(N):
+            x1_p = latt.list[nr1, 0] * latt.a1_p + latt.list[nr1, 1] * latt.a2_p
+            d_p = x_p - x1_p
+            x1 = npbc(d_p, latt.L1_p, latt.L2_p)
+            d_p = npbc(x1, latt.L1_p, latt.L2_p)
+            i1 = int(np.rint(float(latt.BZ1_p @ d_p) / TWOPI))
+            i2 = int(np.rint(float(latt.BZ2_p @ d_p) / TWOPI))
+            imj[nr, nr1] = latt.invlist[(i1, i2)]
+    return imj
+
+
+def correlation_partner(imj: np.ndarray) -> np.ndarray:
+    """jm[I, d] = the cell J with imj[I, J] = d (one per (I, d) on a periodic cluster)."""
+    N = imj.shape[0]
+    jm = np.full((N, N), -1, dtype=np.int64)
+    for I in range(N):
+        jm[I, imj[I]] = np.arange(N)
+    if (jm < 0).any():
+        raise ValueError("distance classes are not a bijection per cell")
+    return jm

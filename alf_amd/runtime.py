@@ -35,6 +35,7 @@ EXPORTS = [
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
+    "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get",
 ]
 
 
@@ -46,6 +47,18 @@ class ModelDesc(C.Structure):
         ("op_P", C.POINTER(C.c_int32)), ("op_type", C.POINTER(C.c_int32)),
         ("op_E", C.POINTER(C.c_double)), ("op_g", C.POINTER(C.c_double)), ("op_alpha", C.POINTER(C.c_double)),
     ]
+
+
+class ObsDesc(C.Structure):
+    _fields_ = [("ncell", C.c_int32), ("norb", C.c_int32), ("nnz", C.c_int32), ("lobs_st", C.c_int32),
+                ("lobs_en", C.c_int32), ("invlist", C.POINTER(C.c_int32)), ("imj", C.POINTER(C.c_int32)),
+                ("o_row", C.POINTER(C.c_int32)), ("o_col", C.POINTER(C.c_int32)),
+                ("o_val", C.POINTER(C.c_double)), ("ham_u", C.c_double)]
+
+
+class ObsSums(C.Structure):
+    _fields_ = [("scal", C.c_double * 8), ("sign_scal", C.c_double), ("n_scal", C.c_int64),
+                ("sign_eq", C.c_double), ("n_eq", C.c_int64)]
 
 
 class Control(C.Structure):
@@ -80,6 +93,10 @@ def load_library(path: str = LIB_PATH):
         "alfq_set_precision": [P, C.c_int32],
         "alfq_mfma_rate": [C.c_int32, dp],
         "alfq_chain_bytes": [C.POINTER(ModelDesc), C.POINTER(C.c_int64)],
+        "alfq_obs_setup": [P, C.POINTER(ObsDesc)],
+        "alfq_obs_enable": [P, C.c_int32],
+        "alfq_obs_reset": [P],
+        "alfq_obs_get": [P, C.POINTER(ObsSums), dp, dp],
         "alfq_fields_set": [P, ip, C.c_int32, C.c_int32],
         "alfq_fields_get": [P, ip, C.c_int32, C.c_int32],
         "alfq_green_set": [P, dp, C.c_int32],
@@ -164,6 +181,24 @@ def model_desc(model):
     return d, keep
 
 
+def obs_desc(model, lobs=None):
+    """alfq_obs_desc of a host Hubbard model (hopping matrix, lattice tables, U) with the
+    arrays it points to; lobs = (LOBS_ST, LOBS_EN), default every slice 1..Ltrot."""
+    from alf_amd.host import lattices
+    latt, unit = model.latt, model.unit
+    O = np.asarray(model.hop_matrix, dtype=np.complex128)
+    rows, cols = np.nonzero(O)
+    imj = lattices.distance_classes(latt) + 1
+    keep = dict(invlist=np.asfortranarray(np.asarray(model.Invlist) + 1, dtype=np.int32),
+                imj=np.asfortranarray(imj, dtype=np.int32),
+                row=(rows + 1).astype(np.int32), col=(cols + 1).astype(np.int32),
+                val=np.ascontiguousarray(O[rows, cols], dtype=np.complex128))
+    st, en = lobs if lobs is not None else (1, model.Ltrot)
+    d = ObsDesc(latt.N, unit.Norb, len(rows), int(st), int(en), _iptr(keep["invlist"]), _iptr(keep["imj"]),
+                _iptr(keep["row"]), _iptr(keep["col"]), _dptr(keep["val"]), float(model.params.Ham_U))
+    return d, keep
+
+
 class Engine:
     """One Markov chain on the GPU behind the C ABI.
 
@@ -200,6 +235,31 @@ class Engine:
     # ---- state ----
     def set_delay(self, k):
         self._c(self.lib.alfq_set_delay(self.ctx, int(k)))
+
+    # ---- equal-time measurements ----
+    def obs_setup(self, model=None, lobs=None):
+        """Measurement tables of `model` (default: this engine's) for the selected chain."""
+        d, keep = obs_desc(model if model is not None else self.model, lobs)
+        self._c(self.lib.alfq_obs_setup(self.ctx, C.byref(d)))
+        self._obs_shape = (int(d.ncell), int(d.norb))
+        del keep
+
+    def obs_enable(self, on=True):
+        self._c(self.lib.alfq_obs_enable(self.ctx, int(bool(on))))
+
+    def obs_reset(self):
+        self._c(self.lib.alfq_obs_reset(self.ctx))
+
+    def obs_get(self):
+        """The selected chain's measurement sums (Obs_scal / Obs_eq of the reference)."""
+        nc, no = self._obs_shape
+        s = ObsSums()
+        latt = np.zeros(3 * nc * no * no, dtype=np.complex128)
+        den0 = np.zeros(no, dtype=np.complex128)
+        self._c(self.lib.alfq_obs_get(self.ctx, C.byref(s), _dptr(latt), _dptr(den0)))
+        sc = np.array(s.scal[:]).view(np.complex128)
+        return {"scal": sc, "sign_scal": s.sign_scal, "n_scal": s.n_scal, "latt": latt.reshape(3, nc, no, no),
+                "den0": den0, "sign_eq": s.sign_eq, "n_eq": s.n_eq}
 
     def set_precision(self, mode):
         """PREC_FP64 (drop-in) or PREC_FP32_WRAPS (hopping wraps of G on FP32 MFMA)."""
@@ -378,6 +438,13 @@ class BatchEngine(Engine):
         self.nchain = len(descs)
         if delay is not None:
             self.set_delay(delay)
+
+    def obs_setup(self, model=None, lobs=None):
+        """Measurement tables for every chain of the batch, each with its own model."""
+        for z, m in enumerate(self.models):
+            self.chain(z)
+            Engine.obs_setup(self, m, lobs)
+        self.chain(0)
 
     def chain(self, z):
         """Select chain z for the per-chain accessors; returns self for chaining calls."""

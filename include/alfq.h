@@ -180,6 +180,42 @@ int alfq_sweep_graphs(alfq_ctx* ctx, int32_t enable);
  * kind 0: FP64 MFMA v_mfma_f64_16x16x4f64, 1: FP32 MFMA v_mfma_f32_16x16x4f32,
  * 2: vector FP64 FMA, in TFLOP/s over all CUs of the current device. */
 int alfq_mfma_rate(int32_t kind, double* tflops);
+/* ---- equal-time measurements (Hubbard plugin, N_FL = 1) ------------------- */
+/* Replaces ham%Obser (Prog/Hamiltonians/Hamiltonian_Hubbard_smod.F90:695-801) called by the
+ * sweep on the measured slices LOBS_ST..LOBS_EN (main.F90:757-772, 789-802) with Symm = .F.:
+ * scalars Kin, Pot, Part, Ener and the correlations Green, SpinZ, Den by distance class,
+ * accumulated on the device per chain.  The geometry (Invlist, Latt%imj) is shared by the
+ * chains of a batch; the hopping values and ham_U are set for the selected chain. */
+typedef struct {
+  int32_t ncell;           /* Latt%N */
+  int32_t norb;            /* Latt_unit%Norb */
+  int32_t nnz;             /* nonzero entries of the hopping matrix O of H_T = c^+ O c */
+  int32_t lobs_st;         /* first and last measured slice NTAU1 (1-based) */
+  int32_t lobs_en;
+  const int32_t* invlist;  /* ncell x norb, column-major: Invlist(I, no), 1-based sites */
+  const int32_t* imj;      /* ncell x ncell, column-major: Latt%imj(I, J), 1-based cells */
+  const int32_t* o_row;    /* nnz 1-based rows and columns of the nonzero O entries */
+  const int32_t* o_col;
+  const double* o_val;     /* nnz complex values (re, im) */
+  double ham_u;
+} alfq_obs_desc;
+
+typedef struct {
+  double scal[8];          /* sums of Kin, Pot, Part, Ener times ZP ZS, (re, im) each */
+  double sign_scal;        /* Obs_scal%Ave_sign sum, and Obs_scal%N */
+  int64_t n_scal;
+  double sign_eq;          /* Obs_eq%Ave_sign sum, and Obs_eq%N */
+  int64_t n_eq;
+} alfq_obs_sums;
+
+int alfq_obs_setup(alfq_ctx* ctx, const alfq_obs_desc* desc);
+/* on != 0: measure during alfq_sweep (off after alfq_obs_setup). */
+int alfq_obs_enable(alfq_ctx* ctx, int32_t on);
+/* Zero every chain's sums (the bin boundary of Print_bin_*). */
+int alfq_obs_reset(alfq_ctx* ctx);
+/* The selected chain's sums: latt[((q ncell + d) norb + a) norb + b] complex (re, im) with
+ * q = 0 Green, 1 SpinZ, 2 Den and d the 0-based distance class; den0[norb] complex. */
+int alfq_obs_get(alfq_ctx* ctx, alfq_obs_sums* sums, double* latt, double* den0);
 /* Library build identification. */
 const char* alfq_version(void);
 

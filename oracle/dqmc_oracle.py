@@ -383,6 +383,8 @@ class OracleSweep:
         self.udvr = UDV.identity(n, "R")
         self.udvst = [None] + [UDV.identity(n, "L") for _ in range(self.nstm)]
         self.G = None
+        self.obs = None                 # oracle.observables.HubbardObs, measured when set
+        self.lobs = None                # (LOBS_ST, LOBS_EN), default 1..Ltrot
         self.phase = 1.0 + 0j
         self.trace = []        # (event, ntau, G-copy or None, phase) at stabilisation points
 
@@ -508,6 +510,14 @@ class OracleSweep:
         self.phase = z ** self.m.n_sun
         self.G = G
 
+    def _measure(self, ntau1):
+        """ham%Obser(GR, PHASE, NTAU1, 1.0) on the measured slices (Symm = .F.)."""
+        if self.obs is None:
+            return
+        st, en = self.lobs if self.lobs is not None else (1, self.m.ltrot)
+        if st <= ntau1 <= en:
+            self.obs.measure(self.G, self.phase)
+
     def sweep(self):
         """One NSW iteration of the sequential sweep (main.F90:714-887)."""
         m = self.m
@@ -525,11 +535,13 @@ class OracleSweep:
                 nvar = 2 if ntau1 > L // 2 else 1
                 self._stabilise(nvar, ntau1)
                 nst += 1
+            self._measure(ntau1)                       # main.F90:757-772
         self.udvl = UDV.identity(m.ndim, "L")
         nst = self.nstm - 1
         for ntau in range(L, 0, -1):
             ntau1 = ntau - 1
             self.G = self.wrapgrdo(self.G, ntau)
+            self._measure(ntau1)                       # main.F90:789-802, before stabilising
             if nst >= 0 and s[nst] == ntau1 and ntau1 != 0:
                 self.udvl = self.wrapul(s[nst + 1], ntau1, self.udvl)
                 self.udvr = self.udvst[nst]

@@ -213,3 +213,103 @@ def test_full_sweep_against_brute_force_and_variants():
     assert rel(a.G, b.G) < 1e-11
     Gb, _ = O.brute_force_green(a.m, a.f, 0, dps=30)
     assert rel(a.G, Gb) < 1e-10
+
+
+# ---------------------------------------------------------------- equal-time observables
+def _obs_oracle(ham):
+    from alf_amd.host import lattices as LT
+    from oracle.observables import HubbardObs
+    imj = LT.distance_classes(ham.latt)
+    return HubbardObs(ham.hop_matrix, ham.params.Ham_U, ham.N_SUN, imj, ham.List[:, 0], ham.List[:, 1],
+                      ham.unit.Norb), imj
+
+
+def _positions(latt, cells):
+    return np.array([latt.list[c, 0] * latt.a1_p + latt.list[c, 1] * latt.a2_p for c in cells])
+
+
+@pytest.mark.parametrize("lattice,L", [("Square", 4), ("Square", 6), ("Honeycomb", 3)])
+def test_distance_classes_and_momenta(lattice, L):
+    """imj[I, J] is R_I - R_J modulo the cluster periods; the momenta are commensurate with
+    the cluster, distinct modulo the reciprocal lattice, and orthogonal on the cells."""
+    from alf_amd.host import lattices as LT
+    ham = hubbard(L1=L, L2=L, lattice=lattice)
+    latt = ham.latt
+    imj = LT.distance_classes(latt)
+    R = _positions(latt, range(latt.N))
+    P = np.array([latt.L1_p, latt.L2_p]).T
+    for I in range(latt.N):
+        for J in range(latt.N):
+            c = np.linalg.solve(P, R[I] - R[J] - R[imj[I, J]])
+            assert np.allclose(c, np.rint(c), atol=1e-9)
+    k = LT.momentum_list(latt)
+    K = np.array([m * latt.b1_p + n * latt.b2_p for m, n in k])
+    assert np.allclose(np.exp(1j * K @ latt.L1_p), 1) and np.allclose(np.exp(1j * K @ latt.L2_p), 1)
+    A = np.exp(1j * K @ R.T)
+    S = A @ A.conj().T / latt.N                                           # sum_r e^{i (k - k') r} / N
+    assert np.allclose(S, np.eye(latt.N), atol=1e-9)
+
+
+def _free_green(ham, beta):
+    e, V = np.linalg.eigh(ham.hop_matrix)
+    f = 1.0 / (1.0 + np.exp(beta * e))
+    G = V @ np.diag(1.0 - f) @ V.conj().T           # <c_i c^+_j> of H = c^+ O c
+    return G, e, f
+
+
+@pytest.mark.parametrize("lattice,L,chem", [("Square", 4, 0.3), ("Square", 6, -0.7), ("Honeycomb", 3, 0.4)])
+def test_observables_of_free_fermions(lattice, L, chem):
+    """U -> 0: Kin, Part, the Green correlation at every k and the k = 0 spin and density
+    correlations have closed forms in the eigenvalues of the hopping matrix."""
+    from alf_amd.host import lattices as LT
+    from oracle.observables import fourier_r_to_k
+    ham = hubbard(L1=L, L2=L, lattice=lattice, chem=chem, beta=2.0)
+    obs, imj = _obs_oracle(ham)
+    G, e, f = _free_green(ham, 2.0)
+    obs.measure(G, 1.0 + 0j)
+    obs.measure(G, 1.0 + 0j)
+    b = obs.bins()
+    ns = ham.N_SUN
+    part = ns * f.sum()
+    assert abs(b["scal"][0] - ns * np.sum(e * f)) < 1e-10
+    assert abs(b["scal"][2] - part) < 1e-10
+    assert abs(b["scal"][3] - b["scal"][0] - b["scal"][1]) < 1e-12 and b["sign_scal"] == 1.0
+    latt = ham.latt
+    R = _positions(latt, range(latt.N))
+    k = LT.momentum_list(latt)
+    K = np.array([m * latt.b1_p + n * latt.b2_p for m, n in k])
+    X = fourier_r_to_k(b["latt"].reshape(3, latt.N, -1).transpose(1, 0, 2), R, K)   # (Nk, 3, norb^2)
+    k0 = int(np.argmin(np.linalg.norm(K, axis=1)))
+    ff = np.sum(f * (1 - f))
+    # sum rules at k = 0 (all orbital pairs): spin N_SUN sum f(1-f) / N, density (Part^2 + that) / N
+    assert abs(X[k0, 1].sum() - ns * ff / latt.N) < 1e-10
+    assert abs(X[k0, 2].sum() - (part ** 2 + ns * ff) / latt.N) < 1e-9
+    assert np.allclose(b["den0"].sum() * latt.N, part)
+    if lattice == "Square":
+        # one orbital: Green(k) = N_SUN f(eps(k)), eps(k) = sum_j O_0j exp(i k (R_j - R_0))
+        O = ham.hop_matrix
+        site0 = ham.Invlist[0, 0]
+        for ik in range(latt.N):
+            eps = sum(O[site0, ham.Invlist[j, 0]] * np.exp(1j * K[ik] @ (R[j] - R[0])) for j in range(latt.N))
+            assert abs(X[ik, 0, 0] - ns / (1 + np.exp(2.0 * eps.real))) < 1e-10
+
+
+def test_potential_energy_at_half_filling_and_phase_factors():
+    """Pot = U n / 4 for free fermions at half filling on a bipartite lattice; a negative
+    real phase flips every accumulated value and the sign; a complex phase multiplies them
+    by PHASE / Re(PHASE)."""
+    ham = hubbard(L1=4, L2=4, U=3.0, beta=3.0)
+    obs, _ = _obs_oracle(ham)
+    G, e, f = _free_green(ham, 3.0)
+    obs.measure(G, 1.0 + 0j)
+    ref = obs.bins()
+    assert abs(ref["scal"][1] - 3.0 * ham.Ndim / 4) < 1e-10
+    obs.reset()
+    obs.measure(G, -2.0 + 0j)
+    neg = obs.bins()
+    assert neg["sign_scal"] == -1.0 and neg["sign_eq"] == -1.0
+    assert np.allclose(neg["scal"], -ref["scal"]) and np.allclose(neg["latt"], -ref["latt"])
+    obs.reset()
+    obs.measure(G, 2.0 + 1.0j)
+    cpx = obs.bins()
+    assert np.allclose(cpx["scal"], ref["scal"] * (1 + 0.5j)) and cpx["sign_scal"] == 1.0
