@@ -1409,6 +1409,21 @@ int alfq_rng_set_philox(alfq_ctx* c, uint64_t seed, uint32_t stream) {
   return ST_NONE;
 }
 
+int alfq_rng_set_position(alfq_ctx* c, int64_t pos) {
+  NULL_STREAM_LOCK;
+  if (!c || pos < 0) return ST_GENERIC;
+  if (c->rng_mode != 2) { alfq_set_error("rng_set_position: the chain has no Philox stream"); return ST_GENERIC; }
+  HIP_OK(hipStreamSynchronize(c->s));
+  ChainState* stc = at(c, c->st);
+  const unsigned long long p = (unsigned long long)pos, zero = 0;
+  const long long len0 = 0;
+  HIP_OK(hipMemcpy(&stc->rng_pos, &p, sizeof(p), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->u_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(&stc->u_len, &len0, sizeof(len0), hipMemcpyHostToDevice));
+  c->rng_next = p;
+  return ST_NONE;
+}
+
 int alfq_rng_position(alfq_ctx* c, int64_t* pos) {
   NULL_STREAM_LOCK;
   if (!c || !pos) return ST_GENERIC;

@@ -30,11 +30,26 @@ module alfq_mod
     real(c_double) :: xmean_g_sum, xmax_g, xmax_p
   end type alfq_control
 
+  ! equal-time measurements (ham%Obser of the Hubbard plugin on the device)
+  type, bind(C), public :: alfq_obs_desc
+    integer(c_int32_t) :: ncell, norb, nnz, lobs_st, lobs_en
+    type(c_ptr) :: invlist, imj, o_row, o_col, o_val
+    real(c_double) :: ham_u
+  end type alfq_obs_desc
+
+  type, bind(C), public :: alfq_obs_sums
+    real(c_double) :: scal(8)
+    real(c_double) :: sign_scal
+    integer(c_int64_t) :: n_scal
+    real(c_double) :: sign_eq
+    integer(c_int64_t) :: n_eq
+  end type alfq_obs_sums
+
   public :: alfq_ctx_create, alfq_ctx_destroy, alfq_set_delay, alfq_set_precision, alfq_fields_set, alfq_fields_get, &
             alfq_green_set, alfq_green_get, alfq_phase_set, alfq_phase_get, alfq_udv_reset, &
             alfq_udv_assign, alfq_rng_set_tape, alfq_rng_set_philox, alfq_wrapgrup, alfq_wrapgrdo, &
             alfq_wrapur, alfq_wrapul, alfq_cgr, alfq_init, alfq_sweep, alfq_sync, alfq_control_get, &
-            alfq_last_error_string
+            alfq_last_error_string, alfq_obs_setup, alfq_obs_enable, alfq_obs_reset, alfq_obs_get
   public :: alfq_WRAPGRUP_shim, alfq_WRAPGRDO_shim, alfq_WRAPUR_shim, alfq_WRAPUL_shim, alfq_CGR_shim
 
   interface
@@ -56,6 +71,26 @@ module alfq_mod
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: mode
+    end function
+    integer(c_int) function alfq_obs_setup(ctx, desc) bind(C, name="alfq_obs_setup")
+      import :: c_int, c_ptr, alfq_obs_desc
+      type(c_ptr), value :: ctx
+      type(alfq_obs_desc), intent(in) :: desc
+    end function
+    integer(c_int) function alfq_obs_enable(ctx, on) bind(C, name="alfq_obs_enable")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function alfq_obs_reset(ctx) bind(C, name="alfq_obs_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function alfq_obs_get(ctx, sums, latt, den0) bind(C, name="alfq_obs_get")
+      import :: c_int, c_ptr, c_double, alfq_obs_sums
+      type(c_ptr), value :: ctx
+      type(alfq_obs_sums), intent(out) :: sums
+      real(c_double), intent(out) :: latt(*), den0(*)
     end function
     integer(c_int) function alfq_fields_set(ctx, f, n_op, ltrot) bind(C, name="alfq_fields_set")
       import :: c_int, c_ptr, c_int32_t

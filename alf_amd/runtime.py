@@ -35,7 +35,7 @@ EXPORTS = [
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
-    "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get",
+    "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
 ]
 
 
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_mfma_rate": [C.c_int32, dp],
         "alfq_chain_bytes": [C.POINTER(ModelDesc), C.POINTER(C.c_int64)],
         "alfq_obs_setup": [P, C.POINTER(ObsDesc)],
+        "alfq_rng_set_position": [P, C.c_int64],
         "alfq_obs_enable": [P, C.c_int32],
         "alfq_obs_reset": [P],
         "alfq_obs_get": [P, C.POINTER(ObsSums), dp, dp],
@@ -326,6 +327,10 @@ class Engine:
         p = C.c_int64()
         self._c(self.lib.alfq_rng_position(self.ctx, C.byref(p)))
         return p.value
+
+    def set_rng_position(self, pos):
+        """Continue the Philox stream at `pos` (restart from a saved configuration)."""
+        self._c(self.lib.alfq_rng_set_position(self.ctx, int(pos)))
 
     # ---- reference routines ----
     def wrapgrup(self, ntau, phase=None):

@@ -93,6 +93,29 @@ def serving_rate(nreq, nsweeps, rate, seed=0):
     return out
 
 
+def measured_rate(ham, fields, delay, steps):
+    """Sweeps/s of one chain with the equal-time measurements on (2 Ltrot - 1 per sweep) and
+    the measured energy and double occupancy of those sweeps."""
+    from alf_amd.runtime import Engine
+    eng = Engine(ham, delay=delay)
+    eng.set_fields(fields)
+    eng.set_philox(seed=0, stream=0)
+    eng.obs_setup()
+    eng.init()
+    eng.sweep(2)
+    eng.obs_enable(True)
+    eng.sweep(1)                  # captures the sweep with its measurements
+    eng.obs_reset()
+    ms = eng.sweep_timed(steps)
+    s = eng.obs_get()
+    eng.close()
+    n, u = ham.Ndim, ham.params.Ham_U
+    return {"sweeps_per_s": steps / (ms / 1e3), "measurements_per_sweep": 2 * ham.Ltrot - 1,
+            "energy_per_site": float((s["scal"][3] / s["n_scal"]).real) / n,
+            "double_occupancy": float((s["scal"][1] / s["n_scal"]).real) / (u * n),
+            "note": f"{steps} sweeps after 3 from a random start (not a thermalised estimate)"}
+
+
 def single_rate(ham, fields, delay, steps, warmup, prec):
     """Sweeps/s of one chain of the workload and its Precision Green/Phase statistics."""
     from alf_amd.runtime import Engine
@@ -181,6 +204,7 @@ def main():
         "roofline": kb,
     }
     eng.close()
+    result["with_measurements"] = measured_rate(ham, fields, args.delay, args.steps)
     if args.batch > 0:
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
     # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64

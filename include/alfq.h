@@ -119,6 +119,10 @@ int alfq_rng_set_tape(alfq_ctx* ctx, const double* u, int64_t n);
 /* Counter-based Philox4x32-10 stream (seed, stream id). */
 int alfq_rng_set_philox(alfq_ctx* ctx, uint64_t seed, uint32_t stream);
 int alfq_rng_position(alfq_ctx* ctx, int64_t* pos);
+/* Continue the selected chain's Philox stream at position pos (a restart from a saved
+ * configuration: the seed line of confout, Fields_mod.F90:739-760, holds seed, stream and
+ * position for this engine). */
+int alfq_rng_set_position(alfq_ctx* ctx, int64_t pos);
 
 /* ---- the reference routines, same semantics ------------------------------ */
 /* WRAPGRUP(GR, NTAU, PHASE, ...) (Prog/Wrapgr_mod.F90:81-156), NTAU in [0, Ltrot-1].
