@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/alfq.h"
+#include "alfq_cb.h"
 #include "alfq_chain.h"
 #include "alfq_obs.h"
 #include "alfq_qr.h"
@@ -171,6 +172,14 @@ struct alfq_ctx {
   std::vector<int> obs_key;              // host copy of the shared tables (batch consistency)
   bool obs_on = false;
   int lobs_st = 1, lobs_en = 0;
+  // checkerboard Op_T: bond sites (shared), bond exponentials per chain
+  CbOps cb{};
+  CbOps cb_inv{};
+  bool cb_on = false;
+  char* cb_buf = nullptr;
+  int2* cb_sites = nullptr;
+  std::vector<int> cb_key;
+  std::vector<char> cb_set;
   hipStream_t s = nullptr;
   // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
   int B = 1;
@@ -244,12 +253,12 @@ struct alfq_ctx {
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
     int var_r, var_l, rounds, K, rng_mode, prec;
-    bool obs_on;
+    bool obs_on, cb_on;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
-             K == o.K && rng_mode == o.rng_mode && prec == o.prec && obs_on == o.obs_on && var_st == o.var_st &&
-             refs == o.refs;
+             K == o.K && rng_mode == o.rng_mode && prec == o.prec && obs_on == o.obs_on && cb_on == o.cb_on &&
+             var_st == o.var_st && refs == o.refs;
     }
   };
   struct SweepGraph {
@@ -413,24 +422,43 @@ int adapt_rounds(alfq_ctx* c) {
   return ST_NONE;
 }
 
+// a batch whose checkerboard operators are set for some chains only cannot propagate
+int hop_ready(const alfq_ctx* c) {
+  if (!c->cb_set.empty() && !c->cb_on) {
+    alfq_set_error("checkerboard operators are set for some chains of the batch only");
+    return ST_HAMILTONIAN;
+  }
+  return ST_NONE;
+}
+
 int wrapgrup_dev(alfq_ctx* c, int ntau) {
   int st;
+  if ((st = hop_ready(c))) return st;
   const int slice = ntau;   // fields column of time slice NTAU1 = NTAU + 1
   if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
                         c->dv_dconj, c->B, c->cs, c->s)))
     return st;
-  // H = exp(-dtau T) G
-  GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
-                          ONE, ZERO);
-  g1.prec = c->prec;
-  if ((st = gemm_c(c, g1))) return st;
-  // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
-  GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
-                          c->n, ONE, ZERO);
-  g2.rs = c->dv_d;
-  g2.cs = c->dv_dinv;
-  g2.prec = c->prec;
-  if ((st = gemm_c(c, g2))) return st;
+  if (c->cb_on) {
+    // H = E_1 ... E_N G (Hop_mod_mmthr, bonds Ncheck..1); G = D (H E_N^-1 ... E_1^-1) D^-1
+    // (Hop_mod_mmthl_m1, bonds Ncheck..1, then Op_Wrapup type 1 with the old fields)
+    if ((st = cb_left(c->H, c->G, c->n, nullptr, nullptr, c->cb, 1, c->B, c->cs, c->s))) return st;
+    if ((st = cb_right(c->Gtmp, c->H, c->n, nullptr, nullptr, c->dv_d, c->dv_dinv, c->cb_inv, 1, c->B, c->cs,
+                       c->s)))
+      return st;
+  } else {
+    // H = exp(-dtau T) G
+    GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
+                            ONE, ZERO);
+    g1.prec = c->prec;
+    if ((st = gemm_c(c, g1))) return st;
+    // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
+    GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
+                            c->n, ONE, ZERO);
+    g2.rs = c->dv_d;
+    g2.cs = c->dv_dinv;
+    g2.prec = c->prec;
+    if ((st = gemm_c(c, g2))) return st;
+  }
   std::swap(c->G, c->Gtmp);
   if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   return metropolis_slice(c, slice, +1);
@@ -438,24 +466,33 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
 
 int wrapgrdo_dev(alfq_ctx* c, int ntau) {
   int st;
+  if ((st = hop_ready(c))) return st;
   const int slice = ntau - 1;
   if (!c->uniforms_ready && (st = fill_uniforms(c, c->per_visit_sum))) return st;
   if ((st = metropolis_slice(c, slice, -1))) return st;
   if ((st = launch_dvec(c->f, c->n_op, slice, 1, c->n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
                         c->dv_dconj, c->B, c->cs, c->s)))
     return st;
-  // H = D^-1 G D exp(-dtau T)   (Op_Wrapdo type 1 with the new fields, then Hop_mod_mmthl)
-  GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->G, c->n, OP_N, true, c->expT, c->n, OP_N, !c->t_real, c->H, c->n,
-                          ONE, ZERO);
-  g1.rs = c->dv_dinv;
-  g1.bs = c->dv_d;
-  g1.prec = c->prec;
-  if ((st = gemm_c(c, g1))) return st;
-  // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
-  GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
-                          c->n, ONE, ZERO);
-  g2.prec = c->prec;
-  if ((st = gemm_c(c, g2))) return st;
+  if (c->cb_on) {
+    // H = (D^-1 G D) E_1 ... E_N (Op_Wrapdo, then Hop_mod_mmthl, bonds 1..Ncheck);
+    // G = E_N^-1 ... E_1^-1 H (Hop_mod_mmthr_m1, bonds 1..Ncheck)
+    if ((st = cb_right(c->H, c->G, c->n, c->dv_dinv, c->dv_d, nullptr, nullptr, c->cb, 0, c->B, c->cs, c->s)))
+      return st;
+    if ((st = cb_left(c->Gtmp, c->H, c->n, nullptr, nullptr, c->cb_inv, 0, c->B, c->cs, c->s))) return st;
+  } else {
+    // H = D^-1 G D exp(-dtau T)   (Op_Wrapdo type 1 with the new fields, then Hop_mod_mmthl)
+    GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->G, c->n, OP_N, true, c->expT, c->n, OP_N, !c->t_real, c->H, c->n,
+                            ONE, ZERO);
+    g1.rs = c->dv_dinv;
+    g1.bs = c->dv_d;
+    g1.prec = c->prec;
+    if ((st = gemm_c(c, g1))) return st;
+    // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
+    GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
+                            c->n, ONE, ZERO);
+    g2.prec = c->prec;
+    if ((st = gemm_c(c, g2))) return st;
+  }
   std::swap(c->G, c->Gtmp);
   return ST_NONE;
 }
@@ -467,6 +504,7 @@ int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
   const int ns = nt_hi - nt_lo;          // slices nt_lo+1 .. nt_hi (1-based)
   if (ns < 0) { alfq_set_error("wrap: NTAU1 < NTAU"); return ST_GENERIC; }
   if (ns > c->W) { alfq_set_error("wrap: more than Nwrap slices"); return ST_GENERIC; }
+  if ((st = hop_ready(c))) return st;
   UDVSlot src = c->pool[*var];
   const cplx* cur = src.U;
   if (ns > 0) {
@@ -480,10 +518,17 @@ int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
     // right chain: slices ascending, U <- D exp(-dtau T) U (Hop_mod_mmthR, Op_mmultR 'n')
     // left chain:  slices descending, U <- exp(-dtau T) conj(D) U (Op_mmultR 'c', Hop_mod_mmthlc)
     const int sl = left ? (ns - 1 - q) : q;
-    GemmArgs g = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, cur, n, OP_N, true, bufs[which], n, ONE, ZERO);
-    if (left) g.bs = c->dv_dconj + (long long)sl * n;
-    else g.rs = c->dv_d + (long long)sl * n;
-    if ((st = gemm_c(c, g))) return st;
+    if (c->cb_on) {
+      // right: D E_1 ... E_N U (bonds Ncheck..1); left: E_N ... E_1 conj(D) U (Hop_mod_mmthlc)
+      st = left ? cb_left(bufs[which], cur, n, c->dv_dconj + (long long)sl * n, nullptr, c->cb, 0, c->B, c->cs, c->s)
+                : cb_left(bufs[which], cur, n, nullptr, c->dv_d + (long long)sl * n, c->cb, 1, c->B, c->cs, c->s);
+      if (st) return st;
+    } else {
+      GemmArgs g = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, cur, n, OP_N, true, bufs[which], n, ONE, ZERO);
+      if (left) g.bs = c->dv_dconj + (long long)sl * n;
+      else g.rs = c->dv_d + (long long)sl * n;
+      if ((st = gemm_c(c, g))) return st;
+    }
     cur = bufs[which];
     which ^= 1;
   }
@@ -644,6 +689,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
   h.prec = c->prec;
   h.obs_on = c->obs_on;
+  h.cb_on = c->cb_on;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -1065,6 +1111,8 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   drop_graphs(c);
   if (c->obs.buf) hipFree(c->obs.buf);
   if (c->obs_tables) hipFree(c->obs_tables);
+  if (c->cb_buf) hipFree(c->cb_buf);
+  if (c->cb_sites) hipFree(c->cb_sites);
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -1173,6 +1221,91 @@ int alfq_obs_setup(alfq_ctx* c, const alfq_obs_desc* d) {
   if (nnz > 0) HIP_OK(hipMemcpy(blk + c->obs.off_ov, d->o_val, (size_t)nnz * sizeof(cplx), hipMemcpyHostToDevice));
   c->lobs_st = d->lobs_st;
   c->lobs_en = d->lobs_en;
+  return ST_NONE;
+}
+
+int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
+  NULL_STREAM_LOCK;
+  if (!c || !d || !d->fam_len || !d->p || !d->expo || !d->expo_inv) {
+    alfq_set_error("set_checkerboard: null argument");
+    return ST_GENERIC;
+  }
+  const int n = c->n, nb = d->nbond, nf = d->nfam;
+  if (nb < 1 || nf < 1 || nf > CB_MAXFAM) {
+    alfq_set_error("set_checkerboard: need 1..16 families and at least one bond");
+    return ST_HAMILTONIAN;
+  }
+  if ((long long)n * (long long)sizeof(cplx) > 65536) {
+    alfq_set_error("set_checkerboard: Ndim above 4096");
+    return ST_HAMILTONIAN;
+  }
+  // key: families, then the bond sites (0-based); a family's bonds must sit on disjoint sites
+  std::vector<int> key;
+  key.reserve(1 + nf + 2 * (size_t)nb);
+  key.push_back(nf);
+  CbFams fam{};
+  fam.nfam = nf;
+  int tot = 0;
+  for (int f = 0; f < nf; ++f) {
+    if (d->fam_len[f] < 0) { alfq_set_error("set_checkerboard: negative family length"); return ST_HAMILTONIAN; }
+    fam.off[f] = tot;
+    tot += d->fam_len[f];
+    key.push_back(d->fam_len[f]);
+  }
+  fam.off[nf] = tot;
+  if (tot != nb) { alfq_set_error("set_checkerboard: family lengths do not add up to nbond"); return ST_HAMILTONIAN; }
+  std::vector<int> mark(n, -1);
+  for (int f = 0; f < nf; ++f)
+    for (int b = fam.off[f]; b < fam.off[f + 1]; ++b) {
+      const int i = d->p[2 * (long long)b] - 1, j = d->p[2 * (long long)b + 1] - 1;
+      if (i < 0 || i >= n || j < 0 || j >= n || i == j) {
+        alfq_set_error("set_checkerboard: bond sites outside 1..Ndim or equal");
+        return ST_HAMILTONIAN;
+      }
+      if (mark[i] == f || mark[j] == f) {
+        alfq_set_error("set_checkerboard: non-commuting bonds in one family (shared site)");
+        return ST_HAMILTONIAN;
+      }
+      mark[i] = mark[j] = f;
+      key.push_back(i);
+      key.push_back(j);
+    }
+  const long long per = 2LL * 4 * nb * (long long)sizeof(cplx);
+  const long long mstride = (per + 255) & ~255LL;
+  if (!c->cb_buf) {
+    char* buf = nullptr;
+    int2* sites = nullptr;
+    if (hipMalloc(&buf, (size_t)mstride * c->B) != hipSuccess) { alfq_set_error("set_checkerboard: hipMalloc"); return ST_GENERIC; }
+    if (hipMalloc(&sites, (size_t)nb * sizeof(int2)) != hipSuccess) {
+      hipFree(buf);
+      alfq_set_error("set_checkerboard: hipMalloc");
+      return ST_GENERIC;
+    }
+    c->cb_buf = buf;
+    c->cb_sites = sites;
+    HIP_OK(hipMemset(c->cb_buf, 0, (size_t)mstride * c->B));
+    HIP_OK(hipMemcpy(c->cb_sites, key.data() + 1 + nf, (size_t)nb * sizeof(int2), hipMemcpyHostToDevice));
+    c->cb_key = key;
+    c->cb_set.assign(c->B, 0);
+    CbOps o{};
+    o.sites = c->cb_sites;
+    o.mats = (const cplx*)c->cb_buf;
+    o.mstride = mstride;
+    o.fam = fam;
+    o.nbond = nb;
+    c->cb = o;
+    o.mats = (const cplx*)(c->cb_buf + 4LL * nb * (long long)sizeof(cplx));
+    c->cb_inv = o;
+  } else if (key != c->cb_key) {
+    alfq_set_error("set_checkerboard: the bonds differ from the batch's");
+    return ST_HAMILTONIAN;
+  }
+  char* blk = c->cb_buf + (long long)c->cur * mstride;
+  const size_t mb = 4 * (size_t)nb * sizeof(cplx);
+  HIP_OK(hipMemcpy(blk, d->expo, mb, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(blk + mb, d->expo_inv, mb, hipMemcpyHostToDevice));
+  c->cb_set[c->cur] = 1;
+  c->cb_on = std::all_of(c->cb_set.begin(), c->cb_set.end(), [](char v) { return v != 0; });
   return ST_NONE;
 }
 

@@ -99,3 +99,117 @@ def dense_hopping_matrix(hop: HoppingMatrix, Invlist, latt, unit) -> np.ndarray:
             I1 = int(Invlist[I, no_i])
             O[I1, I1] = hop.T_Loc[no_i]
     return O
+
+
+@dataclass
+class CheckerboardOpT:
+    """Checkerboard Op_T (Predefined_Hop_mod.F90:1773-1834, Symm = .F.): 2 x 2 bond
+    operators in the reference's order, family after family.  ``E[b]`` = exp(g O_b) and
+    ``Einv[b]`` = exp(-g O_b) with g = -dtau, symmetrised as RealExpOpT/CmplxExpOpT do;
+    on sites ``p1[b], p2[b]``.  The propagation exp(-dtau T) is P = E_1 E_2 ... E_N
+    (Hop_mod_mmthr applies E_N first, Prog/Hop_mod.F90:143-160)."""
+    p1: np.ndarray
+    p2: np.ndarray
+    O: np.ndarray             # (N, 2, 2) bond matrices
+    E: np.ndarray             # (N, 2, 2)
+    Einv: np.ndarray          # (N, 2, 2)
+    fam_len: list
+    is_real: bool
+
+    def dense(self, n: int, which: str = "P") -> np.ndarray:
+        """P = E_1 ... E_N, Pinv = E_N^-1 ... E_1^-1 or Pc = P^+ = E_N ... E_1 as n x n."""
+        M = np.eye(n, dtype=complex)
+        N = len(self.p1)
+        if which == "P":
+            order, mats = range(N - 1, -1, -1), self.E
+        elif which == "Pinv":
+            order, mats = range(N), self.Einv
+        else:
+            order, mats = range(N), self.E
+        for b in order:
+            r = [int(self.p1[b]), int(self.p2[b])]
+            M[r, :] = mats[b] @ M[r, :]
+        return M
+
+
+def checkerboard_families(hop: HoppingMatrix, latt, lattice_type: str):
+    """List_Fam of the square (4 families: even/odd cells x the two bonds; commuting for even L1, L2)
+    and honeycomb (3 families: bond nf of every cell) defaults
+    (Predefined_Hop_mod.F90:326-355, 893-903): lists of (cell, bond)."""
+    t = lattice_type.upper()
+    if t == "SQUARE":
+        fams = [[], [], [], []]
+        for I in range(latt.N):
+            even = (int(latt.list[I, 0]) + int(latt.list[I, 1])) % 2 == 0
+            base = 0 if even else 2
+            fams[base].append((I, 0))
+            fams[base + 1].append((I, 1))
+        return fams
+    if t == "HONEYCOMB":
+        return [[(I, nf) for I in range(latt.N)] for nf in range(3)]
+    raise NotImplementedError(lattice_type)
+
+
+def _bond_sites(hop: HoppingMatrix, Invlist, latt, I: int, nb: int):
+    no_i, no_j, n_1, n_2 = (int(v) for v in hop.List[nb])
+    return int(Invlist[I, no_i]), int(Invlist[int(latt.nnlist[I, n_1 + 1, n_2 + 1]), no_j])
+
+
+def test_checkerboard_decomposition(hop: HoppingMatrix, fams, Invlist, latt) -> list:
+    """test_checkerboard_decomposition (Predefined_Hop_mod.F90:171-246): every bond in
+    exactly one family and the bonds of a family on disjoint sites.  Returns the errors."""
+    errs = []
+    seen = np.zeros((latt.N, hop.N_bonds), dtype=bool)
+    for f, fam in enumerate(fams):
+        for I, nb in fam:
+            if seen[I, nb]:
+                errs.append(f"bond ({I}, {nb}) is present twice")
+            seen[I, nb] = True
+    for I, nb in zip(*np.nonzero(~seen)):
+        errs.append(f"bond ({I}, {nb}) is missing")
+    for f, fam in enumerate(fams):
+        used = {}
+        for I, nb in fam:
+            for s in _bond_sites(hop, Invlist, latt, I, nb):
+                if s in used:
+                    errs.append(f"non-commuting hoppings in family {f}: ({I}, {nb}) and {used[s]} share site {s}")
+                used[s] = (I, nb)
+    return errs
+
+
+def checkerboard_opt(hop: HoppingMatrix, Invlist, latt, unit, lattice_type: str, dtau: float) -> CheckerboardOpT:
+    from .operator import exp_opt_init, op_make, op_set
+    fams = checkerboard_families(hop, latt, lattice_type)
+    errs = test_checkerboard_decomposition(hop, fams, Invlist, latt)
+    if errs:
+        raise ValueError("inconsistent checkerboard decomposition: " + "; ".join(errs[:4]))
+    mult = np.zeros(unit.Norb)
+    for nb in range(hop.N_bonds):
+        mult[int(hop.List[nb, 0])] += 1
+        mult[int(hop.List[nb, 1])] += 1
+    p1, p2, Os, Es, Eis, fam_len = [], [], [], [], [], []
+    real = True
+    for fam in fams:
+        fam_len.append(len(fam))
+        for I, nb in fam:
+            no_i, no_j, n_1, n_2 = (int(v) for v in hop.List[nb])
+            J = int(latt.nnlist[I, n_1 + 1, n_2 + 1])
+            I1, J1 = int(Invlist[I, no_i]), int(Invlist[J, no_j])
+            Z = generic_hopping(I, no_i, n_1, n_2, no_j, hop.N_Phi, hop.Phi_X, hop.Phi_Y, hop.Bulk, latt, unit)
+            op = op_make(2)
+            op.P[:] = [I1, J1]
+            op.O[0, 1] = hop.T[nb] * Z
+            op.O[1, 0] = np.conj(hop.T[nb] * Z)
+            op.O[0, 0] = hop.T_Loc[no_i] / mult[no_i]
+            op.O[1, 1] = hop.T_Loc[no_j] / mult[no_j]
+            op.g = -dtau + 0j
+            op.alpha = 0j
+            Os.append(op.O.copy())
+            op_set(op)
+            ex = exp_opt_init(op)
+            real = real and ex.is_real
+            p1.append(I1)
+            p2.append(J1)
+            Es.append(np.array(ex.mat))
+            Eis.append(np.array(ex.invmat))
+    return CheckerboardOpT(np.array(p1), np.array(p2), np.array(Os), np.array(Es), np.array(Eis), fam_len, real)

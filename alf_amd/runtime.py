@@ -36,6 +36,7 @@ EXPORTS = [
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
+    "alfq_set_checkerboard",
 ]
 
 
@@ -54,6 +55,11 @@ class ObsDesc(C.Structure):
                 ("lobs_en", C.c_int32), ("invlist", C.POINTER(C.c_int32)), ("imj", C.POINTER(C.c_int32)),
                 ("o_row", C.POINTER(C.c_int32)), ("o_col", C.POINTER(C.c_int32)),
                 ("o_val", C.POINTER(C.c_double)), ("ham_u", C.c_double)]
+
+
+class CheckerboardDesc(C.Structure):
+    _fields_ = [("nbond", C.c_int32), ("nfam", C.c_int32), ("fam_len", C.POINTER(C.c_int32)),
+                ("p", C.POINTER(C.c_int32)), ("expo", C.POINTER(C.c_double)), ("expo_inv", C.POINTER(C.c_double))]
 
 
 class ObsSums(C.Structure):
@@ -95,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_chain_bytes": [C.POINTER(ModelDesc), C.POINTER(C.c_int64)],
         "alfq_obs_setup": [P, C.POINTER(ObsDesc)],
         "alfq_rng_set_position": [P, C.c_int64],
+        "alfq_set_checkerboard": [P, C.POINTER(CheckerboardDesc)],
         "alfq_obs_enable": [P, C.c_int32],
         "alfq_obs_reset": [P],
         "alfq_obs_get": [P, C.POINTER(ObsSums), dp, dp],
@@ -182,6 +189,21 @@ def model_desc(model):
     return d, keep
 
 
+def checkerboard_desc(model):
+    """alfq_checkerboard_desc of a host model built with Checkerboard = True (None otherwise),
+    with the arrays it points to."""
+    cb = model.extra.get("checkerboard") if getattr(model, "extra", None) else None
+    if cb is None:
+        return None, None
+    keep = dict(fam=np.asarray(cb.fam_len, dtype=np.int32),
+                p=np.ascontiguousarray(np.stack([cb.p1, cb.p2], axis=1) + 1, dtype=np.int32),
+                E=np.ascontiguousarray(np.transpose(np.asarray(cb.E, dtype=np.complex128), (0, 2, 1))),
+                Ei=np.ascontiguousarray(np.transpose(np.asarray(cb.Einv, dtype=np.complex128), (0, 2, 1))))
+    d = CheckerboardDesc(len(cb.p1), len(cb.fam_len), _iptr(keep["fam"]), _iptr(keep["p"]), _dptr(keep["E"]),
+                         _dptr(keep["Ei"]))
+    return d, keep
+
+
 def obs_desc(model, lobs=None):
     """alfq_obs_desc of a host Hubbard model (hopping matrix, lattice tables, U) with the
     arrays it points to; lobs = (LOBS_ST, LOBS_EN), default every slice 1..Ltrot."""
@@ -216,8 +238,17 @@ class Engine:
         self.ctx = h
         self.n = model.Ndim
         self.nchain = 1
+        self.set_checkerboard(model)
         if delay is not None:
             self.set_delay(delay)
+
+    def set_checkerboard(self, model=None):
+        """Hand the bond operators of a checkerboard model to the selected chain (no-op for
+        a model with the dense hopping exponential)."""
+        d, keep = checkerboard_desc(model if model is not None else self.model)
+        if d is not None:
+            self._c(self.lib.alfq_set_checkerboard(self.ctx, C.byref(d)))
+        del keep
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -441,6 +472,10 @@ class BatchEngine(Engine):
         self.ctx = h
         self.n = self.model.Ndim
         self.nchain = len(descs)
+        for z, m in enumerate(self.models):
+            self.chain(z)
+            self.set_checkerboard(m)
+        self.chain(0)
         if delay is not None:
             self.set_delay(delay)
 

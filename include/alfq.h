@@ -220,6 +220,28 @@ int alfq_obs_reset(alfq_ctx* ctx);
 /* The selected chain's sums: latt[((q ncell + d) norb + a) norb + b] complex (re, im) with
  * q = 0 Green, 1 SpinZ, 2 Den and d the 0-based distance class; den0[norb] complex. */
 int alfq_obs_get(alfq_ctx* ctx, alfq_obs_sums* sums, double* latt, double* den0);
+/* ---- checkerboard Op_T ---------------------------------------------------- */
+/* Replaces the checkerboard branch of Predefined_Hoppings_set_OPT
+ * (Prog/Predefined_Hop_mod.F90:1773-1834) as Hop_mod applies it (Prog/Hop_mod.F90:143-250):
+ * Op_T(1:nbond) are 2 x 2 bond operators, grouped into nfam families of consecutive bonds
+ * whose sites are disjoint (test_checkerboard_decomposition, Predefined_Hop_mod.F90:171-246).
+ * After this call the wraps of G and of the UDV chains apply P = E_1 E_2 ... E_N in the
+ * reference's bond order (mmthr: E_N first; mmthl: E_1 first; mmthlc: E_N ... E_1) instead
+ * of the dense expT of the model description.  The sites and families are shared by the
+ * chains of a batch; the matrices are set for the selected chain, and every chain of a
+ * batch needs them before it is propagated. */
+typedef struct {
+  int32_t nbond;            /* size(Op_T, 1) */
+  int32_t nfam;             /* families, 1..16 */
+  const int32_t* fam_len;   /* bonds per family, in order; they add up to nbond */
+  const int32_t* p;         /* 2 * nbond: Op_T(b)%P(1), Op_T(b)%P(2), 1-based sites */
+  const double* expo;       /* nbond 2 x 2 complex, column-major: exp(g O_b) as applied
+                               (the upper triangle mirrored, OpTTypes_mod.F90:92-133, 203-234) */
+  const double* expo_inv;   /* exp(-g O_b), same layout */
+} alfq_checkerboard_desc;
+
+int alfq_set_checkerboard(alfq_ctx* ctx, const alfq_checkerboard_desc* desc);
+
 /* Library build identification. */
 const char* alfq_version(void);
 

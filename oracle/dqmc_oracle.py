@@ -12,7 +12,8 @@ It follows the reference's call order and arithmetic (LAPACK via scipy):
   Op_Wrapup/Op_Wrapdo ...... Prog/Operator_mod.F90:743-835, 862-951 (diagonal branch)
   Op_mmultR ................ Prog/Operator_mod.F90:648-717 (diagonal branch)
   Op_phase ................. Prog/Operator_mod.F90:160-181
-  Hop_mod_* ................ Prog/Hop_mod.F90:143-250 with OpTTypes_mod.F90:150-304
+  Hop_mod_* ................ Prog/Hop_mod.F90:143-250 with OpTTypes_mod.F90:150-304 (dense
+                             Op_T, or the checkerboard's 2 x 2 bond operators one by one)
   WRAPUR / WRAPUL .......... Prog/wrapur_mod.F90:101-126, Prog/wrapul_mod.F90:108-132
   decompose ................ Prog/udv_state_mod.F90:448-582 (default and STABLOG-style
                              norm-presorted pivoting)
@@ -78,10 +79,54 @@ class OracleModel:
     op_g: np.ndarray
     op_alpha: np.ndarray
     op_type: np.ndarray
+    # checkerboard Op_T: bond sites, exp(g O_b), exp(-g O_b) (2 x 2), bonds per family
+    cb_p: np.ndarray = None
+    cb_E: np.ndarray = None
+    cb_Einv: np.ndarray = None
 
     @property
     def n_op(self):
         return len(self.op_P)
+
+    def _bonds(self, M, mats, order, left):
+        for b in order:
+            r = [int(self.cb_p[b, 0]), int(self.cb_p[b, 1])]
+            if left:
+                M[r, :] = mats[b] @ M[r, :]          # ZDSLSYMM/ZSLHEMM('L', 'U', 2, ...)
+            else:
+                M[:, r] = M[:, r] @ mats[b]          # ZDSLSYMM/ZSLHEMM('R', 'U', 2, ...)
+        return M
+
+    def mmthr(self, M):
+        """Hop_mod_mmthr (Hop_mod.F90:143-160): exp(-dtau T) M, bonds Ncheck..1."""
+        if self.cb_p is None:
+            return self.mat @ M
+        return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p) - 1, -1, -1), True)
+
+    def mmthr_m1(self, M):
+        """Hop_mod_mmthr_m1 (Hop_mod.F90:162-181): exp(dtau T) M, bonds 1..Ncheck."""
+        if self.cb_p is None:
+            return self.invmat @ M
+        return self._bonds(M.copy(), self.cb_Einv, range(len(self.cb_p)), True)
+
+    def mmthl(self, M):
+        """Hop_mod_mmthl (Hop_mod.F90:185-204): M exp(-dtau T), bonds 1..Ncheck."""
+        if self.cb_p is None:
+            return M @ self.mat
+        return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p)), False)
+
+    def mmthl_m1(self, M):
+        """Hop_mod_mmthl_m1 (Hop_mod.F90:231-250): M exp(dtau T), bonds Ncheck..1."""
+        if self.cb_p is None:
+            return M @ self.invmat
+        return self._bonds(M.copy(), self.cb_Einv, range(len(self.cb_p) - 1, -1, -1), False)
+
+    def mmthlc(self, M):
+        """Hop_mod_mmthlc (Hop_mod.F90:208-227): lmult with bonds 1..Ncheck, i.e.
+        exp(-dtau T)^+ M (for the dense Hermitian exponential the same as mmthr)."""
+        if self.cb_p is None:
+            return self.mat @ M
+        return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p)), True)
 
     def e_exp(self, n, sp):
         """E_exp(1, sp) of operator n (Operator_mod.F90:436-466)."""
@@ -434,8 +479,8 @@ class OracleSweep:
         """WRAPGRUP (Wrapgr_mod.F90:81-156); ntau in [0, L-1], updates slice ntau+1."""
         m = self.m
         nt = ntau          # 0-based column of slice ntau1 = ntau+1
-        G = m.mat @ G
-        G = G @ m.invmat
+        G = m.mmthr(G)
+        G = m.mmthl_m1(G)
         for n in range(m.n_op):
             P = int(m.op_P[n])
             sp = int(self.f[n, nt])
@@ -458,8 +503,8 @@ class OracleSweep:
             sp = int(self.f[n, nt])
             G[P, :] *= m.e_exp(n, -sp)
             G[:, P] *= m.e_exp(n, sp)
-        G = G @ m.mat
-        G = m.invmat @ G
+        G = m.mmthl(G)
+        G = m.mmthr_m1(G)
         return G
 
     def wrapur(self, ntau, ntau1, udv: UDV):
@@ -467,7 +512,7 @@ class OracleSweep:
         m = self.m
         U = udv.U.copy()
         for nt in range(ntau + 1, ntau1 + 1):
-            U = m.mat @ U
+            U = m.mmthr(U)
             for n in range(m.n_op):
                 U[int(m.op_P[n]), :] *= m.e_exp(n, int(self.f[n, nt - 1]))
         return decompose(UDV(U, udv.D, udv.V, "R", udv.detU), self.presort)
@@ -479,7 +524,7 @@ class OracleSweep:
         for nt in range(ntau1, ntau, -1):
             for n in range(m.n_op - 1, -1, -1):
                 U[int(m.op_P[n]), :] *= np.conj(m.e_exp(n, int(self.f[n, nt - 1])))
-            U = m.mat @ U
+            U = m.mmthlc(U)
         return decompose(UDV(U, udv.D, udv.V, "L", udv.detU), self.presort)
 
     def _stabilise(self, nvar, ntau1):
@@ -558,7 +603,10 @@ class OracleSweep:
 def model_from_ham(ham) -> OracleModel:
     """Plain arrays from a host plugin object (alf_amd.host.hamiltonian.HamBase)."""
     ops = ham.Op_V
-    return OracleModel(
+    cb = ham.extra.get("checkerboard")
+    cbkw = {} if cb is None else dict(cb_p=np.stack([cb.p1, cb.p2], axis=1), cb_E=np.array(cb.E),
+                                      cb_Einv=np.array(cb.Einv))
+    return OracleModel(**cbkw,
         ndim=ham.Ndim, ltrot=ham.Ltrot, nwrap=ham.Nwrap, n_sun=ham.N_SUN,
         mat=np.array(ham.Op_T.mat), invmat=np.array(ham.Op_T.invmat),
         op_P=np.array([int(o.P[0]) for o in ops]), op_E=np.array([float(o.E[0]) for o in ops]),

@@ -5,9 +5,10 @@ from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
 from oracle.dqmc_oracle import OracleSweep, Tape, model_from_ham
 
 
-def hubbard(L1=4, L2=4, beta=1.0, U=4.0, nwrap=5, lattice="Square", n_phi=0, dtau=0.1, chem=0.0):
+def hubbard(L1=4, L2=4, beta=1.0, U=4.0, nwrap=5, lattice="Square", n_phi=0, dtau=0.1, chem=0.0,
+            checkerboard=False):
     p = HubbardParams(Lattice_type=lattice, L1=L1, L2=L2, Beta=beta, Ham_U=U, Nwrap=nwrap, N_Phi=n_phi,
-                      Dtau=dtau, Ham_chem=chem)
+                      Dtau=dtau, Ham_chem=chem, Checkerboard=checkerboard)
     return hubbard_ham_set(p)
 
 
