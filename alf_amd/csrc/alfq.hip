@@ -180,6 +180,7 @@ struct alfq_ctx {
   int2* cb_sites = nullptr;
   std::vector<int> cb_key;
   std::vector<char> cb_set;
+  std::vector<char> cb_real;
   hipStream_t s = nullptr;
   // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
   int B = 1;
@@ -1235,8 +1236,8 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
     alfq_set_error("set_checkerboard: need 1..16 families and at least one bond");
     return ST_HAMILTONIAN;
   }
-  if ((long long)n * (long long)sizeof(cplx) > 65536) {
-    alfq_set_error("set_checkerboard: Ndim above 4096");
+  if (n > 1024) {
+    alfq_set_error("set_checkerboard: Ndim above 1024");
     return ST_HAMILTONIAN;
   }
   // key: families, then the bond sites (0-based); a family's bonds must sit on disjoint sites
@@ -1304,7 +1305,15 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
   const size_t mb = 4 * (size_t)nb * sizeof(cplx);
   HIP_OK(hipMemcpy(blk, d->expo, mb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(blk + mb, d->expo_inv, mb, hipMemcpyHostToDevice));
+  // real arithmetic (RealExpOpT) while every chain's matrices are real
+  bool real = true;
+  for (size_t i = 0; i < 4 * (size_t)nb && real; ++i) real = d->expo[2 * i + 1] == 0.0 && d->expo_inv[2 * i + 1] == 0.0;
+  if (c->cb_real.empty()) c->cb_real.assign(c->B, 1);
+  c->cb_real[c->cur] = real;
+  const bool all_real = std::all_of(c->cb_real.begin(), c->cb_real.end(), [](char v) { return v != 0; });
+  c->cb.is_real = c->cb_inv.is_real = all_real;
   c->cb_set[c->cur] = 1;
+  drop_graphs(c);                        // captured sweeps may hold the other arithmetic
   c->cb_on = std::all_of(c->cb_set.begin(), c->cb_set.end(), [](char v) { return v != 0; });
   return ST_NONE;
 }

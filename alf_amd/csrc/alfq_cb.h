@@ -23,6 +23,7 @@ struct CbOps {
   long long mstride;
   CbFams fam;
   int nbond;
+  bool is_real;             // every matrix of every chain has zero imaginary parts
 };
 
 // out = diag(post_rs) * Prod * diag(pre_rs) * in, where Prod applies the families in

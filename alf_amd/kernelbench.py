@@ -17,6 +17,11 @@ of known size (``python3 -m alf_amd.kernelbench copy``).  The summary of those r
 is committed beside the other profiles and read here.
 
     python3 -m alf_amd.kernelbench copy [--mib N]   # the calibration copy
+
+The ``sweep`` mode runs sweeps of the 16 x 16 U = 4 beta = 10 Hubbard workload, one chain
+and then a batch of chains, for per-kernel profiles of either hopping form:
+
+    python3 -m alf_amd.kernelbench sweep [--checkerboard] [--batch B] [--steps K]
 """
 from __future__ import annotations
 
@@ -93,9 +98,38 @@ def copy_calibration(nbytes: int, repeats: int = 3) -> None:
     print(f"copied {nbytes} bytes device to device x {repeats}", flush=True)
 
 
+def profile_sweeps(checkerboard: bool, batch: int, steps: int) -> None:
+    """`steps` sweeps of one chain, then of `batch` chains in one batched context."""
+    from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+    from alf_amd.runtime import BatchEngine, Engine
+    ham = hubbard_ham_set(HubbardParams(Lattice_type="Square", L1=16, L2=16, Ham_U=4.0, Beta=10.0, Dtau=0.1,
+                                        Nwrap=10, Checkerboard=checkerboard))
+    ham.nsigma.set_uniform_four_state(0)
+    eng = Engine(ham, delay=32)
+    eng.set_fields(ham.nsigma.f.copy())
+    eng.set_philox(seed=0, stream=0)
+    eng.init()
+    ms = eng.sweep_timed(steps)
+    eng.close()
+    print(f"single chain: {steps} sweeps, {ms / steps:.2f} ms per sweep", flush=True)
+    if batch > 0:
+        b = BatchEngine([ham] * batch, delay=32)
+        for z in range(batch):
+            ham.nsigma.set_uniform_four_state(z)
+            b.chain(z).set_fields(ham.nsigma.f.copy())
+            b.chain(z).set_philox(seed=z, stream=0)
+        b.init()
+        ms = b.sweep_timed(steps)
+        b.close()
+        print(f"batch of {batch}: {steps} sweeps, {ms / steps:.2f} ms per batched sweep", flush=True)
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "copy":
         mib = int(sys.argv[sys.argv.index("--mib") + 1]) if "--mib" in sys.argv else 256
         copy_calibration(mib << 20)
+    elif len(sys.argv) >= 2 and sys.argv[1] == "sweep":
+        arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
+        profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3))
     else:
         print(__doc__)

@@ -53,11 +53,11 @@ def cpu_baseline(ham, fields, seconds_budget=30.0):
                       f"(ZGEQP3 pivoting), one thread, {dt:.1f} s"}
 
 
-def batched_rate(chains, steps, prec=0):
+def batched_rate(chains, steps, prec=0, checkerboard=False):
     """Aggregate sweeps/s of `chains` independent chains of the workload in one batched context."""
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import BatchEngine
-    ham = hubbard_ham_set(HubbardParams(**WORKLOAD))
+    ham = hubbard_ham_set(HubbardParams(**WORKLOAD, Checkerboard=checkerboard))
     b = BatchEngine([ham] * chains, delay=32)
     b.set_precision(prec)
     for z in range(chains):
@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="chains of the batched line (0: skip it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
+    ap.add_argument("--no-checkerboard", action="store_true", help="skip the checkerboard-hopping legs")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--serve-sweeps", type=int, default=10)
     args = ap.parse_args()
@@ -219,6 +220,15 @@ def main():
         mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
                                          mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
         result["mixed"] = mixed
+    if not args.no_checkerboard:
+        # the Hubbard plugin's default hopping: the checkerboard product of bond exponentials
+        cb_ham = hubbard_ham_set(HubbardParams(**WORKLOAD, Checkerboard=True))
+        cb = {"form": "Checkerboard=.T.: 4 families of 2x2 bond exponentials applied from LDS strips "
+                      "(HBM-bound, 2 x 16 n^2 bytes per application) instead of the dense n^3 products",
+              "single": single_rate(cb_ham, fields, args.delay, args.steps, args.warmup, 0)}
+        if args.batch > 0:
+            cb["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), 0, checkerboard=True)
+        result["checkerboard"] = cb
     if args.serve_requests > 0:
         result["serving"] = serving_rate(args.serve_requests, args.serve_sweeps, 50.0)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
