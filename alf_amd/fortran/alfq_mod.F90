@@ -37,6 +37,12 @@ module alfq_mod
     real(c_double) :: ham_u
   end type alfq_obs_desc
 
+  ! checkerboard Op_T: 2 x 2 bond exponentials in families on disjoint sites
+  type, bind(C), public :: alfq_checkerboard_desc
+    integer(c_int32_t) :: nbond, nfam
+    type(c_ptr) :: fam_len, p, expo, expo_inv
+  end type alfq_checkerboard_desc
+
   type, bind(C), public :: alfq_obs_sums
     real(c_double) :: scal(8)
     real(c_double) :: sign_scal
@@ -49,7 +55,8 @@ module alfq_mod
             alfq_green_set, alfq_green_get, alfq_phase_set, alfq_phase_get, alfq_udv_reset, &
             alfq_udv_assign, alfq_rng_set_tape, alfq_rng_set_philox, alfq_wrapgrup, alfq_wrapgrdo, &
             alfq_wrapur, alfq_wrapul, alfq_cgr, alfq_init, alfq_sweep, alfq_sync, alfq_control_get, &
-            alfq_last_error_string, alfq_obs_setup, alfq_obs_enable, alfq_obs_reset, alfq_obs_get
+            alfq_last_error_string, alfq_obs_setup, alfq_obs_enable, alfq_obs_reset, alfq_obs_get, &
+            alfq_set_checkerboard
   public :: alfq_WRAPGRUP_shim, alfq_WRAPGRDO_shim, alfq_WRAPUR_shim, alfq_WRAPUL_shim, alfq_CGR_shim
 
   interface
@@ -71,6 +78,11 @@ module alfq_mod
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: mode
+    end function
+    integer(c_int) function alfq_set_checkerboard(ctx, desc) bind(C, name="alfq_set_checkerboard")
+      import :: c_int, c_ptr, alfq_checkerboard_desc
+      type(c_ptr), value :: ctx
+      type(alfq_checkerboard_desc), intent(in) :: desc
     end function
     integer(c_int) function alfq_obs_setup(ctx, desc) bind(C, name="alfq_obs_setup")
       import :: c_int, c_ptr, alfq_obs_desc
