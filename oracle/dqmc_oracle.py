@@ -19,6 +19,8 @@ It follows the reference's call order and arithmetic (LAPACK via scipy):
                              norm-presorted pivoting)
   QDRP_decompose ........... Prog/QDRP_decompose_mod.F90:60-126
   CGR (STAB3) .............. Prog/cgr1_mod.F90:174-447
+  TAU_M, PROPR, PROPRM1 .... Prog/tau_m_mod.F90:56-248 (Ltau = 1, finite temperature)
+  CGR2_2 (STAB3, QDRP) ..... Prog/cgr2_2_mod.F90:130-194, 196-428
   DET_C .................... Libraries/Modules/mymats_mod.F90:1334-1366
   Control_PrecisionG/P ..... Prog/control_mod.F90:207-320, COMPARE_C mymats_mod.F90:683-704
   RNG consumption .......... 3 uniforms per type-2 field visit (nranf(3), T0 proposal,
@@ -213,6 +215,13 @@ class Control:
     @property
     def precision_green_mean(self):
         return self.XMEANG / self.NCG if self.NCG else 0.0
+
+    def precision_tau(self, A, B):
+        """Control_Precision_tau (control_mod.F90:300-311) with COMPARE_C."""
+        diff = np.abs(A - B)
+        self.NCG_tau = getattr(self, "NCG_tau", 0) + 1
+        self.XMAX_tau = max(getattr(self, "XMAX_tau", 0.0), float(diff.max()))
+        self.XMEAN_tau = getattr(self, "XMEAN_tau", 0.0) + float(diff.sum() / diff.size)
 
 
 def pivot_phase(ipvt):
@@ -430,6 +439,7 @@ class OracleSweep:
         self.G = None
         self.obs = None                 # oracle.observables.HubbardObs, measured when set
         self.lobs = None                # (LOBS_ST, LOBS_EN), default 1..Ltrot
+        self.tau_obs = None             # oracle.observables.HubbardTauObs: TAU_M after each sweep
         self.phase = 1.0 + 0j
         self.trace = []        # (event, ntau, G-copy or None, phase) at stabilisation points
 
@@ -598,6 +608,104 @@ class OracleSweep:
         self.udvr = UDV.identity(m.ndim, "R")
         self._stabilise(1, 0)
         self.udvst[self.nstm] = UDV.identity(m.ndim, "L")
+        if self.tau_obs is not None:                   # Ltau = 1 (main.F90:874-876)
+            self.tau_m()
+
+    def propr(self, A, nt1):
+        """PROPR (tau_m_mod.F90:202-223): A <- B(nt1) A, B = D exp(-dtau T)."""
+        m = self.m
+        A = m.mmthr(A)
+        for n in range(m.n_op):
+            A[int(m.op_P[n]), :] *= m.e_exp(n, int(self.f[n, nt1 - 1]))
+        return A
+
+    def proprm1(self, A, nt1):
+        """PROPRM1 (tau_m_mod.F90:226-248): A <- A B(nt1)^-1 (Hop_mod_mmthl_m1, then
+        Op_mmultL with sign -1 on the columns)."""
+        m = self.m
+        A = m.mmthl_m1(A)
+        for n in range(m.n_op):
+            A[:, int(m.op_P[n])] *= m.e_exp(n, -int(self.f[n, nt1 - 1]))
+        return A
+
+    def tau_m(self, hook=None):
+        """TAU_M (tau_m_mod.F90:56-195), finite temperature, Symm = .F.: G(tau, 0), G(0, tau),
+        G(tau, tau), G(0, 0) for tau = 0..Ltrot, propagated slice by slice and recomputed by
+        CGR2_2 from the right chain and the stored left states at every stabilisation point,
+        measured by ObserT at every tau.  ``hook(nt, GT0, G0T, G00, GTT)`` sees each tau."""
+        m = self.m
+        n, L, s = m.ndim, m.ltrot, self.stab
+        G00, GT0, GTT = self.G.copy(), self.G.copy(), self.G.copy()
+        G0T = -(np.eye(n) - self.G)
+
+        def observe(nt):
+            if hook is not None:
+                hook(nt, GT0, G0T, G00, GTT)
+            if self.tau_obs is not None:
+                self.tau_obs.measure(nt, GT0, G0T, G00, GTT, self.phase)
+        observe(0)
+        udvr = UDV.identity(n, "R")
+        nst = 1
+        for nt in range(L):
+            nt1 = nt + 1
+            GT0 = self.propr(GT0, nt1)
+            G0T = self.proprm1(G0T, nt1)
+            GTT = self.propr(self.proprm1(GTT, nt1), nt1)
+            observe(nt1)
+            if s[nst] == nt1:
+                udvr = self.wrapur(s[nst - 1], nt1, udvr)
+                old_tt, old_t0, old_0t = GTT, GT0, G0T
+                GT0, G00, GTT, G0T = cgr2_2(udvr, self.udvst[nst])
+                self.ctrl.precision_tau(self.G, G00)
+                self.ctrl.precision_tau(old_tt, GTT)
+                self.ctrl.precision_tau(old_t0, GT0)
+                self.ctrl.precision_tau(old_0t, G0T)
+                nst += 1
+
+
+def cgr2_2(udv2: UDV, udv1: UDV):
+    """CGR2_2 (cgr2_2_mod.F90:196-428), the STAB3 branch with QDRP_decompose.
+
+    udv2 is the right state, B(tau, 0) = U D V; udv1 the left state,
+    B(beta, tau)^H = U D V^H.  The 2n x 2n matrix O = [[1, B(beta,tau)], [-B(tau,0), 1]]
+    has the inverse [[G00, G0T], [GT0, GTT]].  O = diag(V1, U2) diag(D1+, D2+) H with the
+    scale-split H = [[D1+^-1 V1^-1, D1- U1^H], [-D2- V2, D2+^-1 U2^H]] (blocks swapped when
+    D1(1) <= D2(1)); with QDRP of H^H, H^H P = Q diag(D3) R~, the inverse is
+    O^-1 = Q conj(D3)^-1 R~^-H P^T diag(D1+^-1 V1^-1, D2+^-1 U2^H) (solve_extended_System,
+    cgr2_2_mod.F90:130-194).  Returns GRT0, GR00, GRTT, GR0T."""
+    n = udv1.U.shape[0]
+    myu2 = udv2.U.conj().T.copy()
+    v1inv = np.linalg.inv(udv1.V)
+    d1m = np.array(udv1.D, dtype=complex)
+    d2m = np.array(udv2.D, dtype=complex)
+    for j in range(n):
+        if udv1.D[j].real > 1.0:
+            d1m[j] = 1.0
+            v1inv[j, :] /= udv1.D[j]
+        if udv2.D[j].real > 1.0:
+            d2m[j] = 1.0
+            myu2[j, :] /= udv2.D[j]
+    H = np.zeros((2 * n, 2 * n), dtype=complex)
+    first = udv1.D[0].real > udv2.D[0].real
+    u1h = d1m[:, None] * udv1.U.conj().T
+    v2 = -d2m[:, None] * udv2.V
+    if first:
+        H[:n, :n], H[n:, n:], H[:n, n:], H[n:, :n] = v1inv, myu2, u1h, v2
+        diag = (v1inv, myu2)
+    else:
+        H[:n, :n], H[n:, n:], H[:n, n:], H[n:, :n] = myu2, v1inv, v2, u1h
+        diag = (myu2, v1inv)
+    qr, D3, ipvt, tau = qdrp_decompose(H.conj().T, pivot=True)
+    hlp = np.zeros((2 * n, 2 * n), dtype=complex)
+    hlp[:n, :n], hlp[n:, n:] = diag
+    hlp = hlp[ipvt - 1, :]                                           # ZLAPMR, forward
+    hlp = sla.solve_triangular(np.triu(qr), hlp, trans="C", lower=False)   # ZTRSM('L','U','C','N')
+    hlp = np.conj(1.0 / D3)[:, None] * hlp
+    out = form_q(qr, tau) @ hlp                                     # ZUNMQR('L','N')
+    a, b, c, d = out[:n, :n], out[:n, n:], out[n:, :n], out[n:, n:]
+    if first:
+        return c, a, d, b          # get_blocks(GR00, GR0T, GRT0, GRTT)
+    return b, d, a, c              # get_blocks(GRTT, GRT0, GR0T, GR00)
 
 
 def model_from_ham(ham) -> OracleModel:
@@ -612,6 +720,46 @@ def model_from_ham(ham) -> OracleModel:
         op_P=np.array([int(o.P[0]) for o in ops]), op_E=np.array([float(o.E[0]) for o in ops]),
         op_g=np.array([o.g for o in ops], dtype=complex), op_alpha=np.array([o.alpha for o in ops], dtype=complex),
         op_type=np.array([o.type for o in ops]))
+
+
+def brute_force_tau(model: OracleModel, fields, ntau, dps=30):
+    """G(tau, 0) = B(tau, 0) G(0), G(0, tau) = -(1 - G(0)) B(tau, 0)^-1, G(tau, tau), G(0, 0)
+    in mpmath, from the products of the same slice matrices as brute_force_green."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    n = model.ndim
+    P = mp.matrix(n, n)
+    for i in range(n):
+        for j in range(n):
+            P[i, j] = mp.mpc(complex(model.mat[i, j]))
+
+    def Bslice(nt):
+        out = mp.matrix(n, n)
+        for i in range(n):
+            for j in range(n):
+                out[i, j] = P[i, j]
+        for k in range(model.n_op):
+            p = int(model.op_P[k])
+            s = int(fields[k, nt - 1])
+            z = mp.exp(mp.mpc(complex(model.op_g[k])) * model.op_E[k] * mp.mpf(phi(abs(s), model.op_type[k])))
+            z = z if s > 0 else 1 / z
+            for j in range(n):
+                out[p, j] = out[p, j] * z
+        return out
+
+    Bt0 = mp.eye(n)
+    for nt in range(1, ntau + 1):
+        Bt0 = Bslice(nt) * Bt0
+    Bbt = mp.eye(n)
+    for nt in range(ntau + 1, model.ltrot + 1):
+        Bbt = Bslice(nt) * Bbt
+    I = mp.eye(n)
+    G00 = (I + Bbt * Bt0) ** -1
+    GT0 = Bt0 * G00
+    GTT = (I + Bt0 * Bbt) ** -1
+    G0T = -(I - G00) * Bt0 ** -1
+    conv = lambda M: np.array([[complex(M[i, j]) for j in range(n)] for i in range(n)])
+    return conv(GT0), conv(G0T), conv(G00), conv(GTT)
 
 
 def brute_force_green(model: OracleModel, fields, ntau, dps=40):

@@ -11,6 +11,8 @@ Follows
   * Prog/observables_mod.F90:355-480, 772-860                bin averages (Print_bin_Latt,
     Print_bin_Vec)
   * Libraries/Modules/lattices_v3_mod.F90:847-875            FT_R_to_K_C
+  * Prog/Hamiltonians/Hamiltonian_Hubbard_smod.F90:826-857   ObserT (Green, SpinZ, Den of
+    tau), Prog/Predefined_Obs_mod.F90:337-446, 533-594
 
 Pinned (tests/test_oracle_pins.py) by free fermions, where every observable has a closed
 form in the eigenvalues of the hopping matrix, and by sum rules at k = 0.
@@ -78,6 +80,63 @@ class HubbardObs:
         return {"scal": self.scal / self.n_scal, "sign_scal": self.sign_scal / self.n_scal,
                 "latt": self.latt / self.n_eq, "den0": self.den0 / (self.n_eq * self.ncell),
                 "sign_eq": self.sign_eq / self.n_eq}
+
+
+class HubbardTauObs:
+    """Time-displaced accumulators of one chain (ObserT of the Hubbard plugin, N_FL = 1,
+    Hamiltonian_Hubbard_smod.F90:826-857): latt[3] = (Green, SpinZ, Den)[tau, d, no_I, no_J]
+    for tau = 0..Ltrot, den0[no], and the count N and sign sum, which grow once per TAU_M
+    call (at tau = 0).
+
+    With G(tau, 0) = GT0, G(0, tau) = G0T, G(0, 0) = G00, G(tau, tau) = GTT and the pair
+    I1 = (cell I, orbital a), J1 = (cell J, orbital b), d = imj(I, J), each term times ZP ZS
+    (Predefined_Obs_mod.F90:337-388, 400-446, 533-594):
+      Green += GT0(I1, J1)
+      SpinZ += -N_SUN G0T(J1, I1) GT0(I1, J1)
+      Den   += Z_I Z_J - N_SUN G0T(J1, I1) GT0(I1, J1),  Z_I = N_SUN (1 - GTT(I1, I1)),
+               Z_J = N_SUN (1 - G00(J1, J1));  Den0(a) += Z_I
+    """
+
+    def __init__(self, ltrot, n_sun, imj, site_cell, site_orb, norb):
+        self.ntau = int(ltrot) + 1
+        self.n_sun = int(n_sun)
+        self.imj = np.asarray(imj)
+        self.cell = np.asarray(site_cell)
+        self.orb = np.asarray(site_orb)
+        self.norb = int(norb)
+        self.ncell = self.imj.shape[0]
+        self.reset()
+
+    def reset(self):
+        self.latt = np.zeros((3, self.ntau, self.ncell, self.norb, self.norb), dtype=complex)
+        self.den0 = np.zeros(self.norb, dtype=complex)
+        self.n = 0
+        self.sign = 0.0
+
+    def measure(self, nt, GT0, G0T, G00, GTT, phase):
+        n = GT0.shape[0]
+        zp = phase / phase.real
+        zs = np.sign(phase.real)
+        w = zp * zs
+        if nt == 0:
+            self.n += 1
+            self.sign += zs
+        dist = self.imj[self.cell[:, None], self.cell[None, :]]
+        oi = np.broadcast_to(self.orb[:, None], (n, n))
+        oj = np.broadcast_to(self.orb[None, :], (n, n))
+        green = np.asarray(GT0)
+        spin = -self.n_sun * np.asarray(G0T).T * green          # G0T(J1, I1) GT0(I1, J1)
+        zi = self.n_sun * (1.0 - np.diag(GTT))
+        zj = self.n_sun * (1.0 - np.diag(G00))
+        den = zi[:, None] * zj[None, :] + spin
+        for q, v in enumerate((green, spin, den)):
+            np.add.at(self.latt[q, nt], (dist, oi, oj), v * w)
+        np.add.at(self.den0, self.orb, zi * w)
+
+    def bins(self):
+        """Print_bin_Latt's averages: correlations / N, the Den background / (N Ns Ntau)."""
+        return {"latt": self.latt / self.n, "den0": self.den0 / (self.n * self.ncell * self.ntau),
+                "sign": self.sign / self.n}
 
 
 def fourier_r_to_k(x_r, r_vecs, k_vecs):
