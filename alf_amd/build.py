@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["alfq.hip", "gemm.hip", "qr.hip", "chain.hip", "rate.hip", "obs.hip", "checkerboard.hip"]
+SOURCES = ["alfq.hip", "gemm.hip", "qr.hip", "chain.hip", "rate.hip", "obs.hip", "checkerboard.hip", "tau.hip"]
 OUT = os.path.join(HERE, "libalfq.so")
 
 HIPCC_FLAGS = [

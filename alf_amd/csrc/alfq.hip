@@ -26,6 +26,7 @@
 #include "alfq_chain.h"
 #include "alfq_obs.h"
 #include "alfq_qr.h"
+#include "alfq_tau.h"
 
 static thread_local std::string g_last_error;
 std::recursive_mutex& null_stream_mutex() {
@@ -181,6 +182,24 @@ struct alfq_ctx {
   std::vector<int> cb_key;
   std::vector<char> cb_set;
   std::vector<char> cb_real;
+  // time-displaced Green's functions and measurements (Ltau = 1): the buffers sit in regions
+  // of cs bytes per chain, so they keep the chain stride of the arena; the sums per chain at
+  // acc + z * acc_cs
+  struct TauDev {
+    char* mem = nullptr;
+    int regions = 0;
+    cplx* b[5] = {};                     // G(tau,0), G(0,tau), G(tau,tau), G(0,0), scratch
+    cplx *Xv = nullptr, *Qv = nullptr, *F1 = nullptr, *F2 = nullptr;
+    cplx *H2 = nullptr, *Q2 = nullptr, *Y2 = nullptr, *O2 = nullptr;
+    cplx *DUPv = nullptr, *DUPvinv = nullptr, *DUP2 = nullptr, *DUP2inv = nullptr, *zs = nullptr;
+    int* flag = nullptr;
+    TauCtl* ctl = nullptr;
+    QRWork w2;                           // QR of the 2n x 2n matrix of CGR2_2
+    char* acc = nullptr;
+    long long acc_cs = 0, off_den0 = 0, off_latt = 0;
+  } tau;
+  bool tau_on = false;
+  int var_tr = -1;                       // TAU_M's right chain
   hipStream_t s = nullptr;
   // batch: B chains in lockstep; chain z's buffers sit at (chain 0's pointer) + z * cs bytes
   int B = 1;
@@ -254,11 +273,12 @@ struct alfq_ctx {
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
     int var_r, var_l, rounds, K, rng_mode, prec;
-    bool obs_on, cb_on;
+    bool obs_on, cb_on, tau_on;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
              K == o.K && rng_mode == o.rng_mode && prec == o.prec && obs_on == o.obs_on && cb_on == o.cb_on &&
+             tau_on == o.tau_on &&
              var_st == o.var_st && refs == o.refs;
     }
   };
@@ -612,6 +632,106 @@ int cgr_dev(alfq_ctx* c, int nvar, int slot_r, int slot_l, int check) {
 
 int sweep_body(alfq_ctx* c);
 
+// CGR2_2 (cgr2_2_mod.F90:196-428) from the right state R = B(tau, 0) and the left state
+// Lf = B(beta, tau)^H into G00, G0T, GT0, GTT, with Control_Precision_tau against the
+// propagated values.  With H the scale-split matrix and H^H P = Q D R~ its QR,
+// O^-1 = Q D^-1 R~^-H P^T diag(F1, F2) (solve_extended_System, cgr2_2_mod.F90:130-194).
+int cgr2_2_dev(alfq_ctx* c, const UDVSlot& R, const UDVSlot& Lf, cplx* G00, cplx* G0T, cplx* GT0, cplx* GTT) {
+  int st;
+  auto& t = c->tau;
+  const int n = c->n, m = 2 * n;
+  // V_L^-1 = P R~^-1 D^-1 Q^H from V_L P = Q D R~ (the equal-time CGR's QR workspace)
+  if ((st = qr_factor_cgr(c->qr, Lf.V, 1, R.sc + 1, Lf.sc + 1, t.Qv, t.DUPv, t.zs, c->s))) return st;
+  hipLaunchKernelGGL(recip_real_kernel, dim3(nblk(n, 256), 1, c->B), dim3(256), 0, c->s, t.DUPv, n, t.DUPvinv,
+                     c->cs);
+  HIP_OK(hipGetLastError());
+  if ((st = tri_inverse_dev(c->qr, c->qr.Rt, c->qr.Rinv, c->s))) return st;
+  GemmArgs gx = gemm_args(n, n, n, c->qr.Rinv, n, OP_N, true, t.Qv, n, OP_C, true, t.Xv, n, ONE, ZERO);
+  gx.bs = t.DUPvinv;
+  if ((st = gemm_c(c, gx))) return st;
+  if ((st = cgr22_build(t.Xv, c->qr.pinv, Lf.U, Lf.D, R.U, R.D, R.V, t.H2, t.F1, t.F2, t.flag, n, c->B, c->cs,
+                        c->s)))
+    return st;
+  if ((st = qr_factor_cgr(t.w2, t.H2, 2, R.sc + 1, Lf.sc + 1, t.Q2, t.DUP2, t.zs, c->s))) return st;
+  hipLaunchKernelGGL(recip_real_kernel, dim3(nblk(m, 256), 1, c->B), dim3(256), 0, c->s, t.DUP2, m, t.DUP2inv,
+                     c->cs);
+  HIP_OK(hipGetLastError());
+  if ((st = tri_inverse_dev(t.w2, t.w2.Rt, t.w2.Rinv, c->s))) return st;
+  // Y = Q D^-1 R~^-H
+  GemmArgs gy = gemm_args(m, m, m, t.Q2, m, OP_N, true, t.w2.Rinv, m, OP_C, true, t.Y2, m, ONE, ZERO);
+  gy.bs = t.DUP2inv;
+  if ((st = gemm_c(c, gy))) return st;
+  // O^-1 = (Y P^T) diag(F1, F2), one column block at a time
+  GemmArgs g1 = gemm_args(m, n, n, t.Y2, m, OP_N, true, t.F1, n, OP_N, true, t.O2, m, ONE, ZERO);
+  g1.amap = t.w2.pinv;
+  if ((st = gemm_c(c, g1))) return st;
+  GemmArgs g2 = gemm_args(m, n, n, t.Y2, m, OP_N, true, t.F2, n, OP_N, true, t.O2 + (long long)m * n, m, ONE, ZERO);
+  g2.amap = t.w2.pinv + n;
+  if ((st = gemm_c(c, g2))) return st;
+  return cgr22_extract(t.O2, t.flag, c->G, G00, G0T, GT0, GTT, t.ctl, n, c->B, c->cs, c->s);
+}
+
+// TAU_M (tau_m_mod.F90:56-195), finite temperature, after a sweep: G(tau, 0), G(0, tau),
+// G(tau, tau), G(0, 0) for tau = 0..Ltrot, measured at every tau.
+int tau_m_dev(alfq_ctx* c) {
+  int st;
+  auto& t = c->tau;
+  const int n = c->n, L = c->L;
+  if (!t.mem || !c->obs.buf) { alfq_set_error("TAU_M: call alfq_tau_enable first"); return ST_GENERIC; }
+  if ((st = hop_ready(c))) return st;
+  cplx *GT0 = t.b[0], *G0T = t.b[1], *GTT = t.b[2], *G00 = t.b[3], *T1 = t.b[4];
+  auto observe = [&](int nt) {
+    return tau_measure(c->obs, GT0, G0T, G00, GTT, c->st, t.acc, t.acc_cs, t.off_den0, t.off_latt, nt, L + 1, c->B,
+                       c->cs, c->s);
+  };
+  if ((st = tau_init(c->G, G00, GT0, GTT, G0T, n, c->B, c->cs, c->s))) return st;
+  if ((st = observe(0))) return st;
+  if ((st = udv_reset_dev(c, &c->var_tr, 'R'))) return st;
+  int nst = 1;
+  for (int nt = 0; nt < L; ++nt) {
+    const int nt1 = nt + 1;
+    if ((st = launch_dvec(c->f, c->n_op, nt, 1, n, c->site_ptr, c->site_ops, c->eexp, c->dv_d, c->dv_dinv,
+                          c->dv_dconj, c->B, c->cs, c->s)))
+      return st;
+    if (c->cb_on) {
+      // PROPR: G(tau,0) <- D P G(tau,0); PROPRM1: G(0,tau) <- G(0,tau) P^-1 D^-1;
+      // G(tau,tau) <- D P G(tau,tau) P^-1 D^-1
+      if ((st = cb_left(T1, GT0, n, nullptr, c->dv_d, c->cb, 1, c->B, c->cs, c->s))) return st;
+      std::swap(GT0, T1);
+      if ((st = cb_right(T1, G0T, n, nullptr, nullptr, nullptr, c->dv_dinv, c->cb_inv, 1, c->B, c->cs, c->s)))
+        return st;
+      std::swap(G0T, T1);
+      if ((st = cb_left(T1, GTT, n, nullptr, nullptr, c->cb, 1, c->B, c->cs, c->s))) return st;
+      if ((st = cb_right(GTT, T1, n, nullptr, nullptr, c->dv_d, c->dv_dinv, c->cb_inv, 1, c->B, c->cs, c->s)))
+        return st;
+    } else {
+      GemmArgs a = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, GT0, n, OP_N, true, T1, n, ONE, ZERO);
+      a.rs = c->dv_d;
+      if ((st = gemm_c(c, a))) return st;
+      std::swap(GT0, T1);
+      GemmArgs b = gemm_args(n, n, n, G0T, n, OP_N, true, c->expTinv, n, OP_N, !c->t_real, T1, n, ONE, ZERO);
+      b.cs = c->dv_dinv;
+      if ((st = gemm_c(c, b))) return st;
+      std::swap(G0T, T1);
+      GemmArgs d1 = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, GTT, n, OP_N, true, T1, n, ONE, ZERO);
+      if ((st = gemm_c(c, d1))) return st;
+      GemmArgs d2 = gemm_args(n, n, n, T1, n, OP_N, true, c->expTinv, n, OP_N, !c->t_real, GTT, n, ONE, ZERO);
+      d2.rs = c->dv_d;
+      d2.cs = c->dv_dinv;
+      if ((st = gemm_c(c, d2))) return st;
+    }
+    if ((st = observe(nt1))) return st;
+    if (nst <= c->nstm && c->stab[nst] == nt1) {
+      if ((st = wrap_udv_dev(c, &c->var_tr, c->stab[nst - 1], nt1, false))) return st;
+      if ((st = cgr2_2_dev(c, c->pool[c->var_tr], c->pool[c->var_st[nst]], G00, G0T, GT0, GTT))) return st;
+      ++nst;
+    }
+  }
+  slot_release(c, c->var_tr);
+  c->var_tr = -1;
+  return ST_NONE;
+}
+
 int init_dev(alfq_ctx* c) {
   int st;
   const auto& s = c->stab;
@@ -681,6 +801,7 @@ int sweep_body(alfq_ctx* c) {
   if ((st = udv_reset_dev(c, &c->var_r, 'R'))) return st;
   if ((st = cgr_dev(c, 1, c->var_r, c->var_l, 1))) return st;
   if ((st = udv_reset_dev(c, &c->var_st[c->nstm], 'L'))) return st;
+  if (c->tau_on && (st = tau_m_dev(c))) return st;        // Ltau = 1 (main.F90:874-876)
   return ST_NONE;
 }
 
@@ -691,6 +812,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   h.prec = c->prec;
   h.obs_on = c->obs_on;
   h.cb_on = c->cb_on;
+  h.tau_on = c->tau_on;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -1113,6 +1235,8 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (c->obs.buf) hipFree(c->obs.buf);
   if (c->obs_tables) hipFree(c->obs_tables);
   if (c->cb_buf) hipFree(c->cb_buf);
+  if (c->tau.mem) hipFree(c->tau.mem);
+  if (c->tau.acc) hipFree(c->tau.acc);
   if (c->cb_sites) hipFree(c->cb_sites);
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape};
   for (void* p : ptrs)
@@ -1318,6 +1442,107 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
   return ST_NONE;
 }
 
+// Buffers of TAU_M: regions of cs bytes per chain (chain stride cs, like the arena) and the
+// sums of the time-displaced measurements.
+int tau_alloc(alfq_ctx* c) {
+  auto& t = c->tau;
+  if (t.mem) return ST_NONE;
+  const int n = c->n, m = 2 * n;
+  const size_t nn = (size_t)n * n, mm = (size_t)m * m;
+  auto place = [&](ArenaLayout& ar) {
+    for (auto& p : t.b) p = ar.take<cplx>(nn);
+    t.Xv = ar.take<cplx>(nn);
+    t.Qv = ar.take<cplx>(nn);
+    t.F1 = ar.take<cplx>(nn);
+    t.F2 = ar.take<cplx>(nn);
+    t.H2 = ar.take<cplx>(mm);
+    t.Q2 = ar.take<cplx>(mm);
+    t.Y2 = ar.take<cplx>(mm);
+    t.O2 = ar.take<cplx>(mm);
+    t.DUPv = ar.take<cplx>(n);
+    t.DUPvinv = ar.take<cplx>(n);
+    t.DUP2 = ar.take<cplx>(m);
+    t.DUP2inv = ar.take<cplx>(m);
+    t.zs = ar.take<cplx>(1);
+    t.flag = ar.take<int>(1);
+    t.ctl = ar.take<TauCtl>(1);
+    qr_work_place(t.w2, m, ar);
+  };
+  ArenaLayout sizing;
+  sizing.cap = (size_t)c->cs;
+  place(sizing);
+  if (sizing.overflow) { alfq_set_error("tau: a 2n x 2n buffer exceeds the chain's arena stride"); return ST_GENERIC; }
+  const int regions = sizing.region + 1;
+  const size_t bytes = (size_t)regions * c->B * (size_t)c->cs;
+  if (hipMalloc(&t.mem, bytes) != hipSuccess) { t.mem = nullptr; alfq_set_error("tau: hipMalloc"); return ST_GENERIC; }
+  HIP_OK(hipMemset(t.mem, 0, bytes));
+  ArenaLayout ar;
+  ar.base = t.mem;
+  ar.cap = (size_t)c->cs;
+  ar.rstride = (long long)c->B * c->cs;
+  place(ar);
+  t.regions = regions;
+  t.w2.nchain = c->B;
+  t.w2.cs = c->cs;
+  // sums: head, Den0[norb], [3][Ltrot + 1][ncell][norb][norb]
+  const int nc = c->obs.ncell, no = c->obs.norb;
+  auto r256 = [](long long b) { return (b + 255) & ~255LL; };
+  t.off_den0 = r256(sizeof(TauHead));
+  t.off_latt = t.off_den0 + r256((long long)no * sizeof(cplx));
+  const long long end = t.off_latt + 3LL * (c->L + 1) * nc * no * no * (long long)sizeof(cplx);
+  t.acc_cs = (end + 4095) & ~4095LL;
+  if (hipMalloc(&t.acc, (size_t)t.acc_cs * c->B) != hipSuccess) {
+    t.acc = nullptr;
+    alfq_set_error("tau: hipMalloc");
+    return ST_GENERIC;
+  }
+  HIP_OK(hipMemset(t.acc, 0, (size_t)t.acc_cs * c->B));
+  return ST_NONE;
+}
+
+int alfq_tau_enable(alfq_ctx* c, int32_t on) {
+  NULL_STREAM_LOCK;
+  if (!c) return ST_GENERIC;
+  if (on) {
+    if (!c->obs.buf) { alfq_set_error("tau_enable: call alfq_obs_setup first"); return ST_GENERIC; }
+    int st;
+    if ((st = tau_alloc(c))) return st;
+  }
+  c->tau_on = on != 0;
+  return ST_NONE;
+}
+
+int alfq_tau_m(alfq_ctx* c) {
+  NULL_STREAM_LOCK;
+  if (!c) return ST_GENERIC;
+  int st;
+  if ((st = tau_m_dev(c))) return st;
+  return check_status(c);
+}
+
+int alfq_tau_get(alfq_ctx* c, alfq_tau_sums* sums, double* latt, double* den0) {
+  NULL_STREAM_LOCK;
+  if (!c || !sums) return ST_GENERIC;
+  auto& t = c->tau;
+  if (!t.acc) { alfq_set_error("tau_get: call alfq_tau_enable first"); return ST_GENERIC; }
+  HIP_OK(hipStreamSynchronize(c->s));
+  const char* blk = t.acc + (long long)c->cur * t.acc_cs;
+  TauHead h;
+  TauCtl k;
+  HIP_OK(hipMemcpy(&h, blk, sizeof(h), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&k, at(c, t.ctl), sizeof(k), hipMemcpyDeviceToHost));
+  sums->sign = h.sign;
+  sums->n = h.n;
+  sums->ncg_tau = k.ncg;
+  sums->xmax_tau = k.xmax;
+  sums->xmean_tau_sum = k.xmean;
+  const int nc = c->obs.ncell, no = c->obs.norb;
+  if (latt)
+    HIP_OK(hipMemcpy(latt, blk + t.off_latt, 3ULL * (c->L + 1) * nc * no * no * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (den0) HIP_OK(hipMemcpy(den0, blk + t.off_den0, (size_t)no * sizeof(cplx), hipMemcpyDeviceToHost));
+  return ST_NONE;
+}
+
 int alfq_obs_enable(alfq_ctx* c, int32_t on) {
   if (!c) return ST_GENERIC;
   if (on && !c->obs.buf) { alfq_set_error("obs_enable: call alfq_obs_setup first"); return ST_GENERIC; }
@@ -1333,6 +1558,7 @@ int alfq_obs_reset(alfq_ctx* c) {
   for (int z = 0; z < c->B; ++z)
     HIP_OK(hipMemsetAsync(c->obs.buf + (long long)z * c->obs_cs + c->obs.off_head, 0,
                           (size_t)(c->obs_cs - c->obs.off_head), c->s));
+  if (c->tau.acc) HIP_OK(hipMemsetAsync(c->tau.acc, 0, (size_t)c->tau.acc_cs * c->B, c->s));
   HIP_OK(hipStreamSynchronize(c->s));
   return ST_NONE;
 }
@@ -1740,6 +1966,10 @@ int alfq_control_reset(alfq_ctx* c) {
   ds.status = 0;
   HIP_OK(hipMemcpy(at(c, c->st), &cs, sizeof(cs), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(at(c, c->sc), &ds, sizeof(ds), hipMemcpyHostToDevice));
+  if (c->tau.ctl) {
+    const TauCtl k{};
+    HIP_OK(hipMemcpy(at(c, c->tau.ctl), &k, sizeof(k), hipMemcpyHostToDevice));
+  }
   return ST_NONE;
 }
 

@@ -50,15 +50,28 @@ __device__ __forceinline__ T* chain_ptr(T* p, long long stride) {
 }
 
 // Two-pass layout of a chain's arena: the first pass (base == nullptr) only sizes it, the
-// second assigns pointers at the same 256-byte aligned offsets.
+// second assigns pointers at the same 256-byte aligned offsets.  With cap > 0 the layout
+// spans regions of cap bytes, region r at base + r * rstride: a buffer that does not fit
+// the rest of a region starts the next one (so buffers in separate allocations can keep
+// the chain stride of the main arena).
 struct ArenaLayout {
   char* base = nullptr;
   size_t off = 0;
+  size_t cap = 0;
+  long long rstride = 0;
+  int region = 0;
+  bool overflow = false;     // a buffer larger than cap was requested
   template <class T>
   T* take(size_t count) {
     off = (off + 255) & ~(size_t)255;
-    T* p = base ? (T*)(base + off) : nullptr;
-    off += count * sizeof(T);
+    const size_t bytes = count * sizeof(T);
+    if (cap > 0 && off + bytes > cap && off > 0) {
+      ++region;
+      off = 0;
+    }
+    if (cap > 0 && bytes > cap) overflow = true;
+    T* p = base ? (T*)(base + (long long)region * rstride + off) : nullptr;
+    off += bytes;
     return p;
   }
 };

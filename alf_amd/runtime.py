@@ -36,7 +36,7 @@ EXPORTS = [
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
-    "alfq_set_checkerboard",
+    "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get",
 ]
 
 
@@ -60,6 +60,11 @@ class ObsDesc(C.Structure):
 class CheckerboardDesc(C.Structure):
     _fields_ = [("nbond", C.c_int32), ("nfam", C.c_int32), ("fam_len", C.POINTER(C.c_int32)),
                 ("p", C.POINTER(C.c_int32)), ("expo", C.POINTER(C.c_double)), ("expo_inv", C.POINTER(C.c_double))]
+
+
+class TauSums(C.Structure):
+    _fields_ = [("sign", C.c_double), ("n", C.c_int64), ("ncg_tau", C.c_int64), ("xmax_tau", C.c_double),
+                ("xmean_tau_sum", C.c_double)]
 
 
 class ObsSums(C.Structure):
@@ -102,6 +107,9 @@ def load_library(path: str = LIB_PATH):
         "alfq_obs_setup": [P, C.POINTER(ObsDesc)],
         "alfq_rng_set_position": [P, C.c_int64],
         "alfq_set_checkerboard": [P, C.POINTER(CheckerboardDesc)],
+        "alfq_tau_enable": [P, C.c_int32],
+        "alfq_tau_m": [P],
+        "alfq_tau_get": [P, C.POINTER(TauSums), dp, dp],
         "alfq_obs_enable": [P, C.c_int32],
         "alfq_obs_reset": [P],
         "alfq_obs_get": [P, C.POINTER(ObsSums), dp, dp],
@@ -278,6 +286,27 @@ class Engine:
 
     def obs_enable(self, on=True):
         self._c(self.lib.alfq_obs_enable(self.ctx, int(bool(on))))
+
+    # ---- time-displaced measurements (Ltau = 1) ----
+    def tau_enable(self, on=True):
+        """TAU_M after every sweep (needs obs_setup)."""
+        self._c(self.lib.alfq_tau_enable(self.ctx, int(bool(on))))
+
+    def tau_m(self):
+        """TAU_M now, after a sweep."""
+        self._c(self.lib.alfq_tau_m(self.ctx))
+
+    def tau_get(self):
+        """The selected chain's time-displaced sums: latt[3 (Green, SpinZ, Den), Ltrot + 1,
+        ncell, norb, norb], den0[norb], the count and sign sum, and Precision tau."""
+        nc, no = self._obs_shape
+        nt = self.model.Ltrot + 1
+        s = TauSums()
+        latt = np.zeros(3 * nt * nc * no * no, dtype=np.complex128)
+        den0 = np.zeros(no, dtype=np.complex128)
+        self._c(self.lib.alfq_tau_get(self.ctx, C.byref(s), _dptr(latt), _dptr(den0)))
+        return {"latt": latt.reshape(3, nt, nc, no, no), "den0": den0, "n": int(s.n), "sign": float(s.sign),
+                "ncg_tau": int(s.ncg_tau), "xmax_tau": float(s.xmax_tau), "xmean_tau_sum": float(s.xmean_tau_sum)}
 
     def obs_reset(self):
         self._c(self.lib.alfq_obs_reset(self.ctx))

@@ -220,6 +220,29 @@ int alfq_obs_reset(alfq_ctx* ctx);
 /* The selected chain's sums: latt[((q ncell + d) norb + a) norb + b] complex (re, im) with
  * q = 0 Green, 1 SpinZ, 2 Den and d the 0-based distance class; den0[norb] complex. */
 int alfq_obs_get(alfq_ctx* ctx, alfq_obs_sums* sums, double* latt, double* den0);
+/* ---- time-displaced measurements (Ltau = 1) ------------------------------- */
+/* Replaces TAU_M (Prog/tau_m_mod.F90:56-195) with CGR2_2 (Prog/cgr2_2_mod.F90:196-428) and
+ * ham%ObserT of the Hubbard plugin (Prog/Hamiltonians/Hamiltonian_Hubbard_smod.F90:826-857),
+ * called by the sweep after the slice-0 refresh when Ltau = 1 (main.F90:874-876): G(tau, 0),
+ * G(0, tau), G(tau, tau), G(0, 0) for tau = 0..Ltrot, and the correlations Green, SpinZ, Den
+ * of tau by distance class, accumulated per chain with the geometry of alfq_obs_setup.
+ * alfq_obs_reset clears these sums too. */
+typedef struct {
+  double sign;              /* Obs_tau%Ave_sign sum and Obs_tau%N (one per TAU_M call) */
+  int64_t n;
+  int64_t ncg_tau;          /* Control_Precision_tau: NCG_tau, XMAX_tau, XMEAN_tau (sum) */
+  double xmax_tau;
+  double xmean_tau_sum;
+} alfq_tau_sums;
+
+/* on != 0: run TAU_M at the end of every alfq_sweep (needs alfq_obs_setup). */
+int alfq_tau_enable(alfq_ctx* ctx, int32_t on);
+/* TAU_M now, on the current G, fields and stored left states (after a sweep). */
+int alfq_tau_m(alfq_ctx* ctx);
+/* The selected chain's sums: latt[(((q (Ltrot + 1) + tau) ncell + d) norb + a) norb + b]
+ * complex (re, im), q = 0 Green, 1 SpinZ, 2 Den; den0[norb] complex. */
+int alfq_tau_get(alfq_ctx* ctx, alfq_tau_sums* sums, double* latt, double* den0);
+
 /* ---- checkerboard Op_T ---------------------------------------------------- */
 /* Replaces the checkerboard branch of Predefined_Hoppings_set_OPT
  * (Prog/Predefined_Hop_mod.F90:1773-1834) as Hop_mod applies it (Prog/Hop_mod.F90:143-250):
