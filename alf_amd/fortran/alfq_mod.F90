@@ -43,6 +43,13 @@ module alfq_mod
     type(c_ptr) :: fam_len, p, expo, expo_inv
   end type alfq_checkerboard_desc
 
+  ! time-displaced sums (Ltau = 1) and Control_Precision_tau
+  type, bind(C), public :: alfq_tau_sums
+    real(c_double) :: sign
+    integer(c_int64_t) :: n, ncg_tau
+    real(c_double) :: xmax_tau, xmean_tau_sum
+  end type alfq_tau_sums
+
   type, bind(C), public :: alfq_obs_sums
     real(c_double) :: scal(8)
     real(c_double) :: sign_scal
@@ -56,7 +63,7 @@ module alfq_mod
             alfq_udv_assign, alfq_rng_set_tape, alfq_rng_set_philox, alfq_wrapgrup, alfq_wrapgrdo, &
             alfq_wrapur, alfq_wrapul, alfq_cgr, alfq_init, alfq_sweep, alfq_sync, alfq_control_get, &
             alfq_last_error_string, alfq_obs_setup, alfq_obs_enable, alfq_obs_reset, alfq_obs_get, &
-            alfq_set_checkerboard
+            alfq_set_checkerboard, alfq_tau_enable, alfq_tau_m, alfq_tau_get
   public :: alfq_WRAPGRUP_shim, alfq_WRAPGRDO_shim, alfq_WRAPUR_shim, alfq_WRAPUL_shim, alfq_CGR_shim
 
   interface
@@ -83,6 +90,21 @@ module alfq_mod
       import :: c_int, c_ptr, alfq_checkerboard_desc
       type(c_ptr), value :: ctx
       type(alfq_checkerboard_desc), intent(in) :: desc
+    end function
+    integer(c_int) function alfq_tau_enable(ctx, on) bind(C, name="alfq_tau_enable")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function alfq_tau_m(ctx) bind(C, name="alfq_tau_m")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function alfq_tau_get(ctx, sums, latt, den0) bind(C, name="alfq_tau_get")
+      import :: c_int, c_ptr, alfq_tau_sums
+      type(c_ptr), value :: ctx
+      type(alfq_tau_sums), intent(out) :: sums
+      type(c_ptr), value :: latt, den0
     end function
     integer(c_int) function alfq_obs_setup(ctx, desc) bind(C, name="alfq_obs_setup")
       import :: c_int, c_ptr, alfq_obs_desc

@@ -13,6 +13,10 @@ A bin is what the device accumulated between two ``alfq_obs_reset`` calls (``Eng
   (sign, Norb, N), Norb background values, then per momentum its coordinates and the
   Norb x Norb values; an ``*_info`` file with the lattice.
 
+``write_tau`` appends a bin of the time-displaced sums to ``Green_tau``, ``SpinZ_tau``,
+``Den_tau`` the same way with Ntau = Ltrot + 1 time slices (channel P for Green, PH for the
+others, Hamiltonian_Hubbard_smod.F90:652-669).
+
 ``read_scal`` and ``jackknife_scal`` read the scalar files back and give the sign-weighted
 averages with the reference's jackknife errors (jack and errcalc,
 Analysis/ana_mod.F90:862-933, Libraries/Modules/errors_mod.F90:98-126).
@@ -83,7 +87,7 @@ class BinWriter:
             xk = np.tensordot(self.ft, x, axes=(1, 0))                     # [k, a, b]
             f = self._path(name + "_eq")
             if not os.path.exists(f + "_info"):
-                self._latt_info(f + "_info", name + "_eq")
+                self._latt_info(f + "_info", name + "_eq", "--", 1)
             with open(f, "a") as fh:
                 fh.write(fortran_e(sign_eq) + f"{norb:11d}{N:11d}\n")
                 for a in range(norb):
@@ -95,14 +99,45 @@ class BinWriter:
                         for b in range(norb):
                             fh.write(_cplx(xk[ik, a, b], lead="(") + "\n")
 
-    def _latt_info(self, path, observable):
+    def write_tau(self, sums: dict) -> None:
+        """Appends a bin of the time-displaced sums (``Engine.tau_get``) to ``Green_tau``,
+        ``SpinZ_tau``, ``Den_tau`` (Print_bin_Latt with Ntau = Ltrot + 1): header (sign, Norb,
+        N, Ntau, dtau), Norb background values, then per momentum its coordinates and the
+        Ntau x Norb x Norb values."""
+        n = sums["n"]
+        if n <= 0:
+            raise ValueError("empty bin")
+        norb, N = self.unit.Norb, self.latt.N
+        ntau = sums["latt"].shape[1]
+        sign = sums["sign"] / n
+        den0 = sums["den0"] / (n * N * ntau)
+        dtau = self.m.params.Dtau
+        for q, (name, channel) in enumerate(zip(CORRELATIONS, ("P", "PH", "PH"))):
+            x = sums["latt"][q] / n                                       # [tau, d, a, b]
+            xk = np.einsum("kd,tdab->ktab", self.ft, x)
+            f = self._path(name + "_tau")
+            if not os.path.exists(f + "_info"):
+                self._latt_info(f + "_info", name + "_tau", channel, ntau)
+            with open(f, "a") as fh:
+                fh.write(fortran_e(sign) + f"{norb:11d}{N:11d}{ntau:11d}" + fortran_e(dtau, 26) + "\n")
+                for a in range(norb):
+                    v = den0[a] if name == "Den" else 0j
+                    fh.write(_cplx(v, lead="(") + "\n")
+                for ik in range(N):
+                    fh.write(fortran_e(self.K[ik, 0]) + " " + fortran_e(self.K[ik, 1]) + "\n")
+                    for t in range(ntau):
+                        for a in range(norb):
+                            for b in range(norb):
+                                fh.write(_cplx(xk[ik, t, a, b], lead="(") + "\n")
+
+    def _latt_info(self, path, observable, channel, ntau):
         L, U = self.latt, self.unit
         row = lambda k, v: f"{k:>20s}: {v}\n"
         vec = lambda a: "".join(fortran_e(t, 26) for t in a)
         with open(path, "w") as fh:
             fh.write(row("Observable", observable))
-            fh.write(row("Channel", "--"))
-            fh.write(f"{'Ntau':>20s}: {1:10d}\n")
+            fh.write(row("Channel", channel))
+            fh.write(f"{'Ntau':>20s}: {ntau:10d}\n")
             fh.write(f"{'dtau':>20s}: {fortran_e(self.m.params.Dtau, 26)}\n")
             fh.write("       ====== Bravais Lattice ======\n")
             fh.write(f"{'Unit cells':>20s}: {L.N:10d}\n")

@@ -7,7 +7,8 @@
 * the start from ``confin_<rank>`` if it exists, else a configuration drawn from the seed
   (Fields_in, Prog/Fields_mod.F90:337-506);
 * NBin bins of NSweep sweeps with the equal-time measurements on every slice of
-  LOBS_ST..LOBS_EN (main.F90:642-887), the bin files written at every bin end
+  LOBS_ST..LOBS_EN (main.F90:642-887) and, with Ltau = 1, the time-displaced ones after
+  every sweep (main.F90:874-876), the bin files written at every bin end
   (``alf_amd.obs_io``) and the configuration saved to ``confout_<rank>``
   (Fields_write_conf, Fields_mod.F90:739-760: a seed line, then one field per line,
   operators inner, slices outer);
@@ -54,7 +55,7 @@ def read_conf(path: str, n_op: int, ltrot: int):
 
 
 def simulate(params: HubbardParams, nsweep: int, nbin: int, directory: str = ".", seed: int = 0, rank: int = 0,
-             cpu_max_hours: float = 0.0, delay: int = 32, lobs=None) -> dict:
+             cpu_max_hours: float = 0.0, delay: int = 32, lobs=None, ltau: int = 0) -> dict:
     os.makedirs(directory, exist_ok=True)
     lock = os.path.join(directory, "RUNNING")
     if os.path.exists(lock):
@@ -80,6 +81,8 @@ def simulate(params: HubbardParams, nsweep: int, nbin: int, directory: str = "."
             eng.set_philox(seed, stream)
         eng.obs_setup(lobs=lobs)
         eng.obs_enable(True)
+        if ltau == 1:
+            eng.tau_enable(True)
         eng.init()
         writer = BinWriter(directory, ham)
         done = 0
@@ -89,6 +92,8 @@ def simulate(params: HubbardParams, nsweep: int, nbin: int, directory: str = "."
             eng.obs_reset()
             eng.sweep(nsweep)
             writer.write(eng.obs_get())
+            if ltau == 1:
+                writer.write_tau(eng.tau_get())
             write_conf(os.path.join(directory, f"confout_{rank}"), eng.get_fields(),
                        (seed, stream, eng.rng_position()))
             done += 1

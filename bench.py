@@ -116,6 +116,26 @@ def measured_rate(ham, fields, delay, steps):
             "note": f"{steps} sweeps after 3 from a random start (not a thermalised estimate)"}
 
 
+def tau_rate(ham, fields, delay, steps):
+    """Sweeps/s of one chain with the equal-time and the time-displaced measurements on
+    (Ltau = 1: TAU_M with CGR2_2 and ObserT after every sweep) and Precision tau."""
+    from alf_amd.runtime import Engine
+    eng = Engine(ham, delay=delay)
+    eng.set_fields(fields)
+    eng.set_philox(seed=0, stream=0)
+    eng.obs_setup()
+    eng.obs_enable(True)
+    eng.tau_enable(True)
+    eng.init()
+    eng.sweep(2)                  # the eager sweep, then the captured one
+    eng.obs_reset()
+    ms = eng.sweep_timed(steps)
+    t = eng.tau_get()
+    eng.close()
+    return {"sweeps_per_s": steps / (ms / 1e3), "tau_slices_per_sweep": ham.Ltrot + 1,
+            "precision_tau_max": t["xmax_tau"], "precision_tau_mean": t["xmean_tau_sum"] / max(t["ncg_tau"], 1)}
+
+
 def single_rate(ham, fields, delay, steps, warmup, prec):
     """Sweeps/s of one chain of the workload and its Precision Green/Phase statistics."""
     from alf_amd.runtime import Engine
@@ -206,6 +226,7 @@ def main():
     }
     eng.close()
     result["with_measurements"] = measured_rate(ham, fields, args.delay, args.steps)
+    result["with_time_displaced"] = tau_rate(ham, fields, args.delay, max(2, args.steps // 2))
     if args.batch > 0:
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
     # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64

@@ -15,6 +15,8 @@ decoupling per slice (Fields_mod.F90:270-301).
 
 Pinned (tests/test_oracle_pins.py): U = 0 against free-fermion closed forms, the atomic
 limit t = 0 against the single-site partition function, and trotter -> thermal as dtau -> 0.
+``trotter_tau`` (tests/test_tau_host.py): U = 0 against e^{-tau T} (1 + e^{-beta T})^-1, and at
+t = 0 the equal-time correlators of ``observables``.
 """
 from __future__ import annotations
 
@@ -63,6 +65,34 @@ class HubbardED:
     def thermal_rho(self, beta):
         M = self._expm(self.HT + self.HV, beta)
         return M / np.trace(M)
+
+    def trotter_tau(self, dtau, L):
+        """Time-displaced correlators of the Trotterised chain for t = 0..L slices,
+        <A(t) B> = Tr[M^(L - t) A M^t B] / Tr[M^L], M = exp(-dtau H_V) exp(-dtau H_T):
+        green[t, i, j] = <c_i(t) c_j^+> of one flavour (the estimator G(tau, 0)),
+        spin[t, i, j] = <S_i(t) S_j> with S = n_up - n_dn, den[t, i, j] = <n_i(t) n_j>."""
+        M = self._expm(self.HV, dtau) @ self._expm(self.HT, dtau)
+        pw = [np.eye(M.shape[0], dtype=complex)]
+        for _ in range(L):
+            pw.append(pw[-1] @ M)
+        Z = np.trace(pw[L])
+        n = self.n
+        sz = [self.nup[i] - self.ndn[i] for i in range(n)]
+        nt = [self.nup[i] + self.ndn[i] for i in range(n)]
+        green = np.zeros((L + 1, n, n), dtype=complex)
+        spin = np.zeros_like(green)
+        den = np.zeros_like(green)
+        for t in range(L + 1):
+            X, Y = pw[L - t], pw[t]
+            for i in range(n):
+                XcY = X @ self.c[i] @ Y
+                XsY = X @ sz[i] @ Y
+                XnY = X @ nt[i] @ Y
+                for j in range(n):
+                    green[t, i, j] = np.sum(XcY * self.c[j].conj()) / Z        # Tr[X c_i Y c_j^+]
+                    spin[t, i, j] = np.sum(XsY * sz[j].T) / Z
+                    den[t, i, j] = np.sum(XnY * nt[j].T) / Z
+        return {"Green": green, "SpinZ": spin, "Den": den}
 
     def observables(self, rho):
         """Kin, Pot (U sum <n_up n_dn>), Part, Ener and the pair correlations

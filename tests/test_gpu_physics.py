@@ -57,3 +57,48 @@ def test_monte_carlo_matches_exact_diagonalisation():
             data = np.array([[s["latt"][q, d, 0, 0] / s["n_eq"] for s in per], [1.0 + 0j] * chains])
             best, err = jackknife(lambda X: X[0] / X[1], data)
             assert abs(best.real - want.real) < 4 * err.real + slack, (name, d, best, err, want)
+
+
+def test_time_displaced_monte_carlo_matches_exact_diagonalisation():
+    """Ltau = 1 on the same cluster: the Monte Carlo averages of Green(tau), SpinZ(tau) and
+    Den(tau) by distance class lie within 4 jackknife errors (plus the decoupling's slack) of
+    the Trotterised <A(tau) B> = Tr[B^(L - t) A B^t B'] / Tr[B^L] (oracle/ed.py), at every
+    fourth tau."""
+    from alf_amd import runtime as rt
+    from alf_amd.host import lattices as LT
+    from alf_amd.obs_io import jackknife
+    from oracle.ed import HubbardED
+
+    U, beta, dtau, nwrap, chains = 4.0, 2.0, 0.1, 10, 256
+    ham = hubbard(L1=2, L2=2, U=U, beta=beta, dtau=dtau, nwrap=nwrap)
+    b = rt.BatchEngine([ham] * chains, delay=8)
+    for z in range(chains):
+        ham.nsigma.set_uniform_four_state(900 + z)
+        b.chain(z).set_fields(ham.nsigma.f.copy())
+        b.chain(z).set_philox(91, z)
+    b.obs_setup()
+    b.init()
+    b.sweep(100)                     # the local moment thermalises slower than the energy
+    b.tau_enable(True)
+    b.obs_reset()
+    b.sweep(100)
+    per = []
+    for z in range(chains):
+        b.chain(z)
+        s = b.tau_get()
+        assert s["n"] == 100 and s["sign"] == 100.0
+        assert s["xmax_tau"] < 1e-8
+        per.append(s)
+    b.close()
+
+    ex = HubbardED(ham.hop_matrix, U).trotter_tau(dtau, ham.Ltrot)
+    imj = LT.distance_classes(ham.latt)
+    site = ham.Invlist[:, 0]
+    slack = 2e-3
+    for q, name in enumerate(("Green", "SpinZ", "Den")):
+        for t in range(0, ham.Ltrot + 1, 4):
+            for d in range(ham.latt.N):
+                want = sum(ex[name][t, site[I], site[J]] for I in range(4) for J in range(4) if imj[I, J] == d)
+                data = np.array([[s["latt"][q, t, d, 0, 0] / s["n"] for s in per], [1.0 + 0j] * chains])
+                best, err = jackknife(lambda X: X[0] / X[1], data)
+                assert abs(best.real - want.real) < 4 * err.real + slack, (name, t, d, best, err, want)

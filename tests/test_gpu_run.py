@@ -35,3 +35,18 @@ def test_restart_continues_the_same_chain(tmp_path):
     assert np.isfinite(best.real) and err.real >= 0
     for name in ("Green", "SpinZ", "Den"):
         assert (a / f"{name}_eq").exists() and (a / f"{name}_eq_info").exists()
+
+
+def test_run_with_time_displaced_bins(tmp_path):
+    """Ltau = 1: every bin appends Green_tau, SpinZ_tau and Den_tau (Ntau = Ltrot + 1) with
+    their info files next to the equal-time ones."""
+    p = HubbardParams(L1=4, L2=4, Beta=1.0, Ham_U=4.0, Nwrap=5, Checkerboard=True)
+    info = simulate(p, nsweep=2, nbin=2, directory=str(tmp_path), seed=3, ltau=1)
+    assert info["bins"] == 2
+    nc, no, nt = 16, 1, 11
+    for name in ("Green", "SpinZ", "Den"):
+        lines = open(tmp_path / f"{name}_tau").read().splitlines()
+        assert len(lines) == 2 * (1 + no + nc * (1 + nt * no * no))
+        head = lines[0].split()
+        assert [int(v) for v in head[1:4]] == [no, nc, nt]
+        assert (tmp_path / f"{name}_tau_info").exists()

@@ -88,3 +88,37 @@ def test_correlation_bins_layout_and_transform(tmp_path):
             assert np.allclose(vals, (np.arange(1, 1 + 3 * no * no).reshape(3, no * no)[q]) / nc)
     info = open(tmp_path / "Green_eq_info").read()
     assert "Observable: Green_eq" in info and "Number of orbitals:          2" in info
+
+
+def test_tau_bins_layout_and_transform(tmp_path):
+    """Green_tau / SpinZ_tau / Den_tau: header with Ntau and dtau, Norb background lines
+    (Den0 / (N Ns Ntau)), then per momentum its coordinates and Ntau x Norb x Norb values; a
+    correlation that lives only at R = 0 is flat in k for every tau."""
+    ham = hubbard(L1=3, L2=3, lattice="Honeycomb", beta=0.3, dtau=0.1)
+    from alf_amd.host import lattices as LT
+    w = obs_io.BinWriter(str(tmp_path), ham)
+    nc, no, nt = ham.latt.N, ham.unit.Norb, ham.Ltrot + 1
+    d0 = int(LT.distance_classes(ham.latt)[0, 0])
+    latt = np.zeros((3, nt, nc, no, no), dtype=complex)
+    for t in range(nt):
+        latt[:, t, d0] = (t + 1) * np.arange(1, 1 + 3 * no * no).reshape(3, no, no)
+    sums = {"latt": 2 * latt, "den0": np.array([2 * nc * nt * 0.5, 2 * nc * nt * 0.25]), "n": 2, "sign": 2.0}
+    w.write_tau(sums)
+    for q, name in enumerate(obs_io.CORRELATIONS):
+        lines = open(tmp_path / f"{name}_tau").read().splitlines()
+        assert len(lines) == 1 + no + nc * (1 + nt * no * no)
+        head = lines[0].split()
+        assert [int(v) for v in head[1:4]] == [no, nc, nt]
+        assert abs(float(head[4].replace("E", "e")) - 0.1) < 1e-15
+        assert float(head[0].replace("E", "e")) == 1.0
+        bg = [complex(*map(float, l.strip("()").replace("E", "e").split(","))) for l in lines[1:1 + no]]
+        assert np.allclose(bg, [0.5, 0.25] if name == "Den" else [0, 0])
+        for ik in range(nc):
+            base = 1 + no + ik * (1 + nt * no * no)
+            vals = np.array([complex(*map(float, l.strip("()").replace("E", "e").split(",")))
+                             for l in lines[base + 1:base + 1 + nt * no * no]]).reshape(nt, no * no)
+            for t in range(nt):
+                assert np.allclose(vals[t], (t + 1) * np.arange(1, 1 + 3 * no * no).reshape(3, no * no)[q] / nc)
+        info = open(tmp_path / f"{name}_tau_info").read()
+        assert f"Observable: {name}_tau" in info and f"{'Ntau':>20s}: {nt:10d}" in info
+        assert ("Channel: P\n" if name == "Green" else "Channel: PH\n") in info

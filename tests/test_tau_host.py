@@ -127,3 +127,28 @@ def test_tau_measurements_at_tau_zero_equal_the_equal_time_ones(lattice, L, nphi
     np.add.at(want, (dist, np.broadcast_to(ham.List[:, 1][:, None], (n, n)),
                      np.broadcast_to(ham.List[:, 1][None, :], (n, n))), (np.eye(n) - s.G) * w)
     assert rel(t.latt[0, ham.Ltrot], want) < 1e-9
+
+
+def test_ed_time_displaced_correlators():
+    """oracle/ed.py's Trotterised <A(t) B>: at U = 0 the one-flavour Green's function is
+    e^{-t dtau T} (1 + e^{-beta T})^-1 and the density correlation follows from Wick's theorem;
+    at t = 0 the spin and density correlators equal the equal-time ones."""
+    from oracle.ed import HubbardED
+    ham = hubbard(L1=2, L2=2, U=0.0, beta=1.0, dtau=0.1, chem=0.2)
+    T = ham.hop_matrix
+    ed = HubbardED(T, 0.0)
+    tau = ed.trotter_tau(0.1, ham.Ltrot)
+    n = ham.Ndim
+    G0 = np.linalg.inv(np.eye(n) + expm(-1.0 * T))
+    for t in range(ham.Ltrot + 1):
+        Gt = expm(-0.1 * t * T) @ G0
+        assert rel(tau["Green"][t], Gt) < 1e-10
+        # <n_i(t) n_j> = <n_i><n_j> + 2 <c_i^+(t) c_j> <c_i(t) c_j^+> for two free flavours
+        G0t = -(np.eye(n) - G0) @ expm(0.1 * t * T)               # G(0, t)
+        dens = 2 * (1 - np.diag(G0))
+        want = dens[:, None] * dens[None, :] - 2 * G0t.T * Gt
+        assert rel(tau["Den"][t], want) < 1e-10
+    ed4 = HubbardED(T, 4.0)
+    tau4 = ed4.trotter_tau(0.1, ham.Ltrot)
+    eq = ed4.observables(ed4.trotter_rho(0.1, ham.Ltrot))
+    assert rel(tau4["SpinZ"][0], eq["SpinZ"]) < 1e-12 and rel(tau4["Den"][0], eq["Den"]) < 1e-12
