@@ -194,6 +194,7 @@ struct alfq_ctx {
     cplx *DUPv = nullptr, *DUPvinv = nullptr, *DUP2 = nullptr, *DUP2inv = nullptr, *zs = nullptr;
     int* flag = nullptr;
     TauCtl* ctl = nullptr;
+    double* part = nullptr;              // partial precision sums of CGR2_2's extraction
     QRWork w2;                           // QR of the 2n x 2n matrix of CGR2_2
     char* acc = nullptr;
     long long acc_cs = 0, off_den0 = 0, off_latt = 0;
@@ -668,7 +669,7 @@ int cgr2_2_dev(alfq_ctx* c, const UDVSlot& R, const UDVSlot& Lf, cplx* G00, cplx
   GemmArgs g2 = gemm_args(m, n, n, t.Y2, m, OP_N, true, t.F2, n, OP_N, true, t.O2 + (long long)m * n, m, ONE, ZERO);
   g2.amap = t.w2.pinv + n;
   if ((st = gemm_c(c, g2))) return st;
-  return cgr22_extract(t.O2, t.flag, c->G, G00, G0T, GT0, GTT, t.ctl, n, c->B, c->cs, c->s);
+  return cgr22_extract(t.O2, t.flag, c->G, G00, G0T, GT0, GTT, t.part, t.ctl, n, c->B, c->cs, c->s);
 }
 
 // TAU_M (tau_m_mod.F90:56-195), finite temperature, after a sweep: G(tau, 0), G(0, tau),
@@ -1466,6 +1467,7 @@ int tau_alloc(alfq_ctx* c) {
     t.zs = ar.take<cplx>(1);
     t.flag = ar.take<int>(1);
     t.ctl = ar.take<TauCtl>(1);
+    t.part = ar.take<double>(8 * (size_t)CGR22_PARTS);
     qr_work_place(t.w2, m, ar);
   };
   ArenaLayout sizing;

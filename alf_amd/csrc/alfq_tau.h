@@ -31,10 +31,13 @@ int cgr22_build(const cplx* X, const int* pinv, const cplx* UL, const cplx* DL, 
                 const cplx* VR, cplx* H, cplx* F1, cplx* F2, int* flag, int n, int nchain, long long cs,
                 hipStream_t s);
 
-// The blocks of O^-1 (2n x 2n, ld 2n) into G00, G0T, GT0, GTT, after Control_Precision_tau of
-// (GR, G00), (old GTT, GTT), (old GT0, GT0), (old G0T, G0T) into ctl.
+// Workgroups of the block extraction (part holds 8 doubles per workgroup and chain).
+constexpr int CGR22_PARTS = 64;
+
+// The blocks of O^-1 (2n x 2n, ld 2n) into G00, G0T, GT0, GTT, with Control_Precision_tau of
+// (GR, G00), (old GTT, GTT), (old GT0, GT0), (old G0T, G0T) into ctl (partials in part).
 int cgr22_extract(const cplx* Oinv, const int* flag, const cplx* GR, cplx* G00, cplx* G0T, cplx* GT0, cplx* GTT,
-                  TauCtl* ctl, int n, int nchain, long long cs, hipStream_t s);
+                  double* part, TauCtl* ctl, int n, int nchain, long long cs, hipStream_t s);
 
 // ObserT of the Hubbard plugin at time slice nt (0..Ltrot) into the chains' blocks of `acc`
 // (tau_cs bytes each: TauHead, Den0[norb] at off_den0, [3][ntau][ncell][norb][norb] at

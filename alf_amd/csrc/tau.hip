@@ -78,14 +78,18 @@ __global__ void cgr22_build_kernel(const cplx* __restrict__ X, const int* __rest
   if (i >= n && j >= n) F2[r + (long long)c * n] = v;
 }
 
-constexpr int XT = 512;
+constexpr int XT = 256;
+constexpr int XB = CGR22_PARTS;   // workgroups per chain of the block extraction
 
+// Grid-wide pass: the blocks of O^-1 into G00, G0T, GT0, GTT and, per workgroup, the
+// maxima and sums of |old - new| of the four Control_Precision_tau comparisons into
+// part[blockIdx.x][8].
 __global__ __launch_bounds__(XT) void cgr22_extract_kernel(const cplx* __restrict__ Oinv,
-                                                             const int* __restrict__ flag,
-                                                             const cplx* __restrict__ GR, cplx* __restrict__ G00,
-                                                             cplx* __restrict__ G0T, cplx* __restrict__ GT0,
-                                                             cplx* __restrict__ GTT, TauCtl* ctl, int n,
-                                                             long long cs) {
+                                                           const int* __restrict__ flag,
+                                                           const cplx* __restrict__ GR, cplx* __restrict__ G00,
+                                                           cplx* __restrict__ G0T, cplx* __restrict__ GT0,
+                                                           cplx* __restrict__ GTT, double* __restrict__ part, int n,
+                                                           long long cs) {
   __shared__ double smax[4][XT];
   __shared__ double ssum[4][XT];
   Oinv = chain_ptr(Oinv, cs);
@@ -95,13 +99,13 @@ __global__ __launch_bounds__(XT) void cgr22_extract_kernel(const cplx* __restric
   G0T = chain_ptr(G0T, cs);
   GT0 = chain_ptr(GT0, cs);
   GTT = chain_ptr(GTT, cs);
-  ctl = chain_ptr(ctl, cs);
+  part = chain_ptr(part, cs);
   const int tid = threadIdx.x;
   const bool first = *flag != 0;
   const int m = 2 * n;
   double mx[4] = {0, 0, 0, 0}, sm[4] = {0, 0, 0, 0};
   const long long nn = (long long)n * n;
-  for (long long e = tid; e < nn; e += blockDim.x) {
+  for (long long e = (long long)blockIdx.x * XT + tid; e < nn; e += (long long)XB * XT) {
     const int i = (int)(e % n), j = (int)(e / n);
     const cplx a = Oinv[i + (long long)j * m];               // upper-left
     const cplx b = Oinv[i + (long long)(j + n) * m];         // upper-right
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(XT) void cgr22_extract_kernel(const cplx* __restric
     ssum[q][tid] = sm[q];
   }
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = XT / 2; s > 0; s >>= 1) {
     if (tid < s)
       for (int q = 0; q < 4; ++q) {
         smax[q][tid] = fmax(smax[q][tid], smax[q][tid + s]);
@@ -135,12 +139,28 @@ __global__ __launch_bounds__(XT) void cgr22_extract_kernel(const cplx* __restric
       }
     __syncthreads();
   }
-  if (tid == 0) {
-    for (int q = 0; q < 4; ++q) {                 // Control_Precision_tau in the reference's order
-      ctl->ncg += 1;
-      ctl->xmax = fmax(ctl->xmax, smax[q][0]);
-      ctl->xmean += ssum[q][0] / (double)nn;
+  if (tid < 4) {
+    part[blockIdx.x * 8 + tid] = smax[tid][0];
+    part[blockIdx.x * 8 + 4 + tid] = ssum[tid][0];
+  }
+}
+
+// The workgroups' partial results in a fixed order, then Control_Precision_tau in the
+// reference's order: (GR, G00), (old GTT, GTT), (old GT0, GT0), (old G0T, G0T).
+__global__ void cgr22_precision_kernel(const double* __restrict__ part, TauCtl* ctl, int n, long long cs) {
+  part = chain_ptr(part, cs);
+  ctl = chain_ptr(ctl, cs);
+  if (threadIdx.x != 0) return;
+  const double nn = (double)n * (double)n;
+  for (int q = 0; q < 4; ++q) {
+    double mx = 0.0, sm = 0.0;
+    for (int b = 0; b < XB; ++b) {
+      mx = fmax(mx, part[b * 8 + q]);
+      sm += part[b * 8 + 4 + q];
     }
+    ctl->ncg += 1;
+    ctl->xmax = fmax(ctl->xmax, mx);
+    ctl->xmean += sm / nn;
   }
 }
 
@@ -244,9 +264,10 @@ int cgr22_build(const cplx* X, const int* pinv, const cplx* UL, const cplx* DL, 
 }
 
 int cgr22_extract(const cplx* Oinv, const int* flag, const cplx* GR, cplx* G00, cplx* G0T, cplx* GT0, cplx* GTT,
-                  TauCtl* ctl, int n, int nchain, long long cs, hipStream_t s) {
-  hipLaunchKernelGGL(cgr22_extract_kernel, dim3(1, 1, nchain), dim3(XT), 0, s, Oinv, flag, GR, G00, G0T, GT0, GTT,
-                     ctl, n, cs);
+                  double* part, TauCtl* ctl, int n, int nchain, long long cs, hipStream_t s) {
+  hipLaunchKernelGGL(cgr22_extract_kernel, dim3(XB, 1, nchain), dim3(XT), 0, s, Oinv, flag, GR, G00, G0T, GT0, GTT,
+                     part, n, cs);
+  hipLaunchKernelGGL(cgr22_precision_kernel, dim3(1, 1, nchain), dim3(64), 0, s, (const double*)part, ctl, n, cs);
   HIP_OK(hipGetLastError());
   return ST_NONE;
 }

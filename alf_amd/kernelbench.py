@@ -21,7 +21,7 @@ is committed beside the other profiles and read here.
 The ``sweep`` mode runs sweeps of the 16 x 16 U = 4 beta = 10 Hubbard workload, one chain
 and then a batch of chains, for per-kernel profiles of either hopping form:
 
-    python3 -m alf_amd.kernelbench sweep [--checkerboard] [--batch B] [--steps K]
+    python3 -m alf_amd.kernelbench sweep [--checkerboard] [--tau] [--batch B] [--steps K]
 """
 from __future__ import annotations
 
@@ -98,8 +98,9 @@ def copy_calibration(nbytes: int, repeats: int = 3) -> None:
     print(f"copied {nbytes} bytes device to device x {repeats}", flush=True)
 
 
-def profile_sweeps(checkerboard: bool, batch: int, steps: int) -> None:
-    """`steps` sweeps of one chain, then of `batch` chains in one batched context."""
+def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False) -> None:
+    """`steps` sweeps of one chain, then of `batch` chains in one batched context; with tau,
+    the equal-time and time-displaced measurements on (TAU_M after every sweep)."""
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import BatchEngine, Engine
     ham = hubbard_ham_set(HubbardParams(Lattice_type="Square", L1=16, L2=16, Ham_U=4.0, Beta=10.0, Dtau=0.1,
@@ -108,6 +109,10 @@ def profile_sweeps(checkerboard: bool, batch: int, steps: int) -> None:
     eng = Engine(ham, delay=32)
     eng.set_fields(ham.nsigma.f.copy())
     eng.set_philox(seed=0, stream=0)
+    if tau:
+        eng.obs_setup()
+        eng.obs_enable(True)
+        eng.tau_enable(True)
     eng.init()
     ms = eng.sweep_timed(steps)
     eng.close()
@@ -118,6 +123,10 @@ def profile_sweeps(checkerboard: bool, batch: int, steps: int) -> None:
             ham.nsigma.set_uniform_four_state(z)
             b.chain(z).set_fields(ham.nsigma.f.copy())
             b.chain(z).set_philox(seed=z, stream=0)
+        if tau:
+            b.obs_setup()
+            b.obs_enable(True)
+            b.tau_enable(True)
         b.init()
         ms = b.sweep_timed(steps)
         b.close()
@@ -130,6 +139,6 @@ if __name__ == "__main__":
         copy_calibration(mib << 20)
     elif len(sys.argv) >= 2 and sys.argv[1] == "sweep":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
-        profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3))
+        profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3), "--tau" in sys.argv)
     else:
         print(__doc__)

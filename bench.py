@@ -136,6 +136,33 @@ def tau_rate(ham, fields, delay, steps):
             "precision_tau_max": t["xmax_tau"], "precision_tau_mean": t["xmean_tau_sum"] / max(t["ncg_tau"], 1)}
 
 
+def config_rates(delay, cpu=True):
+    """The other single-chain configurations: the 4x4 U=4 beta=5 case (with the oracle's
+    rate on one host core beside it) and the 12x12 honeycomb with a pi/4 flux per plaquette
+    (complex hopping), U=3, beta=12, Nwrap=8, each on the dense and the checkerboard hopping."""
+    from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+    out = {}
+    cases = {
+        "square_4x4_U4_beta5": dict(Lattice_type="Square", L1=4, L2=4, Ham_U=4.0, Beta=5.0, Dtau=0.1, Nwrap=10),
+        "honeycomb_12x12_flux_pi4_U3_beta12": dict(Lattice_type="Honeycomb", L1=12, L2=12, Ham_U=3.0, Beta=12.0,
+                                                   Dtau=0.1, Nwrap=8, N_Phi=18),
+    }
+    for name, kw in cases.items():
+        row = {}
+        for cb in (False, True):
+            ham = hubbard_ham_set(HubbardParams(**kw, Checkerboard=cb))
+            ham.nsigma.set_uniform_four_state(0)
+            f = ham.nsigma.f.copy()
+            r = single_rate(ham, f, delay, 5 if ham.Ndim > 64 else 20, 2, 0)
+            row["checkerboard" if cb else "dense"] = {k: r[k] for k in ("sweeps_per_s", "precision_green_max")}
+            if cpu and not cb and ham.Ndim <= 16:
+                row["cpu_oracle_sweeps_per_s"] = cpu_baseline(ham, f)["value"]
+        row["ndim"] = ham.Ndim
+        row["ltrot"] = ham.Ltrot
+        out[name] = row
+    return out
+
+
 def single_rate(ham, fields, delay, steps, warmup, prec):
     """Sweeps/s of one chain of the workload and its Precision Green/Phase statistics."""
     from alf_amd.runtime import Engine
@@ -166,6 +193,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
     ap.add_argument("--no-checkerboard", action="store_true", help="skip the checkerboard-hopping legs")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 4x4 and 12x12-honeycomb single chains")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--serve-sweeps", type=int, default=10)
     args = ap.parse_args()
@@ -252,6 +280,8 @@ def main():
         result["checkerboard"] = cb
     if args.serve_requests > 0:
         result["serving"] = serving_rate(args.serve_requests, args.serve_sweeps, 50.0)
+    if not args.no_configs:
+        result["other_configs"] = config_rates(args.delay, cpu=not args.no_cpu_baseline)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
     from alf_amd.runtime import mfma_rate
     result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
