@@ -23,9 +23,30 @@ namespace {
 constexpr int PT = 256;   // threads of the single-workgroup kernels
 constexpr int NBMAX = 32; // largest panel width
 
+// A double moved between the lanes of a 16-lane row by the DPP control CTRL (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over each 16-lane row, in every lane of the row: quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror (register moves, no LDS round trip).  Every lane
+// adds the same two values, so all lanes hold the bit-identical sum.
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return v;
+}
+
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = row16_sum(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
   return v;
 }
 
@@ -358,12 +379,9 @@ __global__ __launch_bounds__(PLT) void panel_lds_kernel(cplx* __restrict__ A, in
       }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-      for (int i = 0; i < QPW; ++i) {
-        d[i].x += __shfl_xor(d[i].x, o, 64);
-        d[i].y += __shfl_xor(d[i].y, o, 64);
-      }
+    for (int i = 0; i < QPW; ++i) {
+      d[i].x = wave_sum(d[i].x);
+      d[i].y = wave_sum(d[i].y);
     }
     const cplx tc = cconj(t), scc = cconj(sc);
 #pragma unroll
@@ -497,9 +515,8 @@ constexpr int RPL = 8;          // rows per lane: 256 rows
 constexpr int RP = 32 * RPL;
 
 __device__ __forceinline__ double hsum32(double v) {
-#pragma unroll
-  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
-  return v;
+  v = row16_sum(v);
+  return v + __shfl_xor(v, 16, 32);
 }
 
 __global__ __launch_bounds__(1024) void panel_reg_kernel(cplx* __restrict__ A, int n, int k, int nb,
