@@ -43,11 +43,37 @@ __device__ __forceinline__ double row16_sum(double v) {
   return v;
 }
 
-__device__ inline double wave_sum(double v) {
+// gfx950 lane swaps (register moves, no LDS): with both operands x,
+// permlane16_swap gives [0] = rows (0, 0, 2, 2) and [1] = rows (1, 1, 3, 3) of x,
+// permlane32_swap gives [0] = (low half, low half) and [1] = (high half, high half).
+__device__ __forceinline__ void swap16_f64(double v, double& even, double& odd) {
+  const long long b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)(b & 0xffffffffLL), (unsigned)(b & 0xffffffffLL), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  even = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+  odd = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+}
+__device__ __forceinline__ void swap32_f64(double v, double& low, double& high) {
+  const long long b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)(b & 0xffffffffLL), (unsigned)(b & 0xffffffffLL), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  low = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+  high = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+}
+
+// Sum over each 32-lane half of the wave, in every lane of the half (bit-identical).
+__device__ __forceinline__ double half_sum(double v) {
   v = row16_sum(v);
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
+  double e, o;
+  swap16_f64(v, e, o);
+  return e + o;
+}
+
+__device__ inline double wave_sum(double v) {
+  v = half_sum(v);
+  double lo, hi;
+  swap32_f64(v, lo, hi);
+  return lo + hi;
 }
 
 // block-wide sum of a double; all threads get the result.  red: >= PT/64 doubles of LDS.
@@ -514,10 +540,7 @@ __global__ __launch_bounds__(PLT) void panel_lds_kernel(cplx* __restrict__ A, in
 constexpr int RPL = 8;          // rows per lane: 256 rows
 constexpr int RP = 32 * RPL;
 
-__device__ __forceinline__ double hsum32(double v) {
-  v = row16_sum(v);
-  return v + __shfl_xor(v, 16, 32);
-}
+__device__ __forceinline__ double hsum32(double v) { return half_sum(v); }
 
 __global__ __launch_bounds__(1024) void panel_reg_kernel(cplx* __restrict__ A, int n, int k, int nb,
                                                          cplx* __restrict__ tau, cplx* __restrict__ Vw,
@@ -558,8 +581,17 @@ __global__ __launch_bounds__(1024) void panel_reg_kernel(cplx* __restrict__ A, i
       double beta = alpha.x;
       if (!(xs == 0.0 && alpha.y == 0.0)) {
         beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs), alpha.x);
-        tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-        sc = cinv(make_double2(alpha.x - beta, alpha.y));
+        const double rb = 1.0 / beta;
+        tj = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
+        // 1 / (alpha - beta) by Smith's scaling with one reciprocal
+        const double zx = alpha.x - beta, zy = alpha.y;
+        if (fabs(zx) >= fabs(zy)) {
+          const double r = zy / zx, id = 1.0 / (zx + r * zy);
+          sc = make_double2(id, -r * id);
+        } else {
+          const double r = zx / zy, id = 1.0 / (zy + r * zx);
+          sc = make_double2(r * id, -id);
+        }
       }
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
