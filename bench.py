@@ -194,9 +194,14 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
     ap.add_argument("--no-checkerboard", action="store_true", help="skip the checkerboard-hopping legs")
     ap.add_argument("--no-configs", action="store_true", help="skip the 4x4 and 12x12-honeycomb single chains")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="the single-chain line and its roofline only (for kernel profiles of that run)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--serve-sweeps", type=int, default=10)
     args = ap.parse_args()
+    if args.only_headline:
+        args.batch, args.no_mixed, args.no_checkerboard, args.no_configs = 0, True, True, True
+        args.serve_requests, args.no_cpu_baseline = 0, True
     if args.gpus != 1:
         print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
               "independent chains shard across GPUs without communication, so use --gpus 1", file=sys.stderr)
@@ -253,8 +258,9 @@ def main():
         "roofline": kb,
     }
     eng.close()
-    result["with_measurements"] = measured_rate(ham, fields, args.delay, args.steps)
-    result["with_time_displaced"] = tau_rate(ham, fields, args.delay, max(2, args.steps // 2))
+    if not args.only_headline:
+        result["with_measurements"] = measured_rate(ham, fields, args.delay, args.steps)
+        result["with_time_displaced"] = tau_rate(ham, fields, args.delay, max(2, args.steps // 2))
     if args.batch > 0:
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
     # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64
@@ -284,8 +290,9 @@ def main():
         result["other_configs"] = config_rates(args.delay, cpu=not args.no_cpu_baseline)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
     from alf_amd.runtime import mfma_rate
-    result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
-                                        "fp64_vector_fma": mfma_rate(2)}
+    if not args.only_headline:
+        result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
+                                            "fp64_vector_fma": mfma_rate(2)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ham, fields)
     print(json.dumps(result), flush=True)
