@@ -81,6 +81,9 @@ def device_free_bytes() -> int:
     return int(free.value)
 
 
+GRAPH_MIN_SWEEPS = 32        # requests with fewer sweeps run them eagerly
+
+
 class _HubbardBackend:
     """Builds models and batched engines of the gfx950 sweep engine."""
 
@@ -95,11 +98,14 @@ class _HubbardBackend:
     def model(self, r: Request):
         key = (r.lattice, r.L, r.beta, r.U, r.nwrap, r.dtau, r.n_phi)
         with self._lock:
-            if key not in self._models:
-                self._models[key] = self._ham_set(self._params(Lattice_type=r.lattice, L1=r.L, L2=r.L, Beta=r.beta,
-                                                               Ham_U=r.U, Nwrap=r.nwrap, Dtau=r.dtau,
-                                                               N_Phi=r.n_phi))
-            return self._models[key]
+            m = self._models.get(key)
+        if m is None:
+            # built outside the lock, so that the batches' threads set up their models in parallel
+            m = self._ham_set(self._params(Lattice_type=r.lattice, L1=r.L, L2=r.L, Beta=r.beta, Ham_U=r.U,
+                                           Nwrap=r.nwrap, Dtau=r.dtau, N_Phi=r.n_phi))
+            with self._lock:
+                m = self._models.setdefault(key, m)
+        return m
 
     def chain_bytes(self, r: Request) -> int:
         return self.rt.chain_bytes(self.model(r))
@@ -117,6 +123,10 @@ class _HubbardBackend:
         return eng
 
     def run(self, eng, nsweeps: int):
+        # capturing a whole-sweep graph (two parities) costs more than a few eager sweeps
+        # save, and captures of concurrent batches take turns
+        if nsweeps < GRAPH_MIN_SWEEPS:
+            eng.sweep_graphs(False)
         eng.sweep(nsweeps)
         eng.sync()
 

@@ -105,6 +105,25 @@ def serving_rate(nreq, nsweeps, rate, seed=0):
     return out
 
 
+def c4_mix_rate(chains, nsweeps, seed=0):
+    """The batched-chains mix: `chains` independent Hubbard chains, square L uniform in
+    {6, 8, ..., 20} (Ndim 36..400), U uniform in [2, 8], beta uniform in [2, 12] (Ltrot =
+    nint(beta / 0.1)), Nwrap 10, chain i seeded with i, all submitted at once to the serving
+    loop: chains of one shape (L, Ltrot) share a batch, the batches run concurrently."""
+    from alf_amd.serving import Request, Server
+    rng = np.random.default_rng(seed)
+    reqs = [Request(rid=i, L=int(rng.choice(np.arange(6, 21, 2))), beta=float(rng.uniform(2.0, 12.0)),
+                    U=float(rng.uniform(2.0, 8.0)), seed=i, nsweeps=nsweeps, arrival=0.0) for i in range(chains)]
+    srv = Server()
+    res = srv.run(reqs)
+    return {"chains": chains, "sweeps_per_chain": nsweeps, "distinct_shapes": len({r.shape() for r in reqs}),
+            "aggregate_sweeps_per_s": chains * nsweeps / srv.wall, "wall_s": srv.wall,
+            "peak_concurrent_batches": srv.peak_batches, "largest_batch": max(r.batch_size for r in res),
+            "precision_green_max": max(r.control["XMAXG"] for r in res),
+            "form": "per-shape batches on concurrent streams (a batch shares Ndim and Ltrot); wall includes "
+                    "each chain's model set-up, context creation, init and download"}
+
+
 def measured_rate(ham, fields, delay, steps):
     """Sweeps/s of one chain with the equal-time measurements on (2 Ltrot - 1 per sweep) and
     the measured energy and double occupancy of those sweeps."""
@@ -206,6 +225,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-wrap legs")
     ap.add_argument("--no-checkerboard", action="store_true", help="skip the checkerboard-hopping legs")
     ap.add_argument("--no-configs", action="store_true", help="skip the 4x4 and 12x12-honeycomb single chains")
+    ap.add_argument("--c4-chains", type=int, default=256, help="chains of the uneven-size mix (0: skip it)")
+    ap.add_argument("--c4-sweeps", type=int, default=4)
     ap.add_argument("--only-headline", action="store_true",
                     help="the single-chain line and its roofline only (for kernel profiles of that run)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
@@ -213,7 +234,7 @@ def main():
     args = ap.parse_args()
     if args.only_headline:
         args.batch, args.no_mixed, args.no_checkerboard, args.no_configs = 0, True, True, True
-        args.serve_requests, args.no_cpu_baseline = 0, True
+        args.serve_requests, args.no_cpu_baseline, args.c4_chains = 0, True, 0
     if args.gpus != 1:
         print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
               "independent chains shard across GPUs without communication, so use --gpus 1", file=sys.stderr)
@@ -304,6 +325,8 @@ def main():
         result["serving"] = serving_rate(args.serve_requests, args.serve_sweeps, 50.0)
     if not args.no_configs:
         result["other_configs"] = config_rates(args.delay, cpu=not args.no_cpu_baseline)
+    if args.c4_chains > 0:
+        result["uneven_batch_mix"] = c4_mix_rate(args.c4_chains, args.c4_sweeps)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
     from alf_amd.runtime import mfma_rate
     if not args.only_headline:
