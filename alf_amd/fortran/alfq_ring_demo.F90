@@ -6,10 +6,19 @@
 !  for the first up pass slices (alfq_WRAPGRUP_shim with G on the host, as
 !  main.F90 calls WRAPGRUP) and then runs whole sweeps on the device.
 !
-!  usage: alfq_ring_demo <tape file> <output file>
+!  With a third file name it runs the ring again with the checkerboard hopping
+!  (two families of bonds on disjoint sites, E_b = exp(dtau t_b sigma_x), with
+!  unequal t_b so that the families do not commute and the bond order matters)
+!  set through alfq_set_checkerboard, the measurement geometry of the ring, and
+!  TAU_M after each of two sweeps, as main.F90 does with Ltau = 1.
+!
+!  usage: alfq_ring_demo <tape file> <output file> [<checkerboard output file>]
 !    tape file: count, then that many uniforms (one per line)
 !    output:    final fields, then G (re im per line, column-major), PHASE,
 !               and the NC_up / ACC_up counters
+!    checkerboard output: the same, then N, NCG_tau, the sign sum, XMAX_tau and
+!               the time-displaced sums (Green, SpinZ, Den) x (tau = 0..Ltrot)
+!               x (distance 0..3), then Den0, re im per line
 program alfq_ring_demo
   use iso_c_binding
   use alfq_mod
@@ -23,12 +32,21 @@ program alfq_ring_demo
   type(alfq_model_desc) :: desc
   type(alfq_control) :: ctl
   type(c_ptr) :: ctx
-  integer :: i, j, d, st, nt, ntape, unit_in, unit_out
+  ! checkerboard bonds (1,2) (3,4) | (2,3) (4,1) and the ring's measurement tables
+  integer(c_int32_t), target :: famlen(2), bp(2, n), invlist(n), imj(n, n), orow(2*n), ocol(2*n)
+  complex(c_double_complex), target :: expo(2, 2, n), expoi(2, 2, n), oval(2*n)
+  real(c_double), target :: tlatt(2, n, 0:ltrot, 3), tden0(2)
+  type(alfq_checkerboard_desc) :: cbd
+  type(alfq_obs_desc) :: od
+  type(alfq_tau_sums) :: ts
+  integer :: i, j, d, st, nt, ntape, unit_in, unit_out, b, q
   real(kind=8) :: a, dtau, U
-  character(len=512) :: tape_file, out_file
+  character(len=512) :: tape_file, out_file, cb_file
 
   call get_command_argument(1, tape_file)
   call get_command_argument(2, out_file)
+  cb_file = ''
+  if (command_argument_count() >= 3) call get_command_argument(3, cb_file)
   dtau = 0.1d0
   U = 4.d0
   ! exp(-dtau T) for T = -t (S + S^T) on the ring: entries depend on the distance d only
@@ -102,8 +120,112 @@ program alfq_ring_demo
   write(unit_out, '(2I12)') ctl%nc_up, ctl%acc_up
   close(unit_out)
   print '(A,ES10.2,A,I8)', 'alfq_ring_demo: Precision Green max ', ctl%xmax_g, '  accepted ', ctl%acc_up
+  if (len_trim(cb_file) > 0) call checkerboard_tau_run()
 
 contains
+
+  ! The ring with the checkerboard Op_T and TAU_M: bonds b = 1..4 join sites bp(:, b)
+  ! with amplitude tb(b); family 1 holds (1,2) and (3,4), family 2 holds (2,3) and (4,1).
+  ! H_T = c^+ O c with O = -t_b on the bonds, so exp(-dtau O_b) = [[ch, sh], [sh, ch]] of
+  ! dtau t_b.
+  subroutine checkerboard_tau_run()
+    complex(c_double_complex) :: P(n, n), Pi(n, n), Eb(n, n)
+    real(kind=8), parameter :: tb(n) = [1.0d0, 0.8d0, 1.2d0, 0.9d0]
+    real(kind=8) :: ch, sh
+    famlen = [2, 2]
+    bp(:, 1) = [1, 2]; bp(:, 2) = [3, 4]; bp(:, 3) = [2, 3]; bp(:, 4) = [4, 1]
+    do b = 1, n
+      ch = cosh(dtau*tb(b)); sh = sinh(dtau*tb(b))
+      expo(:, :, b) = reshape([ch, sh, sh, ch], [2, 2])
+      expoi(:, :, b) = reshape([ch, -sh, -sh, ch], [2, 2])
+    enddo
+    ! the descriptor's dense exp(-dtau T) is P = E_1 E_2 E_3 E_4 and its inverse
+    P = 0; Pi = 0
+    do i = 1, n
+      P(i, i) = 1; Pi(i, i) = 1
+    enddo
+    do b = 1, n
+      call embed(expo(:, :, b), bp(:, b), Eb)
+      P = matmul(P, Eb)
+      call embed(expoi(:, :, n + 1 - b), bp(:, n + 1 - b), Eb)
+      Pi = matmul(Pi, Eb)
+    enddo
+    expT = real(P); expTi = real(Pi)
+    cbd%nbond = n; cbd%nfam = 2
+    cbd%fam_len = c_loc(famlen); cbd%p = c_loc(bp); cbd%expo = c_loc(expo); cbd%expo_inv = c_loc(expoi)
+    ! one orbital per cell; Latt%imj(I, J) = cell of R_I - R_J on the ring
+    do i = 1, n
+      invlist(i) = i
+      do j = 1, n
+        imj(i, j) = modulo(i - j, n) + 1
+      enddo
+      orow(2*i - 1) = bp(1, i); ocol(2*i - 1) = bp(2, i); oval(2*i - 1) = cmplx(-tb(i), 0.d0, kind=8)
+      orow(2*i) = bp(2, i); ocol(2*i) = bp(1, i); oval(2*i) = cmplx(-tb(i), 0.d0, kind=8)
+    enddo
+    od%ncell = n; od%norb = 1; od%nnz = 2*n; od%lobs_st = 1; od%lobs_en = ltrot
+    od%invlist = c_loc(invlist); od%imj = c_loc(imj); od%o_row = c_loc(orow); od%o_col = c_loc(ocol)
+    od%o_val = c_loc(oval); od%ham_u = U
+
+    call check(alfq_ctx_create(desc, ctx))
+    call check(alfq_set_checkerboard(ctx, cbd))
+    call check(alfq_obs_setup(ctx, od))
+    call check(alfq_tau_enable(ctx, 1_c_int32_t))
+    do nt = 1, ltrot
+      do i = 1, n_op
+        f(i, nt) = merge(1, -1, modulo(i + nt, 2) == 0) * merge(2, 1, modulo(i * nt, 3) == 0)
+      enddo
+    enddo
+    call check(alfq_fields_set(ctx, f, int(n_op, c_int32_t), int(ltrot, c_int32_t)))
+    call check(alfq_rng_set_tape(ctx, tape, int(ntape, c_int64_t)))
+    call check(alfq_init(ctx))
+    call check(alfq_sweep(ctx, 2_c_int32_t))
+    call check(alfq_fields_get(ctx, f, int(n_op, c_int32_t), int(ltrot, c_int32_t)))
+    call check(alfq_green_get(ctx, GR, int(n, c_int32_t)))
+    call check(alfq_phase_get(ctx, phase))
+    call check(alfq_control_get(ctx, ctl))
+    call check(alfq_tau_get(ctx, ts, c_loc(tlatt), c_loc(tden0)))
+    call check(alfq_ctx_destroy(ctx))
+
+    open(newunit=unit_out, file=trim(cb_file), status='replace', action='write')
+    do nt = 1, ltrot
+      do i = 1, n_op
+        write(unit_out, '(I3)') f(i, nt)
+      enddo
+    enddo
+    do j = 1, n
+      do i = 1, n
+        write(unit_out, '(2ES26.17)') real(GR(i, j)), aimag(GR(i, j))
+      enddo
+    enddo
+    write(unit_out, '(2ES26.17)') real(phase), aimag(phase)
+    write(unit_out, '(2I12)') ctl%nc_up, ctl%acc_up
+    write(unit_out, '(2I12)') ts%n, ts%ncg_tau
+    write(unit_out, '(2ES26.17)') ts%sign, ts%xmax_tau
+    do q = 1, 3
+      do nt = 0, ltrot
+        do d = 1, n
+          write(unit_out, '(2ES26.17)') tlatt(1, d, nt, q), tlatt(2, d, nt, q)
+        enddo
+      enddo
+    enddo
+    write(unit_out, '(2ES26.17)') tden0(1), tden0(2)
+    close(unit_out)
+    print '(A,I4,A,ES10.2)', 'alfq_ring_demo: checkerboard TAU_M calls ', ts%n, '  Precision tau max ', ts%xmax_tau
+  end subroutine
+
+  ! the 2 x 2 bond matrix e on sites s(1), s(2) of the n x n identity
+  subroutine embed(e, s, M)
+    complex(c_double_complex), intent(in) :: e(2, 2)
+    integer(c_int32_t), intent(in) :: s(2)
+    complex(c_double_complex), intent(out) :: M(n, n)
+    integer :: k
+    M = 0
+    do k = 1, n
+      M(k, k) = 1
+    enddo
+    M(s(1), s(1)) = e(1, 1); M(s(2), s(1)) = e(2, 1)
+    M(s(1), s(2)) = e(1, 2); M(s(2), s(2)) = e(2, 2)
+  end subroutine
 
   subroutine check(status)
     integer(c_int), intent(in) :: status
