@@ -246,6 +246,16 @@ __device__ __forceinline__ cplx renorm_pow2(cplx z) {
   return make_double2(ldexp(z.x, -e), ldexp(z.y, -e));
 }
 
+// the complex value of lane l (l uniform) in every lane
+__device__ __forceinline__ cplx lane_c(cplx v, int l) {
+  const long long re = __builtin_bit_cast(long long, v.x), im = __builtin_bit_cast(long long, v.y);
+  const long long r = ((long long)(unsigned)__builtin_amdgcn_readlane((int)(re >> 32), l) << 32) |
+                      (unsigned)__builtin_amdgcn_readlane((int)re, l);
+  const long long i = ((long long)(unsigned)__builtin_amdgcn_readlane((int)(im >> 32), l) << 32) |
+                      (unsigned)__builtin_amdgcn_readlane((int)im, l);
+  return make_double2(__builtin_bit_cast(double, r), __builtin_bit_cast(double, i));
+}
+
 __device__ __forceinline__ void block_sync_lds() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   __builtin_amdgcn_s_barrier();
@@ -465,8 +475,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
     __syncthreads();
-    // every lane l of every wave holds the site index of visit l, read with readlane
-    const int loc_lane = vr[r0 + (ga < nb ? ga : 0)].loc;
     // X[S, j] = sum_l G[S, A_l] C(l, j),  Y[S, j] = sum_l Yhat[S, A_l] D(l, j)   (zero for j >= m)
     {
       cplx ax[4], ay[4];
@@ -519,75 +527,74 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
 
     // ---- the visits: decisions from g, rank-1 updates of g on acceptance ----
+    // A rejected proposal leaves g and the phase unchanged, so the decisions of all visits
+    // up to the next acceptance follow from the same g: lane j (= ga) decides visit j of
+    // the block with the record and diagonal entry of that visit, and the first visit at
+    // or after k whose test passes is the next acceptance (every one before it is
+    // rejected).  Each lane also forms its visit's x = Delta / (1 + v_P Delta), so the
+    // division runs beside the test.  The arithmetic of each decision is that of a
+    // sequential visit with the same g(p, p) and phase.
     int buf = 0, mb = 0, k = 0;
-    BVisit wA = vr[r0], wB;
-    cplx d = gs[0][wA.loc * (SB_V + 1)];
-    // One visit with record CUR; loads the next record into NXT and, speculating a
-    // rejection, the next diagonal entry.  Leaves the loop when the round is full.
-#define ALFQ_VISIT(CUR, NXT)                                                                     \
-    {                                                                                          \
-      const int kn = k + 1 < nb ? k + 1 : nb - 1;                                              \
-      NXT = vr[r0 + kn];                                                                       \
-      const int pn = __builtin_amdgcn_readlane(loc_lane, kn);                                  \
-      const cplx dn_rej = gs[buf][pn * (SB_V + 1)];                                            \
-      const int p = CUR.loc;                                                                   \
-      const cplx ratio = make_double2(fma(-CUR.cb.x, d.x, fma(CUR.cb.y, d.y, CUR.ca.x)),         \
-                                      fma(-CUR.cb.x, d.y, fma(-CUR.cb.y, d.x, CUR.ca.y)));       \
-      cplx rt = ratio;                                                                         \
-      if (a.n_sun == 2) rt = cmul(ratio, ratio);                                               \
-      else for (int s = 1; s < a.n_sun; ++s) rt = cmul(rt, ratio);                             \
-      const double lhs = fabs(fma(z.x, rt.x, -z.y * rt.y));                                    \
-      const int accept = __builtin_amdgcn_readfirstlane(lhs > CUR.thr * fabs(z.x) ? 1 : 0);    \
-      ++k;                                                                                     \
-      if (accept) {                                                                            \
-        z = renorm_pow2(cmul(z, rt));                                                          \
-        /* x = Delta / (1 + v_P Delta), v_P = 1 - G_cur(P,P) */                                \
-        const cplx delta = CUR.delta;                                                          \
-        const cplx den = make_double2(1.0 + (1.0 - d.x) * delta.x + d.y * delta.y,             \
-                                      (1.0 - d.x) * delta.y - d.y * delta.x);                  \
-        const double rden = 1.0 / cabs2(den);                                                  \
-        const cplx x = cscale(cmul(delta, cconj(den)), rden);                                  \
-        const cplx mx = make_double2(-x.x, -x.y);                                              \
-        const cplx gcol = gs[buf][ga + SB_V * p];                                              \
-        const cplx g_np = gs[buf][pn + SB_V * p];                                              \
-        const cplx g_pn = gs[buf][p + SB_V * pn];                                              \
-        cplx gpb[4];                                                                           \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q) gpb[q] = gs[buf][p + SB_V * (gq + 8 * q)]; \
-        const cplx ua = cmul(mx, gcol);                                                        \
-        cplx wb[4];                                                                            \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                        \
-          const int b = gq + 8 * q;                                                            \
-          wb[q] = make_double2((b == p ? 1.0 : 0.0) - gpb[q].x, -gpb[q].y);                    \
-          gown[q] = cfma(ua, wb[q], gown[q]);                                                  \
-          gs[buf ^ 1][ga + SB_V * b] = gown[q];                                                \
-        }                                                                                      \
-        if (gq == 0) Us[mb * SB_V + ga] = ua;                                                  \
-        if (ga == 0) {                                                                         \
-          _Pragma("unroll") for (int q = 0; q < 4; ++q) Ws[mb * SB_V + gq + 8 * q] = wb[q];    \
-          if (tid == 0) {                                                                      \
-            const int2 ax = vaux[r0 + k - 1];                                                  \
-            a.f[(long long)a.slice * a.n_op + ax.y] = (int8_t)ax.x;                            \
-            s_x[m + mb] = x;                                                                   \
-            s_aP[m + mb] = CUR.P;                                                              \
-            s_aloc[mb] = p;                                                                    \
-          }                                                                                    \
-        }                                                                                      \
-        /* the next visit's diagonal entry after this update, as its owner computes it */      \
-        d = cfma(cmul(mx, g_np), make_double2((p == pn ? 1.0 : 0.0) - g_pn.x, -g_pn.y), dn_rej); \
-        mb += 1;                                                                               \
-        buf ^= 1;                                                                              \
-        block_sync_lds();                                                                      \
-        if (m + mb >= kmax || mb >= SB_A) break;                                               \
-      } else {                                                                                 \
-        d = dn_rej;                                                                            \
-      }                                                                                        \
-    }
+    const BVisit rec = vr[r0 + (ga < nb ? ga : nb - 1)];
+    const bool lane_live = ga < nvis;
     while (k < nvis) {
-      ALFQ_VISIT(wA, wB)
-      if (k >= nvis) break;
-      ALFQ_VISIT(wB, wA)
+      const cplx d = gs[buf][rec.loc * (SB_V + 1)];
+      const cplx ratio = make_double2(fma(-rec.cb.x, d.x, fma(rec.cb.y, d.y, rec.ca.x)),
+                                      fma(-rec.cb.x, d.y, fma(-rec.cb.y, d.x, rec.ca.y)));
+      cplx rt = ratio;
+      if (a.n_sun == 2) rt = cmul(ratio, ratio);
+      else for (int s = 1; s < a.n_sun; ++s) rt = cmul(rt, ratio);
+      const double lhs = fabs(fma(z.x, rt.x, -z.y * rt.y));
+      // x = Delta / (1 + v_P Delta), v_P = 1 - G_cur(P,P)
+      const cplx delta = rec.delta;
+      const cplx den = make_double2(1.0 + (1.0 - d.x) * delta.x + d.y * delta.y,
+                                    (1.0 - d.x) * delta.y - d.y * delta.x);
+      const double rden = 1.0 / cabs2(den);
+      const cplx xl = cscale(cmul(delta, cconj(den)), rden);
+      // keep the division here, beside the test, instead of after the ballot
+      __asm__ volatile("" ::"v"(xl.x), "v"(xl.y));
+      const bool pass = lane_live && ga >= k && lhs > rec.thr * fabs(z.x);
+      const unsigned lo = (unsigned)__ballot(pass);   // lanes 0..31: visits 0..31 of the block
+      if (lo == 0) {
+        k = nvis;
+        break;
+      }
+      const int js = __builtin_amdgcn_readfirstlane(__builtin_ctz(lo));
+      k = js + 1;
+      z = renorm_pow2(cmul(z, lane_c(rt, js)));
+      const cplx x = lane_c(xl, js);
+      const int p = __builtin_amdgcn_readlane(rec.loc, js);
+      const cplx mx = make_double2(-x.x, -x.y);
+      const cplx gcol = gs[buf][ga + SB_V * p];
+      cplx gpb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gpb[q] = gs[buf][p + SB_V * (gq + 8 * q)];
+      const cplx ua = cmul(mx, gcol);
+      cplx wb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int b = gq + 8 * q;
+        wb[q] = make_double2((b == p ? 1.0 : 0.0) - gpb[q].x, -gpb[q].y);
+        gown[q] = cfma(ua, wb[q], gown[q]);
+        gs[buf ^ 1][ga + SB_V * b] = gown[q];
+      }
+      if (gq == 0) Us[mb * SB_V + ga] = ua;
+      if (ga == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ws[mb * SB_V + gq + 8 * q] = wb[q];
+        if (tid == 0) {
+          const int2 ax = vaux[r0 + js];
+          a.f[(long long)a.slice * a.n_op + ax.y] = (int8_t)ax.x;
+          s_x[m + mb] = x;
+          s_aP[m + mb] = vr[r0 + js].P;
+          s_aloc[mb] = p;
+        }
+      }
+      mb += 1;
+      buf ^= 1;
+      block_sync_lds();
+      if (m + mb >= kmax || mb >= SB_A) break;
     }
-#undef ALFQ_VISIT
     nvisits += k;
     naccepts += mb;
     pos += k;
