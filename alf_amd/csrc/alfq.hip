@@ -393,16 +393,21 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
     hipLaunchKernelGGL(chain_reset_kernel, dim3(1, 1, c->B), dim3(1), 0, c->s, c->st, c->cs);
     HIP_OK(hipGetLastError());
   }
+  // X = G(:, A) W: the delayed-update columns of a round from its sites and W = C D^T.  When
+  // it runs on the direct-load kernel (few chains), that kernel also copies the rows A of G
+  // into Yhat, spread over its grid, in place of the block kernel's single workgroup.
+  GemmArgs gx = gemm_args(c->n, c->K, c->K, c->G, c->n, OP_N, true, c->Wk, CHAIN_W_LD, OP_N, true, c->X, c->n,
+                          ONE, ZERO);
+  gx.amap = c->amapk;
+  gx.kdev = &c->st->m;
+  gx.nchain = c->B;
+  a.yhat_external = reg_kernel && gemm_direct_path(gx) ? 1 : 0;
+  if (a.yhat_external) gx.yhat = c->Y;
   for (int r = 0; r < rounds; ++r) {
     a.first_round = (r == 0);
     a.final_round = (r == rounds - 1);
     if ((st = launch_chain(a, c->s, &c->chain_timer))) return st;
     if (reg_kernel) {
-      // X = G(:, A) W: the delayed-update columns of the round from its sites and W = C D^T
-      GemmArgs gx = gemm_args(c->n, c->K, c->K, c->G, c->n, OP_N, true, c->Wk, CHAIN_W_LD, OP_N, true, c->X, c->n,
-                              ONE, ZERO);
-      gx.amap = c->amapk;
-      gx.kdev = &c->st->m;
       if ((st = gemm_c(c, gx))) return st;
     }
     // G += X Y^T with the rank read from the chain state

@@ -43,6 +43,9 @@ struct ChainArgs {
   ChainState* st;
   // block kernel outputs for the flush G += (G(:, amap) W) Y^T: W (32 x 32, ld 32), site list
   cplx* W; int* amap;
+  // nonzero: the round's X = G(:, amap) W product writes Yhat (GemmArgs::yhat), so the block
+  // kernel leaves Y alone except in its own in-place flush
+  int yhat_external;
   // batches: byte stride between the chains' arenas; G, X, Y, f, the coupling tables
   // (op_myexp, op_expalpha, gama_ratio), u, st, W and amap are per chain, op_site,
   // op_type and uoff are shared

@@ -132,12 +132,18 @@ struct GemmArgs {
   int nchain;               // chains of the batch (grid z = batch * nchain)
   long long cstride;        // byte stride between the chains' arenas (applies to every pointer)
   int prec;                 // 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA and accumulation
+  // direct-load kernel only (gemm_direct_path): with amap set and a complex opA = A (OP_N),
+  // also write yhat(r, j) = delta(r, amap[j]) - A(amap[j], r) for r < M, j < K (ld M), the
+  // rows amap of A read before any later kernel changes A
+  cplx* yhat;
 };
 
 GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_cplx,
                    const void* B, int ldb, int opB, bool b_cplx, void* C, int ldc,
                    cplx alpha, cplx beta);
 int gemm_launch(const GemmArgs& g, hipStream_t s);
+// true when gemm_launch runs g on the direct-load kernel (the only one that honours yhat)
+bool gemm_direct_path(const GemmArgs& g);
 // G += X Y^T (all n x n / n x m, column-major, ld n) with the rank m read from *mdev and
 // clamped to kmax <= 32, for the nchain chains of a batch (pointers advance by cs bytes).
 int flush_launch(cplx* G, const cplx* X, const cplx* Y, const int* mdev, int n, int kmax, int nchain,

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   // ---- the round's flush operands: W = C D^T, the site list, Yhat ----
   form_w();
-  form_yhat();
+  if (!a.yhat_external) form_yhat();
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) a.W[ga + SB_K * (gq + 8 * q)] = gs[1][ga + SB_K * (gq + 8 * q)];

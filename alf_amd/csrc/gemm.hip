@@ -55,6 +55,7 @@ __device__ __forceinline__ void chain_operands(GemmArgs& g) {
   g.A = mv(g.A); g.B = mv(g.B); g.C = (void*)mv((const char*)g.C);
   g.rs = mv(g.rs); g.cs = mv(g.cs); g.bs = mv(g.bs); g.rs_real = mv(g.rs_real);
   g.amap = mv(g.amap); g.bmap = mv(g.bmap); g.kdev = mv(g.kdev);
+  g.yhat = mv(g.yhat);
 }
 
 // One K stage of an LDS-tiled kernel's operands, held in registers from its global loads to
@@ -559,6 +560,19 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     const int kd = *g.kdev;
     K = kd < K ? kd : K;
   }
+  if (A_CPLX && g.yhat && g.amap) {
+    // the side copy of rows amap of A, spread over every thread of the grid
+    const int nwg = gridDim.x * gridDim.y;
+    const long long total = (long long)g.M * (K > 0 ? K : 0);
+    const cplx* Ag = (const cplx*)g.A;
+    for (long long e = (long long)(blockIdx.x + gridDim.x * blockIdx.y) * blockDim.x + tid; e < total;
+         e += (long long)nwg * blockDim.x) {
+      const int r = (int)(e % g.M), j = (int)(e / g.M);
+      const int P = g.amap[j];
+      const cplx v = Ag[P + (long long)r * g.lda];
+      g.yhat[e] = make_double2((r == P ? 1.0 : 0.0) - v.x, -v.y);
+    }
+  }
   if (K <= 0 && g.beta.x == 1.0 && g.beta.y == 0.0) return;  // nothing to add
   const double* Ar = (const double*)g.A;
   const cplx* Ac = (const cplx*)g.A;
@@ -743,11 +757,19 @@ static int gemm_tile_shape(int M, int N, int K, int batch) {
   return opts[best].code;
 }
 
+bool gemm_direct_path(const GemmArgs& g) {
+  return g.M > 0 && g.N > 0 && cdivl(g.M, 32) * cdivl(g.N, 32) * (long long)gemm_zdim(g) < 256;
+}
+
 int gemm_launch(const GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return ST_NONE;
   if (gemm_zdim(g) > 65535) { alfq_set_error("gemm_launch: batch too large"); return ST_GENERIC; }
+  if (g.yhat && !(gemm_direct_path(g) && g.amap && g.a_cplx && g.opA == OP_N && g.batch <= 1)) {
+    alfq_set_error("gemm_launch: the row copy yhat needs the direct kernel, a complex opA = A and amap");
+    return ST_GENERIC;
+  }
   const long long b = gemm_zdim(g);
-  if (cdivl(g.M, 32) * cdivl(g.N, 32) * b < 256) {
+  if (gemm_direct_path(g)) {
     // too few 32 x 32 tiles to fill the device: 16 x 16 tiles, K split over up to 8 waves
     const int ksteps = g.K > 0 ? (g.K + 3) / 4 : 1;
     if (ksteps >= 32) launch_direct<8>(g, s);
