@@ -536,6 +536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // sequential visit with the same g(p, p) and phase.
     int buf = 0, mb = 0, k = 0;
     const BVisit rec = vr[r0 + (ga < nb ? ga : nb - 1)];
+    const int2 rec_aux = vaux[r0 + (ga < nb ? ga : nb - 1)];
     const bool lane_live = ga < nvis;
     while (k < nvis) {
       const cplx d = gs[buf][rec.loc * (SB_V + 1)];
@@ -564,6 +565,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       z = renorm_pow2(cmul(z, lane_c(rt, js)));
       const cplx x = lane_c(xl, js);
       const int p = __builtin_amdgcn_readlane(rec.loc, js);
+      const int P_acc = __builtin_amdgcn_readlane(rec.P, js);
+      const int f_new = __builtin_amdgcn_readlane(rec_aux.x, js), op_acc = __builtin_amdgcn_readlane(rec_aux.y, js);
       const cplx mx = make_double2(-x.x, -x.y);
       const cplx gcol = gs[buf][ga + SB_V * p];
       cplx gpb[4];
@@ -583,10 +586,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
         for (int q = 0; q < 4; ++q) Ws[mb * SB_V + gq + 8 * q] = wb[q];
         if (tid == 0) {
-          const int2 ax = vaux[r0 + js];
-          a.f[(long long)a.slice * a.n_op + ax.y] = (int8_t)ax.x;
+          a.f[(long long)a.slice * a.n_op + op_acc] = (int8_t)f_new;
           s_x[m + mb] = x;
-          s_aP[m + mb] = vr[r0 + js].P;
+          s_aP[m + mb] = P_acc;
           s_aloc[mb] = p;
         }
       }
