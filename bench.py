@@ -35,14 +35,15 @@ def algorithmic_flops_per_sweep(n, L, S, acc, complex_t=False):
     return n ** 3 * (6 * L * cT + 117.3 * S) + 16 * acc * L * n ** 3
 
 
-def attainable_sweep_seconds(n, L, S, acc, complex_t=False, peak=78.6e12, bw=8.0e12):
+def attainable_sweep_seconds(n, L, S, acc, complex_t=False, peak=78.6e12, bw=8.0e12, delay=1):
     """Time-weighted roofline of one sweep: sum over the kernel classes of max(F_k / P, B_k / BW)
     with the per-sweep work model of the reference's algorithm: wraps F = 6 L c_T n^3,
     B = 2 L (3 (T_B + 32) + 64) n^2; rank-1 updates F = 16 acc L n^3, B = 64 acc L n^3 (G streamed
-    once per update); QR and solves F = 117.3 S n^3."""
+    once per update) divided by `delay` for updates delayed to rank-`delay` flushes (G streamed
+    once per flush); QR and solves F = 117.3 S n^3."""
     cT, TB = (8, 16) if complex_t else (4, 8)
     t_wrap = max(6 * L * cT * n ** 3 / peak, 2 * L * (3 * (TB + 32) + 64) * n ** 2 / bw)
-    t_rank1 = max(16 * acc * L * n ** 3 / peak, 64 * acc * L * n ** 3 / bw)
+    t_rank1 = max(16 * acc * L * n ** 3 / peak, 64 * acc * L * n ** 3 / (delay * bw))
     t_qr = 117.3 * S * n ** 3 / peak
     return t_wrap + t_rank1 + t_qr
 
@@ -288,7 +289,9 @@ def main():
                    "wall_s": wall,
                    "algorithmic_gflop_per_sweep": algorithmic_flops_per_sweep(n, L, S, acc) / 1e9,
                    "sweep_fraction_of_fp64_peak": algorithmic_flops_per_sweep(n, L, S, acc) * sweeps_per_s / 78.6e12,
-                   "sweep_roofline_fraction": attainable_sweep_seconds(n, L, S, acc) * sweeps_per_s},
+                   "sweep_roofline_fraction": attainable_sweep_seconds(n, L, S, acc) * sweeps_per_s,
+                   "sweep_roofline_fraction_delayed": attainable_sweep_seconds(n, L, S, acc, delay=args.delay) *
+                                                      sweeps_per_s},
         "roofline": kb,
     }
     eng.close()
@@ -297,9 +300,11 @@ def main():
         result["with_time_displaced"] = tau_rate(ham, fields, args.delay, max(2, args.steps // 2))
     if args.batch > 0:
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
-        # above 1: the delayed updates keep the rank-1 traffic out of HBM, which the model charges
-        result["batched"]["sweep_roofline_fraction"] = (attainable_sweep_seconds(n, L, S, acc) *
-                                                        result["batched"]["aggregate_sweeps_per_s"])
+        # above 1: the delayed updates keep the rank-1 traffic out of HBM, which the model charges;
+        # the delayed form prices the rank-32 flushes the engine runs
+        agg = result["batched"]["aggregate_sweeps_per_s"]
+        result["batched"]["sweep_roofline_fraction"] = attainable_sweep_seconds(n, L, S, acc) * agg
+        result["batched"]["sweep_roofline_fraction_delayed"] = attainable_sweep_seconds(n, L, S, acc, delay=32) * agg
     # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64
     from alf_amd.runtime import PREC_FP32_WRAPS
     mixed = None if args.no_mixed else {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) with FP32 operands, "
