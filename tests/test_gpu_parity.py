@@ -355,6 +355,35 @@ def test_batched_chains_at_full_tile_grids(rt):
     b.close()
 
 
+def test_batched_chains_rows_copied_by_the_x_product(rt):
+    """16 chains of Ndim = 256: the round's X = G(:, A) W product still runs on the direct-load
+    kernel (it also copies the rows A of G for the flush), while the flush G += X Yhat^T runs on
+    the batched flush kernel.  Chains 0, 7 and 15 follow their single-context trajectories."""
+    hams = [hubbard(L1=16, L2=16, beta=1.0, nwrap=10, U=4.0 if z % 2 == 0 else 3.0) for z in range(16)]
+    fs = [fields_for(h, 300 + z) for z, h in enumerate(hams)]
+    b = rt.BatchEngine(hams, delay=32)
+    for z, f in enumerate(fs):
+        b.chain(z).set_fields(f)
+        b.chain(z).set_philox(9, z)
+    b.init()
+    b.sweep(1)
+    for z in (0, 7, 15):
+        e = rt.Engine(hams[z], delay=32)
+        e.set_fields(fs[z])
+        e.set_philox(9, z)
+        e.init()
+        e.sweep(1)
+        b.chain(z)
+        assert np.array_equal(b.get_fields(), e.get_fields())
+        assert rel(b.get_green(), e.get_green()) < TOL
+        assert abs(b.get_phase() - e.get_phase()) < TOL
+        cb, ce = b.control(), e.control()
+        assert cb["ACC_up"] == ce["ACC_up"] and cb["NC_up"] == ce["NC_up"]
+        assert cb["XMAXG"] < 1e-8
+        e.close()
+    b.close()
+
+
 # ---------------------------------------------------------------- FP32 hopping wraps
 @pytest.mark.parametrize("shape", [(16, 16, 16), (37, 50, 29), (256, 256, 256), (256, 224, 32), (32, 224, 256),
                                    (300, 17, 5), (2048, 1024, 48)])
