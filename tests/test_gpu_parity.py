@@ -157,6 +157,29 @@ def test_sweeps_match_oracle(rt, L1, L2, beta, nwrap, delay):
     e.close()
 
 
+def test_high_acceptance_sweeps_match_oracle(rt):
+    """U = 0.5: about half of the proposals are accepted (16 per block of 32 visits on
+    average), so many blocks end at their 16-acceptance limit and rounds fill in mid-block;
+    fields, G, PHASE and the counters still follow the oracle's sequential visits."""
+    ham = hubbard(L1=6, L2=6, beta=2.0, nwrap=10, U=0.5)
+    f = fields_for(ham, 0)
+    u = tape(2_000_000)
+    s = oracle(ham, f, u)
+    e = engine(rt, ham, f, u, delay=32)
+    s.init()
+    e.init()
+    for _ in range(2):
+        s.sweep()
+        e.sweep(1)
+        assert np.array_equal(e.get_fields(), s.f)
+        assert rel(e.get_green(), s.G) < TOL
+        assert abs(e.get_phase() - s.phase) < TOL
+    c = e.control()
+    assert c["NC_up"] == s.ctrl.NC_up and c["ACC_up"] == s.ctrl.ACC_up
+    assert c["ACC_up"] > 0.45 * c["NC_up"]
+    e.close()
+
+
 def test_complex_hopping_flux_sweep(rt):
     """Honeycomb lattice with a Peierls flux: complex exp(-dtau T) (CmplxExpOpT path)."""
     ham = hubbard(L1=3, L2=3, beta=2.0, U=3.0, nwrap=8, lattice="Honeycomb", n_phi=1)
