@@ -1627,8 +1627,13 @@ int alfq_green_set(alfq_ctx* c, const double* G, int32_t ld) {
   NULL_STREAM_LOCK;
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy2D(at(c, c->G), c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
-                     hipMemcpyHostToDevice));
+  // a contiguous G (ld = n, the shims' case) moves as one copy: a 2-D copy from pageable memory
+  // is staged row by row
+  if (ld == c->n)
+    HIP_OK(hipMemcpy(at(c, c->G), G, (size_t)c->n * c->n * sizeof(cplx), hipMemcpyHostToDevice));
+  else
+    HIP_OK(hipMemcpy2D(at(c, c->G), c->n * sizeof(cplx), G, (size_t)ld * sizeof(cplx), c->n * sizeof(cplx), c->n,
+                       hipMemcpyHostToDevice));
   return ST_NONE;
 }
 
@@ -1636,8 +1641,11 @@ int alfq_green_get(alfq_ctx* c, double* G, int32_t ld) {
   NULL_STREAM_LOCK;
   if (!c || !G || ld < c->n) return ST_GENERIC;
   HIP_OK(hipStreamSynchronize(c->s));
-  HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), at(c, c->G), c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
-                     hipMemcpyDeviceToHost));
+  if (ld == c->n)
+    HIP_OK(hipMemcpy(G, at(c, c->G), (size_t)c->n * c->n * sizeof(cplx), hipMemcpyDeviceToHost));
+  else
+    HIP_OK(hipMemcpy2D(G, (size_t)ld * sizeof(cplx), at(c, c->G), c->n * sizeof(cplx), c->n * sizeof(cplx), c->n,
+                       hipMemcpyDeviceToHost));
   return ST_NONE;
 }
 

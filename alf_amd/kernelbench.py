@@ -22,6 +22,12 @@ The ``sweep`` mode runs sweeps of the 16 x 16 U = 4 beta = 10 Hubbard workload, 
 and then a batch of chains, for per-kernel profiles of either hopping form:
 
     python3 -m alf_amd.kernelbench sweep [--checkerboard] [--tau] [--batch B] [--steps K]
+
+The ``shim`` mode times the up-pass slices of the same workload called one at a time, as
+main.F90 calls WRAPGRUP: with G kept on the device, and with G uploaded and downloaded
+around every call (the alfq_WRAPGRUP_shim form, whose rate includes PCIe):
+
+    python3 -m alf_amd.kernelbench shim
 """
 from __future__ import annotations
 
@@ -133,6 +139,43 @@ def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False
         print(f"batch of {batch}: {steps} sweeps, {ms / steps:.2f} ms per batched sweep", flush=True)
 
 
+def shim_slices() -> None:
+    """Per-slice time of alfq_wrapgrup with G on the device and with G crossing PCIe both
+    ways around every call (PHASE is returned by every call in both forms)."""
+    import time
+
+    from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+    from alf_amd.runtime import Engine
+    ham = hubbard_ham_set(HubbardParams(Lattice_type="Square", L1=16, L2=16, Ham_U=4.0, Beta=10.0, Dtau=0.1,
+                                        Nwrap=10))
+    ham.nsigma.set_uniform_four_state(0)
+    eng = Engine(ham, delay=32)
+    eng.set_fields(ham.nsigma.f.copy())
+    eng.set_philox(seed=0, stream=0)
+    eng.init()
+    eng.sync()
+    G, ph = eng.get_green(), eng.get_phase()
+    L = ham.Ltrot
+    t = time.perf_counter()
+    for nt in range(L):
+        ph = eng.wrapgrup(nt, ph)
+    eng.sync()
+    dev = (time.perf_counter() - t) / L
+    eng.set_green(G)
+    t = time.perf_counter()
+    for nt in range(L):
+        eng.set_green(G)
+        ph = eng.wrapgrup(nt, ph)
+        G = eng.get_green()
+    eng.sync()
+    host = (time.perf_counter() - t) / L
+    eng.close()
+    mb = 2 * 16 * ham.Ndim ** 2 / 1e6
+    print(f"WRAPGRUP per slice, {L} slices: G on the device {dev * 1e6:.1f} us; G to and from the host "
+          f"{host * 1e6:.1f} us ({mb:.2f} MB per call, {mb / 1e3 / max(host - dev, 1e-9):.1f} GB/s over the "
+          f"difference)", flush=True)
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "copy":
         mib = int(sys.argv[sys.argv.index("--mib") + 1]) if "--mib" in sys.argv else 256
@@ -140,5 +183,7 @@ if __name__ == "__main__":
     elif len(sys.argv) >= 2 and sys.argv[1] == "sweep":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
         profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3), "--tau" in sys.argv)
+    elif len(sys.argv) >= 2 and sys.argv[1] == "shim":
+        shim_slices()
     else:
         print(__doc__)
