@@ -916,7 +916,9 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
   HIP_OK(hipGetLastError());
   const cplx one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
   for (int k = 0; k < n; k += nb) {
-    if (n <= RP && nb <= 32) {
+    // the register panel takes any panel of at most RP rows (the kernel works on rows k..n-1
+    // with leading dimension n), so the later panels of a larger matrix use it too
+    if (n - k <= RP && nb <= 32) {
       static std::once_flag once_r;
       static hipError_t attr_r = hipSuccess;
       std::call_once(once_r, [] {
