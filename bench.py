@@ -66,11 +66,12 @@ def cpu_baseline(ham, fields, seconds_budget=30.0):
                       f"(ZGEQP3 pivoting), one thread, {dt:.1f} s"}
 
 
-def batched_rate(chains, steps, prec=0, checkerboard=False):
-    """Aggregate sweeps/s of `chains` independent chains of the workload in one batched context."""
+def batched_rate(chains, steps, prec=0, checkerboard=False, params=None):
+    """Aggregate sweeps/s of `chains` independent chains of the workload (or of the model
+    `params` names) in one batched context."""
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import BatchEngine
-    ham = hubbard_ham_set(HubbardParams(**WORKLOAD, Checkerboard=checkerboard))
+    ham = hubbard_ham_set(HubbardParams(**(WORKLOAD if params is None else params), Checkerboard=checkerboard))
     b = BatchEngine([ham] * chains, delay=32)
     b.set_precision(prec)
     for z in range(chains):
@@ -189,6 +190,11 @@ def config_rates(delay, cpu=True):
             row["checkerboard" if cb else "dense"] = {k: r[k] for k in ("sweeps_per_s", "precision_green_max")}
             if cpu and not cb and ham.Ndim <= 16:
                 row["cpu_oracle_sweeps_per_s"] = cpu_baseline(ham, f)["value"]
+        if kw["L1"] * kw["L2"] <= 16:
+            # a lattice this small leaves a single chain latency-bound: many chains per launch
+            r = batched_rate(256, 5, params=kw)
+            row["dense_batch_256"] = {k: r[k] for k in ("aggregate_sweeps_per_s", "ms_per_batched_sweep",
+                                                        "precision_green_max")}
         row["ndim"] = ham.Ndim
         row["ltrot"] = ham.Ltrot
         out[name] = row
