@@ -269,6 +269,8 @@ struct alfq_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   KTimer chain_timer;
   int rounds = 1 << 30;                  // chain rounds per slice (adapted per sweep)
+  int* stat_dev = nullptr;               // 3 words per chain read after every sweep
+  int* stat_host = nullptr;              // their pinned host copy
   bool uniforms_ready = false;           // a whole sweep's uniforms are already on the device
   // sweeps replayed as HIP graphs, keyed by the host state they start from
   struct HostState {
@@ -425,27 +427,60 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   return ST_NONE;
 }
 
-// Choose the number of chain rounds per slice from the largest accepted count seen in
-// the previous sweep; reads one integer from the device.
-int adapt_rounds(alfq_ctx* c) {
-  std::vector<int> mxz(c->B, 0);
-  for (int z = 0; z < c->B; ++z) {
-    int* p = (int*)((char*)&c->st->max_slice_acc + (long long)z * c->cs);
-    HIP_OK(hipMemcpyAsync(&mxz[z], p, sizeof(int), hipMemcpyDeviceToHost, c->s));
-  }
+// The words the host reads after a sweep, gathered over the chains into one small buffer:
+// [3z] the largest accepted count of a slice of chain z since the last read (then cleared
+// when clear_acc), [3z + 1] its random-stream status, [3z + 2] its Green-function status.
+__global__ void status_gather_kernel(ChainState* st, const DevScalars* sc, int* out, int clear_acc, long long cs) {
+  const int z = blockIdx.z;
+  st = chain_ptr(st, cs);
+  sc = chain_ptr(sc, cs);
+  out[3 * z] = st->max_slice_acc;
+  out[3 * z + 1] = st->status;
+  out[3 * z + 2] = sc->status;
+  if (clear_acc) st->max_slice_acc = 0;
+}
+
+int gather_status(alfq_ctx* c, bool clear_acc) {
+  hipLaunchKernelGGL(status_gather_kernel, dim3(1, 1, c->B), dim3(1), 0, c->s, c->st, c->sc, c->stat_dev,
+                     clear_acc ? 1 : 0, c->cs);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(c->stat_host, c->stat_dev, 3 * sizeof(int) * c->B, hipMemcpyDeviceToHost, c->s));
   HIP_OK(hipStreamSynchronize(c->s));
-  const int mx = *std::max_element(mxz.begin(), mxz.end());
-  const int zero = 0;
+  return ST_NONE;
+}
+
+// Terminate_on_error's conditions (Control_PrecisionG: NaN or XMAX > 10; an exhausted
+// random stream) of any chain, from the last gather_status.
+int status_error(const alfq_ctx* c) {
   for (int z = 0; z < c->B; ++z) {
-    int* p = (int*)((char*)&c->st->max_slice_acc + (long long)z * c->cs);
-    HIP_OK(hipMemcpyAsync(p, &zero, sizeof(int), hipMemcpyHostToDevice, c->s));
+    const std::string who = c->B > 1 ? "chain " + std::to_string(z) + ": " : "";
+    if (c->stat_host[3 * z + 1] == 1) { alfq_set_error(who + "random stream exhausted"); return ST_GENERIC; }
+    if (c->stat_host[3 * z + 2] == 1) {
+      alfq_set_error(who + "Green function contains NaN, calculation is being aborted!");
+      return ST_GENERIC;
+    }
+    if (c->stat_host[3 * z + 2] == 3) {
+      alfq_set_error(who + "XMAX is exceeding the threshold of 10 for G difference! Try with smaller Nwrap or dtau.");
+      return ST_UNSTABLE_MATRIX;
+    }
   }
+  return ST_NONE;
+}
+
+// After a sweep: stop at the first failing sweep (the reference terminates at once), and
+// choose the number of chain rounds per slice from the largest accepted count of a slice
+// seen in the sweep.
+int end_of_sweep(alfq_ctx* c) {
+  int st;
+  if ((st = gather_status(c, true))) return st;
+  if ((st = status_error(c))) return st;
+  int mx = 0;
+  for (int z = 0; z < c->B; ++z) mx = std::max(mx, c->stat_host[3 * z]);
   const int worst = c->n_op / c->K + 1;
   // enough rounds for the busiest slice seen; a slice that needs more is finished by the
   // last round's in-place flush
   const int want = (mx + c->K - 1) / c->K;
   c->rounds = std::max(1, std::min(worst, want));
-  HIP_OK(hipStreamSynchronize(c->s));
   return ST_NONE;
 }
 
@@ -897,27 +932,9 @@ T* at(const alfq_ctx* c, T* p) {
 }
 
 int check_status(alfq_ctx* c) {
-  // copies on the context's own stream: another context may be capturing a sweep meanwhile
-  std::vector<ChainState> cs(c->B);
-  std::vector<DevScalars> ds(c->B);
-  for (int z = 0; z < c->B; ++z) {
-    HIP_OK(hipMemcpyAsync(&cs[z], at_chain(c, c->st, z), sizeof(ChainState), hipMemcpyDeviceToHost, c->s));
-    HIP_OK(hipMemcpyAsync(&ds[z], at_chain(c, c->sc, z), sizeof(DevScalars), hipMemcpyDeviceToHost, c->s));
-  }
-  HIP_OK(hipStreamSynchronize(c->s));
-  for (int z = 0; z < c->B; ++z) {
-    const std::string who = c->B > 1 ? "chain " + std::to_string(z) + ": " : "";
-    if (cs[z].status == 1) { alfq_set_error(who + "random stream exhausted"); return ST_GENERIC; }
-    if (ds[z].status == 1) {
-      alfq_set_error(who + "Green function contains NaN, calculation is being aborted!");
-      return ST_GENERIC;
-    }
-    if (ds[z].status == 3) {
-      alfq_set_error(who + "XMAX is exceeding the threshold of 10 for G difference! Try with smaller Nwrap or dtau.");
-      return ST_UNSTABLE_MATRIX;
-    }
-  }
-  return ST_NONE;
+  int st;
+  if ((st = gather_status(c, false))) return st;
+  return status_error(c);
 }
 
 double phi_st(int s, int typ) {
@@ -1194,6 +1211,9 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
   for (int k = 1; k <= c->nstm; ++k)
     if (udv_reset_dev(c, &c->var_st[k], 'L')) return fail("udv");
   if (hipEventCreate(&c->ev0) || hipEventCreate(&c->ev1)) return fail("events");
+  if (hipMalloc(&c->stat_dev, 3 * sizeof(int) * nchain) ||
+      hipHostMalloc((void**)&c->stat_host, 3 * sizeof(int) * nchain, hipHostMallocDefault))
+    return fail("status buffers");
   if (hipStreamSynchronize(c->s)) return fail("sync");
   // ALFQ_SWEEP_GRAPHS=0 launches every kernel from the host (kernel tracers that do not
   // follow graph replays need this)
@@ -1244,10 +1264,12 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (c->tau.mem) hipFree(c->tau.mem);
   if (c->tau.acc) hipFree(c->tau.acc);
   if (c->cb_sites) hipFree(c->cb_sites);
-  void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape};
+  void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape,
+                  c->stat_dev};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& pr : c->chain_timer.ev) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+  if (c->stat_host) hipHostFree(c->stat_host);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->s) hipStreamDestroy(c->s);
@@ -1893,9 +1915,9 @@ int alfq_sweep(alfq_ctx* c, int32_t nsweep) {
   int st;
   for (int i = 0; i < nsweep; ++i) {
     if ((st = sweep_graph(c))) return st;
-    if ((st = adapt_rounds(c))) return st;
+    if ((st = end_of_sweep(c))) return st;
   }
-  return check_status(c);
+  return ST_NONE;
 }
 
 int alfq_sweep_many(alfq_ctx* const* ctxs, int32_t count, int32_t nsweep, double* ms) {
@@ -1911,9 +1933,8 @@ int alfq_sweep_many(alfq_ctx* const* ctxs, int32_t count, int32_t nsweep, double
     int st = ST_NONE;
     for (int k = 0; k < nsweep && st == ST_NONE; ++k) {
       st = sweep_graph(c);
-      if (st == ST_NONE) st = adapt_rounds(c);
+      if (st == ST_NONE) st = end_of_sweep(c);
     }
-    if (st == ST_NONE) st = check_status(c);
     status[i] = st;
     if (st) errors[i] = alfq_last_error();
   };
@@ -1936,12 +1957,13 @@ int alfq_sweep_timed(alfq_ctx* c, int32_t nsweep, double* ms) {
   for (int i = 0; i < nsweep; ++i)
     if ((st = sweep_graph(c))) return st;
   HIP_OK(hipEventRecord(c->ev1, c->s));
-  if ((st = adapt_rounds(c))) return st;
+  // a timed run reads the status once, after its sweeps
+  if ((st = end_of_sweep(c))) return st;
   HIP_OK(hipEventSynchronize(c->ev1));
   float t = 0.f;
   HIP_OK(hipEventElapsedTime(&t, c->ev0, c->ev1));
   *ms = t;
-  return check_status(c);
+  return ST_NONE;
 }
 
 int alfq_sync(alfq_ctx* c) {

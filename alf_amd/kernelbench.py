@@ -23,6 +23,15 @@ and then a batch of chains, for per-kernel profiles of either hopping form:
 
     python3 -m alf_amd.kernelbench sweep [--checkerboard] [--tau] [--batch B] [--steps K]
 
+The ``counters`` mode is the program for hardware-counter passes (rocprofv3 --pmc): one
+chain and then a batch of chains of the same workload, each initialised and swept once with
+every kernel launched from the host (no graph replay, no timing events), with a progress
+line on stderr before and after every step.  A counter pass serialises the dispatches it
+instruments, so the pass names the kernels to instrument (--kernel-include-regex) and the
+program issues a bounded number of dispatches (about 5000 per sweep of one chain):
+
+    python3 -m alf_amd.kernelbench counters [--batch B] [--checkerboard]
+
 The ``shim`` mode times the up-pass slices of the same workload called one at a time, as
 main.F90 calls WRAPGRUP: with G kept on the device, and with G uploaded and downloaded
 around every call (the alfq_WRAPGRUP_shim form, whose rate includes PCIe):
@@ -139,6 +148,53 @@ def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False
         print(f"batch of {batch}: {steps} sweeps, {ms / steps:.2f} ms per batched sweep", flush=True)
 
 
+def counter_run(batch: int, checkerboard: bool) -> None:
+    """Init and one eagerly launched sweep of one chain, then of `batch` chains, with
+    progress lines on stderr around every step."""
+    import time
+
+    from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+    from alf_amd.runtime import BatchEngine, Engine
+    t0 = time.perf_counter()
+
+    def say(msg):
+        print(f"[counters {time.perf_counter() - t0:7.2f} s] {msg}", file=sys.stderr, flush=True)
+
+    ham = hubbard_ham_set(HubbardParams(Lattice_type="Square", L1=16, L2=16, Ham_U=4.0, Beta=10.0, Dtau=0.1,
+                                        Nwrap=10, Checkerboard=checkerboard))
+    ham.nsigma.set_uniform_four_state(0)
+    say("single chain: create")
+    eng = Engine(ham, delay=32)
+    eng.sweep_graphs(False)
+    eng.set_fields(ham.nsigma.f.copy())
+    eng.set_philox(seed=0, stream=0)
+    say("single chain: init")
+    eng.init()
+    eng.sync()
+    say("single chain: sweep")
+    eng.sweep(1)
+    eng.sync()
+    c = eng.control()
+    say(f"single chain: done, {c['ACC_up']} accepted of {c['NC_up']} visits")
+    eng.close()
+    if batch > 0:
+        say(f"batch of {batch}: create")
+        b = BatchEngine([ham] * batch, delay=32)
+        b.sweep_graphs(False)
+        for z in range(batch):
+            ham.nsigma.set_uniform_four_state(z)
+            b.chain(z).set_fields(ham.nsigma.f.copy())
+            b.chain(z).set_philox(seed=z, stream=0)
+        say(f"batch of {batch}: init")
+        b.init()
+        b.sync()
+        say(f"batch of {batch}: sweep")
+        b.sweep(1)
+        b.sync()
+        say(f"batch of {batch}: done")
+        b.close()
+
+
 def shim_slices() -> None:
     """Per-slice time of alfq_wrapgrup with G on the device and with G crossing PCIe both
     ways around every call (PHASE is returned by every call in both forms)."""
@@ -183,6 +239,9 @@ if __name__ == "__main__":
     elif len(sys.argv) >= 2 and sys.argv[1] == "sweep":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
         profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3), "--tau" in sys.argv)
+    elif len(sys.argv) >= 2 and sys.argv[1] == "counters":
+        arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
+        counter_run(arg("--batch", 64), "--checkerboard" in sys.argv)
     elif len(sys.argv) >= 2 and sys.argv[1] == "shim":
         shim_slices()
     else:

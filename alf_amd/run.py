@@ -64,57 +64,58 @@ def simulate(params: HubbardParams, nsweep: int, nbin: int, directory: str = "."
     with open(lock, "w") as fh:
         fh.write(" ALF is running\n")
     t_start = time.time()
-    try:
-        ham = hubbard_ham_set(params)
-        eng = Engine(ham, delay=delay)
-        confin = os.path.join(directory, f"confin_{rank}")
-        stream = rank
-        if os.path.exists(confin):
-            line, f = read_conf(confin, ham.n_op, ham.Ltrot)
-            seed, stream, pos = line
-            eng.set_fields(f)
-            eng.set_philox(seed, stream)
-            eng.set_rng_position(pos)
-        else:
-            ham.nsigma.set_uniform_four_state(seed)
-            eng.set_fields(ham.nsigma.f.copy())
-            eng.set_philox(seed, stream)
-        eng.obs_setup(lobs=lobs)
-        eng.obs_enable(True)
+    # RUNNING stays in place when the run fails, as in the reference, which removes it only
+    # after a finished run (main.F90:961-965): the bins and confout of a failed run need checking
+    # before the next run appends to them.
+    ham = hubbard_ham_set(params)
+    eng = Engine(ham, delay=delay)
+    confin = os.path.join(directory, f"confin_{rank}")
+    stream = rank
+    if os.path.exists(confin):
+        line, f = read_conf(confin, ham.n_op, ham.Ltrot)
+        seed, stream, pos = line
+        eng.set_fields(f)
+        eng.set_philox(seed, stream)
+        eng.set_rng_position(pos)
+    else:
+        ham.nsigma.set_uniform_four_state(seed)
+        eng.set_fields(ham.nsigma.f.copy())
+        eng.set_philox(seed, stream)
+    eng.obs_setup(lobs=lobs)
+    eng.obs_enable(True)
+    if ltau == 1:
+        eng.tau_enable(True)
+    eng.init()
+    writer = BinWriter(directory, ham)
+    done = 0
+    truncated = False
+    for _ in range(nbin):
+        t_bin = time.time()
+        eng.obs_reset()
+        eng.sweep(nsweep)
+        writer.write(eng.obs_get())
         if ltau == 1:
-            eng.tau_enable(True)
-        eng.init()
-        writer = BinWriter(directory, ham)
-        done = 0
-        truncated = False
-        for _ in range(nbin):
-            t_bin = time.time()
-            eng.obs_reset()
-            eng.sweep(nsweep)
-            writer.write(eng.obs_get())
-            if ltau == 1:
-                writer.write_tau(eng.tau_get())
-            write_conf(os.path.join(directory, f"confout_{rank}"), eng.get_fields(),
-                       (seed, stream, eng.rng_position()))
-            done += 1
-            if cpu_max_hours > 0:
-                t_end = time.time()
-                remain = t_start + cpu_max_hours * 3600.0 - t_end
-                if remain / max(t_end - t_bin, 1e-9) < TRUNCATION_BINS:
-                    truncated = True
-                    break
-        c = eng.control()
-        eng.close()
-        elapsed = time.time() - t_start
-        info = {"bins": done, "truncated": truncated, "precision_green_mean": c["precision_green_mean"],
-                "precision_green_max": c["XMAXG"], "precision_phase_max": c["XMAXP"],
-                "acceptance": c["ACC_up"] / max(c["NC_up"], 1), "cpu_time_s": elapsed}
-        with open(os.path.join(directory, "info"), "a") as fh:
-            fh.write(f" Precision Green  Mean, Max :  {info['precision_green_mean']:.16E}  "
-                     f"{info['precision_green_max']:.16E}\n")
-            fh.write(f" Precision Phase, Max       :  {info['precision_phase_max']:.16E}\n")
-            fh.write(f" Acceptance                 :  {info['acceptance']:.16E}\n")
-            fh.write(f" CPU Time                   :  {elapsed:.16E}\n")
-        return info
-    finally:
-        os.remove(lock)
+            writer.write_tau(eng.tau_get())
+        write_conf(os.path.join(directory, f"confout_{rank}"), eng.get_fields(),
+                   (seed, stream, eng.rng_position()))
+        done += 1
+        if cpu_max_hours > 0:
+            t_end = time.time()
+            remain = t_start + cpu_max_hours * 3600.0 - t_end
+            if remain / max(t_end - t_bin, 1e-9) < TRUNCATION_BINS:
+                truncated = True
+                break
+    c = eng.control()
+    eng.close()
+    elapsed = time.time() - t_start
+    info = {"bins": done, "truncated": truncated, "precision_green_mean": c["precision_green_mean"],
+            "precision_green_max": c["XMAXG"], "precision_phase_max": c["XMAXP"],
+            "acceptance": c["ACC_up"] / max(c["NC_up"], 1), "cpu_time_s": elapsed}
+    with open(os.path.join(directory, "info"), "a") as fh:
+        fh.write(f" Precision Green  Mean, Max :  {info['precision_green_mean']:.16E}  "
+                 f"{info['precision_green_max']:.16E}\n")
+        fh.write(f" Precision Phase, Max       :  {info['precision_phase_max']:.16E}\n")
+        fh.write(f" Acceptance                 :  {info['acceptance']:.16E}\n")
+        fh.write(f" CPU Time                   :  {elapsed:.16E}\n")
+    os.remove(lock)
+    return info

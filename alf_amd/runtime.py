@@ -1,7 +1,7 @@
 """ctypes binding of the C ABI declared in include/alfq.h.
 
 This is the Python side of the drop-in boundary: the same entry points a
-Fortran host binds with ISO_C_BINDING (see alf_amd/fortran/alfq_binding.F90).
+Fortran host binds with ISO_C_BINDING (alf_amd/fortran/alfq_mod.F90).
 The library is the in-tree ``alf_amd/libalfq.so`` built for gfx950; there is
 no fallback -- loading fails loudly when it is missing.
 """

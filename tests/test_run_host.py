@@ -26,3 +26,20 @@ def test_running_lock_refuses_a_second_run(tmp_path):
     with pytest.raises(RuntimeError, match="already running"):
         simulate(HubbardParams(L1=4, L2=4, Beta=1.0), nsweep=1, nbin=1, directory=str(tmp_path))
     assert (tmp_path / "RUNNING").exists()        # the lock of the other run is left alone
+
+
+def test_failed_run_keeps_the_lock(tmp_path, monkeypatch):
+    """A run that raises leaves RUNNING in place (main.F90:961-965 removes it only after a
+    finished run), so the next run is refused until its files are checked."""
+    import alf_amd.run as run
+
+    class FailingEngine:
+        def __init__(self, *a, **k):
+            raise RuntimeError("device error during the run")
+
+    monkeypatch.setattr(run, "Engine", FailingEngine)
+    with pytest.raises(RuntimeError, match="device error"):
+        simulate(HubbardParams(L1=4, L2=4, Beta=1.0), nsweep=1, nbin=1, directory=str(tmp_path))
+    assert (tmp_path / "RUNNING").exists()
+    with pytest.raises(RuntimeError, match="already running"):
+        simulate(HubbardParams(L1=4, L2=4, Beta=1.0), nsweep=1, nbin=1, directory=str(tmp_path))
