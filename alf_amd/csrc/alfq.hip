@@ -166,6 +166,7 @@ struct alfq_ctx {
   bool t_real = true;
   int K = 32;
   int prec = ALFQ_PREC_FP64;             // arithmetic of the hopping wraps of G
+  bool symm = false;                     // Symm = .T.: measurements on Hop_mod_Symm(G)
   // equal-time measurements
   ObsDev obs;
   long long obs_cs = 0;                  // bytes per chain of the measurement blocks
@@ -176,6 +177,9 @@ struct alfq_ctx {
   // checkerboard Op_T: bond sites (shared), bond exponentials per chain
   CbOps cb{};
   CbOps cb_inv{};
+  CbOps cb_half{};                       // Symm: bond half steps exp(g O_b / 2) and inverses
+  CbOps cb_half_inv{};
+  bool cb_symm = false;
   bool cb_on = false;
   char* cb_buf = nullptr;
   int2* cb_sites = nullptr;
@@ -211,6 +215,8 @@ struct alfq_ctx {
   // model
   void* expT = nullptr;
   void* expTinv = nullptr;
+  void* expTh = nullptr;                 // Symm: exp(-dtau T / 2), exp(dtau T / 2)
+  void* expThinv = nullptr;
   int* op_site = nullptr;
   int* op_type = nullptr;
   int* site_ptr = nullptr;
@@ -798,10 +804,37 @@ int sweep_dev(alfq_ctx* c) {
   return st;
 }
 
-// ham%Obser on slice NTAU1 when measurements are on and NTAU1 is in LOBS_ST..LOBS_EN
+// Hop_mod_Symm (Hop_mod.F90:279-299): out = exp(-dtau T/2) G exp(dtau T/2), dense or bond by
+// bond (for bonds Ncheck..1: E_b^1/2 (.) E_b^-1/2); tmp is scratch.
+int hop_symm_dev(alfq_ctx* c, const cplx* G, cplx* tmp, cplx* out) {
+  int st;
+  const int n = c->n;
+  if (c->cb_on) {
+    if (!c->cb_symm) {
+      alfq_set_error("Symm with a checkerboard Op_T needs the bond half steps (alfq_checkerboard_desc)");
+      return ST_HAMILTONIAN;
+    }
+    if ((st = cb_left(tmp, G, n, nullptr, nullptr, c->cb_half, 1, c->B, c->cs, c->s))) return st;
+    return cb_right(out, tmp, n, nullptr, nullptr, nullptr, nullptr, c->cb_half_inv, 1, c->B, c->cs, c->s);
+  }
+  GemmArgs a = gemm_args(n, n, n, c->expTh, n, OP_N, !c->t_real, G, n, OP_N, true, tmp, n, ONE, ZERO);
+  if ((st = gemm_c(c, a))) return st;
+  GemmArgs b = gemm_args(n, n, n, tmp, n, OP_N, true, c->expThinv, n, OP_N, !c->t_real, out, n, ONE, ZERO);
+  return gemm_c(c, b);
+}
+
+// ham%Obser on slice NTAU1 when measurements are on and NTAU1 is in LOBS_ST..LOBS_EN, on
+// Hop_mod_Symm(GR) with Symm = .T. (main.F90:757-772, 789-802).  H and Gtmp are free
+// between the wraps and the stabilisation, so they hold the transformed G.
 int measure_dev(alfq_ctx* c, int ntau1) {
   if (!c->obs_on || ntau1 < c->lobs_st || ntau1 > c->lobs_en) return ST_NONE;
-  return obs_measure(c->obs, c->G, c->st, c->B, c->cs, c->obs_cs, c->s);
+  const cplx* g = c->G;
+  if (c->symm) {
+    int st;
+    if ((st = hop_symm_dev(c, c->G, c->H, c->Gtmp))) return st;
+    g = c->Gtmp;
+  }
+  return obs_measure(c->obs, g, c->st, c->B, c->cs, c->obs_cs, c->s);
 }
 
 int sweep_body(alfq_ctx* c) {
@@ -982,7 +1015,15 @@ int check_desc(const alfq_model_desc* d, const alfq_model_desc* d0) {
       return ST_HAMILTONIAN;
     }
   }
+  if ((d->expT_half == nullptr) != (d->expT_half_inv == nullptr)) {
+    alfq_set_error("Symm: give both half-step exponentials or neither");
+    return ST_HAMILTONIAN;
+  }
   if (d0 && d != d0) {
+    if ((d->expT_half == nullptr) != (d0->expT_half == nullptr)) {
+      alfq_set_error("batch: the chains differ in Symm");
+      return ST_HAMILTONIAN;
+    }
     if (d->ndim != d0->ndim || d->ltrot != d0->ltrot || d->nwrap != d0->nwrap || d->n_sun != d0->n_sun ||
         d->n_op != d0->n_op || d->t_is_real != d0->t_is_real) {
       alfq_set_error("batch: the chains differ in Ndim, Ltrot, Nwrap, N_SUN, the operator count or the type of T");
@@ -1052,6 +1093,10 @@ void place_chain(alfq_ctx* c, ArenaLayout& ar, int nslot) {
   const size_t tsz = c->t_real ? sizeof(double) : sizeof(cplx);
   c->expT = ar.take<char>(nn * tsz);
   c->expTinv = ar.take<char>(nn * tsz);
+  if (c->symm) {
+    c->expTh = ar.take<char>(nn * tsz);
+    c->expThinv = ar.take<char>(nn * tsz);
+  }
   c->eexp = ar.take<cplx>(5 * (size_t)no);
   c->myexp = ar.take<cplx>(25 * (size_t)no);
   c->expalpha = ar.take<cplx>(25 * (size_t)no);
@@ -1101,6 +1146,7 @@ size_t chain_arena_bytes(const alfq_model_desc* d) {
   alfq_ctx t;
   t.n = d->ndim; t.L = d->ltrot; t.W = d->nwrap; t.n_op = d->n_op; t.n_sun = d->n_sun;
   t.t_real = d->t_is_real != 0;
+  t.symm = d->expT_half != nullptr;
   t.nstm = (t.L % t.W == 0) ? t.L / t.W : t.L / t.W + 1;
   long long per = 0;
   for (int i = 0; i < d->n_op; ++i) per += (d->op_type[i] == 2) ? 3 : 2;
@@ -1122,6 +1168,7 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
   c->B = nchain;
   c->n = d->ndim; c->L = d->ltrot; c->W = d->nwrap; c->n_op = d->n_op; c->n_sun = d->n_sun;
   c->t_real = d->t_is_real != 0;
+  c->symm = d->expT_half != nullptr;
   c->nstm = (c->L % c->W == 0) ? c->L / c->W : c->L / c->W + 1;
   c->stab.assign(c->nstm + 1, 0);
   for (int k = 1; k < c->nstm; ++k) c->stab[k] = c->W * k;
@@ -1194,6 +1241,9 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
     ds.z_cgr = make_double2(1.0, 0.0);
     ds.opph = make_double2(1.0, 0.0);
     std::vector<int8_t> f1((size_t)std::max(no, 1) * c->L, 1);
+    if (c->symm && (put_chain(c, z, (char*)c->expTh, (const char*)dz->expT_half, nn * tsz) ||
+                    put_chain(c, z, (char*)c->expThinv, (const char*)dz->expT_half_inv, nn * tsz)))
+      return fail("copy half-step exponentials");
     if (put_chain(c, z, (char*)c->expT, (const char*)dz->expT, nn * tsz) ||
         put_chain(c, z, (char*)c->expTinv, (const char*)dz->expT_inv, nn * tsz) ||
         put_chain(c, z, c->st, &cs, 1) || put_chain(c, z, c->sc, &ds, 1) ||
@@ -1423,7 +1473,13 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
       key.push_back(i);
       key.push_back(j);
     }
-  const long long per = 2LL * 4 * nb * (long long)sizeof(cplx);
+  const bool half = d->expo_half != nullptr && d->expo_half_inv != nullptr;
+  if ((d->expo_half == nullptr) != (d->expo_half_inv == nullptr)) {
+    alfq_set_error("set_checkerboard: give both half-step bond matrices or neither");
+    return ST_HAMILTONIAN;
+  }
+  key.push_back(half ? 1 : 0);
+  const long long per = (half ? 4LL : 2LL) * 4 * nb * (long long)sizeof(cplx);
   const long long mstride = (per + 255) & ~255LL;
   if (!c->cb_buf) {
     char* buf = nullptr;
@@ -1449,6 +1505,13 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
     c->cb = o;
     o.mats = (const cplx*)(c->cb_buf + 4LL * nb * (long long)sizeof(cplx));
     c->cb_inv = o;
+    if (half) {
+      o.mats = (const cplx*)(c->cb_buf + 8LL * nb * (long long)sizeof(cplx));
+      c->cb_half = o;
+      o.mats = (const cplx*)(c->cb_buf + 12LL * nb * (long long)sizeof(cplx));
+      c->cb_half_inv = o;
+    }
+    c->cb_symm = half;
   } else if (key != c->cb_key) {
     alfq_set_error("set_checkerboard: the bonds differ from the batch's");
     return ST_HAMILTONIAN;
@@ -1457,13 +1520,19 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
   const size_t mb = 4 * (size_t)nb * sizeof(cplx);
   HIP_OK(hipMemcpy(blk, d->expo, mb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(blk + mb, d->expo_inv, mb, hipMemcpyHostToDevice));
+  if (half) {
+    HIP_OK(hipMemcpy(blk + 2 * mb, d->expo_half, mb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(blk + 3 * mb, d->expo_half_inv, mb, hipMemcpyHostToDevice));
+  }
   // real arithmetic (RealExpOpT) while every chain's matrices are real
   bool real = true;
   for (size_t i = 0; i < 4 * (size_t)nb && real; ++i) real = d->expo[2 * i + 1] == 0.0 && d->expo_inv[2 * i + 1] == 0.0;
+  for (size_t i = 0; half && i < 4 * (size_t)nb && real; ++i)
+    real = d->expo_half[2 * i + 1] == 0.0 && d->expo_half_inv[2 * i + 1] == 0.0;
   if (c->cb_real.empty()) c->cb_real.assign(c->B, 1);
   c->cb_real[c->cur] = real;
   const bool all_real = std::all_of(c->cb_real.begin(), c->cb_real.end(), [](char v) { return v != 0; });
-  c->cb.is_real = c->cb_inv.is_real = all_real;
+  c->cb.is_real = c->cb_inv.is_real = c->cb_half.is_real = c->cb_half_inv.is_real = all_real;
   c->cb_set[c->cur] = 1;
   drop_graphs(c);                        // captured sweeps may hold the other arithmetic
   c->cb_on = std::all_of(c->cb_set.begin(), c->cb_set.end(), [](char v) { return v != 0; });
@@ -1534,6 +1603,10 @@ int alfq_tau_enable(alfq_ctx* c, int32_t on) {
   if (!c) return ST_GENERIC;
   if (on) {
     if (!c->obs.buf) { alfq_set_error("tau_enable: call alfq_obs_setup first"); return ST_GENERIC; }
+    if (c->symm) {
+      alfq_set_error("tau_enable: time-displaced measurements with Symm = .T. are not built");
+      return ST_HAMILTONIAN;
+    }
     int st;
     if ((st = tau_alloc(c))) return st;
   }

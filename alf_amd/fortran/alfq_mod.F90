@@ -23,6 +23,8 @@ module alfq_mod
   type, bind(C), public :: alfq_model_desc
     integer(c_int32_t) :: ndim, ltrot, nwrap, n_sun, n_fl, n_op, t_is_real
     type(c_ptr) :: expT, expT_inv, op_P, op_type, op_E, op_g, op_alpha
+    ! Symm = .T.: mat_1D2 and invmat_1D2 of Op_T; c_null_ptr for Symm = .F.
+    type(c_ptr) :: expT_half = c_null_ptr, expT_half_inv = c_null_ptr
   end type alfq_model_desc
 
   type, bind(C), public :: alfq_control
@@ -41,6 +43,8 @@ module alfq_mod
   type, bind(C), public :: alfq_checkerboard_desc
     integer(c_int32_t) :: nbond, nfam
     type(c_ptr) :: fam_len, p, expo, expo_inv
+    ! Symm = .T.: the bonds' mat_1D2 and invmat_1D2; c_null_ptr for Symm = .F.
+    type(c_ptr) :: expo_half = c_null_ptr, expo_half_inv = c_null_ptr
   end type alfq_checkerboard_desc
 
   ! time-displaced sums (Ltau = 1) and Control_Precision_tau

@@ -7,7 +7,8 @@
 ``Hubbard`` restates ``Prog/Hamiltonians/Hamiltonian_Hubbard_smod.F90`` for the
 density-channel discrete HS coupling (``Mz=.F.``, ``Continuous=.F.``) on the
 square and honeycomb lattices, with the dense hopping exponential or the checkerboard
-decomposition (``Checkerboard``, Symm = .F.):
+decomposition (``Checkerboard``), and the symmetric Trotter form (``Symm``: symmetrised
+checkerboard families, and measurements on exp(-dtau T/2) G exp(dtau T/2)):
   * Ham_Set 207-337 (Ltrot = nint(beta/dtau), N_FL handling)
   * Ham_Hop 362-448 -> Predefined_Hoppings_set_OPT (dense and checkerboard branches)
   * Ham_V 477-542 -> Predefined_Int_U_SUN (Prog/Predefined_Int_mod.F90:59-76)
@@ -91,8 +92,6 @@ def hubbard_ham_set(p: HubbardParams) -> HamBase:
     """Ham_Set of the Hubbard plugin (Hamiltonian_Hubbard_smod.F90:207-337)."""
     if p.Mz or p.Continuous:
         raise NotImplementedError("this build covers the density-channel discrete HS coupling (Mz=.F.)")
-    if p.Checkerboard and p.Symm:
-        raise NotImplementedError("this build covers the non-symmetric checkerboard (Symm=.F.)")
     if p.N_FL != 1:
         raise NotImplementedError("this build covers N_FL = 1")
     ltrot = int(np.rint(p.Beta / p.Dtau))
@@ -115,8 +114,9 @@ def hubbard_ham_set(p: HubbardParams) -> HamBase:
     if p.Checkerboard:
         # bond operators (Predefined_Hop_mod.F90:1773-1834); Op_T holds their dense products
         # P = E_1 ... E_N and P^-1, the matrices Hop_mod_mmthr / Hop_mod_mmthr_m1 apply
-        cb = hoppings.checkerboard_opt(hop, Invlist, latt, unit, p.Lattice_type, p.Dtau)
-        exp_t = ExpOpT(cb.is_real, np.arange(ndim), cb.dense(ndim, "P"), cb.dense(ndim, "Pinv"), None, None,
+        cb = hoppings.checkerboard_opt(hop, Invlist, latt, unit, p.Lattice_type, p.Dtau, symm=p.Symm)
+        exp_t = ExpOpT(cb.is_real, np.arange(ndim), cb.dense(ndim, "P"), cb.dense(ndim, "Pinv"),
+                       cb.dense(ndim, "Q") if p.Symm else None, cb.dense(ndim, "Qinv") if p.Symm else None,
                        -p.Dtau + 0j)
     else:
         exp_t = exp_opt_init(op_t)
@@ -132,6 +132,7 @@ def hubbard_ham_set(p: HubbardParams) -> HamBase:
     ham = HamBase(ndim, p.N_FL, p.N_SUN, ltrot, p.Nwrap, exp_t, op_v, nsigma, p,
                   List, Invlist, latt, unit, O)
     ham.extra["op_t"] = op_t
+    ham.extra["symm"] = bool(p.Symm)
     if cb is not None:
         ham.extra["checkerboard"] = cb
     return ham

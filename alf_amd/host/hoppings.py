@@ -115,15 +115,26 @@ class CheckerboardOpT:
     Einv: np.ndarray          # (N, 2, 2)
     fam_len: list
     is_real: bool
+    # half steps exp(g O_b / 2), exp(-g O_b / 2) (mat_1D2, invmat_1D2) of every bond, used by
+    # Hop_mod_Symm before the measurements with Symm = .T.
+    Eh: np.ndarray = None
+    Eih: np.ndarray = None
+    symm: bool = False
 
     def dense(self, n: int, which: str = "P") -> np.ndarray:
-        """P = E_1 ... E_N, Pinv = E_N^-1 ... E_1^-1 or Pc = P^+ = E_N ... E_1 as n x n."""
+        """P = E_1 ... E_N, Pinv = E_N^-1 ... E_1^-1 or Pc = P^+ = E_N ... E_1 as n x n; with
+        Symm = .T. also Q = E_1^1/2 ... E_N^1/2 ("Q") and Q^-1 ("Qinv"), so that Hop_mod_Symm
+        (Hop_mod.F90:279-299) is G -> Q G Q^-1."""
         M = np.eye(n, dtype=complex)
         N = len(self.p1)
         if which == "P":
             order, mats = range(N - 1, -1, -1), self.E
         elif which == "Pinv":
             order, mats = range(N), self.Einv
+        elif which == "Q":
+            order, mats = range(N - 1, -1, -1), self.Eh
+        elif which == "Qinv":
+            order, mats = range(N), self.Eih
         else:
             order, mats = range(N), self.E
         for b in order:
@@ -148,6 +159,25 @@ def checkerboard_families(hop: HoppingMatrix, latt, lattice_type: str):
     if t == "HONEYCOMB":
         return [[(I, nf) for I in range(latt.N)] for nf in range(3)]
     raise NotImplementedError(lattice_type)
+
+
+def symmetrize_families(fams):
+    """Symmetrize_Families (Predefined_Hop_mod.F90:1422-1495): families 1..N-1, N, N-1..1 with
+    time step 1/2 each and 1 for the middle one, the longest family placed in the middle.
+    Returns (families, Prop_Fam)."""
+    nc = len(fams)
+    order = list(range(nc)) + list(range(nc - 2, -1, -1))
+    lmax, fmax = 0, -1
+    for f, fam in enumerate(fams):
+        if len(fam) > lmax:
+            lmax, fmax = len(fam), f
+    prop = [0.5] * len(order)
+    prop[nc - 1] = 1.0
+    if fmax != nc - 1:
+        order[nc - 1] = fmax
+        order[0] = nc - 1
+        order[-1] = nc - 1
+    return [fams[f] for f in order], prop
 
 
 def _bond_sites(hop: HoppingMatrix, Invlist, latt, I: int, nb: int):
@@ -177,19 +207,25 @@ def test_checkerboard_decomposition(hop: HoppingMatrix, fams, Invlist, latt) -> 
     return errs
 
 
-def checkerboard_opt(hop: HoppingMatrix, Invlist, latt, unit, lattice_type: str, dtau: float) -> CheckerboardOpT:
+def checkerboard_opt(hop: HoppingMatrix, Invlist, latt, unit, lattice_type: str, dtau: float,
+                     symm: bool = False) -> CheckerboardOpT:
+    """The checkerboard branch of Predefined_Hoppings_set_OPT (Predefined_Hop_mod.F90:1773-1834),
+    with Symmetrize_Families first when Symm = .T."""
     from .operator import exp_opt_init, op_make, op_set
     fams = checkerboard_families(hop, latt, lattice_type)
     errs = test_checkerboard_decomposition(hop, fams, Invlist, latt)
     if errs:
         raise ValueError("inconsistent checkerboard decomposition: " + "; ".join(errs[:4]))
+    prop = [1.0] * len(fams)
+    if symm:
+        fams, prop = symmetrize_families(fams)
     mult = np.zeros(unit.Norb)
     for nb in range(hop.N_bonds):
         mult[int(hop.List[nb, 0])] += 1
         mult[int(hop.List[nb, 1])] += 1
-    p1, p2, Os, Es, Eis, fam_len = [], [], [], [], [], []
+    p1, p2, Os, Es, Eis, Ehs, Eihs, fam_len = [], [], [], [], [], [], [], []
     real = True
-    for fam in fams:
+    for fam, pf in zip(fams, prop):
         fam_len.append(len(fam))
         for I, nb in fam:
             no_i, no_j, n_1, n_2 = (int(v) for v in hop.List[nb])
@@ -202,7 +238,7 @@ def checkerboard_opt(hop: HoppingMatrix, Invlist, latt, unit, lattice_type: str,
             op.O[1, 0] = np.conj(hop.T[nb] * Z)
             op.O[0, 0] = hop.T_Loc[no_i] / mult[no_i]
             op.O[1, 1] = hop.T_Loc[no_j] / mult[no_j]
-            op.g = -dtau + 0j
+            op.g = -dtau * pf + 0j
             op.alpha = 0j
             Os.append(op.O.copy())
             op_set(op)
@@ -212,4 +248,7 @@ def checkerboard_opt(hop: HoppingMatrix, Invlist, latt, unit, lattice_type: str,
             p2.append(J1)
             Es.append(np.array(ex.mat))
             Eis.append(np.array(ex.invmat))
-    return CheckerboardOpT(np.array(p1), np.array(p2), np.array(Os), np.array(Es), np.array(Eis), fam_len, real)
+            Ehs.append(np.array(ex.mat_1D2))
+            Eihs.append(np.array(ex.invmat_1D2))
+    return CheckerboardOpT(np.array(p1), np.array(p2), np.array(Os), np.array(Es), np.array(Eis), fam_len, real,
+                           np.array(Ehs), np.array(Eihs), symm)

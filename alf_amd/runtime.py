@@ -47,6 +47,7 @@ class ModelDesc(C.Structure):
         ("expT", C.POINTER(C.c_double)), ("expT_inv", C.POINTER(C.c_double)),
         ("op_P", C.POINTER(C.c_int32)), ("op_type", C.POINTER(C.c_int32)),
         ("op_E", C.POINTER(C.c_double)), ("op_g", C.POINTER(C.c_double)), ("op_alpha", C.POINTER(C.c_double)),
+        ("expT_half", C.POINTER(C.c_double)), ("expT_half_inv", C.POINTER(C.c_double)),
     ]
 
 
@@ -59,7 +60,8 @@ class ObsDesc(C.Structure):
 
 class CheckerboardDesc(C.Structure):
     _fields_ = [("nbond", C.c_int32), ("nfam", C.c_int32), ("fam_len", C.POINTER(C.c_int32)),
-                ("p", C.POINTER(C.c_int32)), ("expo", C.POINTER(C.c_double)), ("expo_inv", C.POINTER(C.c_double))]
+                ("p", C.POINTER(C.c_int32)), ("expo", C.POINTER(C.c_double)), ("expo_inv", C.POINTER(C.c_double)),
+                ("expo_half", C.POINTER(C.c_double)), ("expo_half_inv", C.POINTER(C.c_double))]
 
 
 class TauSums(C.Structure):
@@ -191,9 +193,17 @@ def model_desc(model):
                 E=np.array([float(o.E[0]) for o in ops], dtype=np.float64),
                 g=np.array([o.g for o in ops], dtype=np.complex128),
                 alpha=np.array([o.alpha for o in ops], dtype=np.complex128))
+    half = (None, None)
+    if getattr(model, "extra", None) and model.extra.get("symm"):
+        # Symm = .T.: the half steps mat_1D2 / invmat_1D2 for Hop_mod_Symm before the measurements
+        conv = (lambda m: np.asfortranarray(np.real(m), dtype=np.float64)) if t_real else \
+            (lambda m: np.asfortranarray(m, dtype=np.complex128))
+        keep["expTh"] = conv(model.Op_T.mat_1D2)
+        keep["expThi"] = conv(model.Op_T.invmat_1D2)
+        half = (_dptr(keep["expTh"]), _dptr(keep["expThi"]))
     d = ModelDesc(model.Ndim, model.Ltrot, model.Nwrap, model.N_SUN, model.N_FL, len(ops), int(t_real),
                   _dptr(keep["expT"]), _dptr(keep["expTi"]), _iptr(keep["P"]), _iptr(keep["type"]),
-                  _dptr(keep["E"]), _dptr(keep["g"]), _dptr(keep["alpha"]))
+                  _dptr(keep["E"]), _dptr(keep["g"]), _dptr(keep["alpha"]), *half)
     return d, keep
 
 
@@ -207,8 +217,13 @@ def checkerboard_desc(model):
                 p=np.ascontiguousarray(np.stack([cb.p1, cb.p2], axis=1) + 1, dtype=np.int32),
                 E=np.ascontiguousarray(np.transpose(np.asarray(cb.E, dtype=np.complex128), (0, 2, 1))),
                 Ei=np.ascontiguousarray(np.transpose(np.asarray(cb.Einv, dtype=np.complex128), (0, 2, 1))))
+    half = (None, None)
+    if cb.symm:
+        keep["Eh"] = np.ascontiguousarray(np.transpose(np.asarray(cb.Eh, dtype=np.complex128), (0, 2, 1)))
+        keep["Eih"] = np.ascontiguousarray(np.transpose(np.asarray(cb.Eih, dtype=np.complex128), (0, 2, 1)))
+        half = (_dptr(keep["Eh"]), _dptr(keep["Eih"]))
     d = CheckerboardDesc(len(cb.p1), len(cb.fam_len), _iptr(keep["fam"]), _iptr(keep["p"]), _dptr(keep["E"]),
-                         _dptr(keep["Ei"]))
+                         _dptr(keep["Ei"]), *half)
     return d, keep
 
 

@@ -45,6 +45,13 @@ typedef struct alfq_model_desc {
   const double* op_E;      /* Op_V(n)%E(1) */
   const double* op_g;      /* Op_V(n)%g, complex, 2*n_op doubles */
   const double* op_alpha;  /* Op_V(n)%alpha, complex, 2*n_op doubles */
+  /* Symm = .T. (main.F90:763-772, 794-802): the half steps exp(g T/2) and exp(-g T/2)
+   * (mat_1D2, invmat_1D2 of RealExpOpT/CmplxExpOpT, OpTTypes_mod.F90:110-129, 213-232), same
+   * layout as expT; the measurements then see Hop_mod_Symm(GR) = exp(g T/2) GR exp(-g T/2)
+   * (Hop_mod.F90:279-299).  NULL (both): Symm = .F.  With a checkerboard Op_T the bond half
+   * steps of alfq_checkerboard_desc are applied instead. */
+  const double* expT_half;
+  const double* expT_half_inv;
 } alfq_model_desc;
 
 /* Statistics of Control_upgrade / Control_PrecisionG / Control_PrecisionP
@@ -261,6 +268,12 @@ typedef struct {
   const double* expo;       /* nbond 2 x 2 complex, column-major: exp(g O_b) as applied
                                (the upper triangle mirrored, OpTTypes_mod.F90:92-133, 203-234) */
   const double* expo_inv;   /* exp(-g O_b), same layout */
+  /* Symm = .T.: exp(g O_b / 2), exp(-g O_b / 2) of every bond (mat_1D2, invmat_1D2), same
+   * layout, for Hop_mod_Symm: for bonds Ncheck..1, GR <- E_b^1/2 GR E_b^-1/2; the families
+   * are then the symmetrised ones (Symmetrize_Families, Predefined_Hop_mod.F90:1422-1495).
+   * NULL (both) when the model description has no half steps. */
+  const double* expo_half;
+  const double* expo_half_inv;
 } alfq_checkerboard_desc;
 
 int alfq_set_checkerboard(alfq_ctx* ctx, const alfq_checkerboard_desc* desc);

@@ -85,6 +85,13 @@ class OracleModel:
     cb_p: np.ndarray = None
     cb_E: np.ndarray = None
     cb_Einv: np.ndarray = None
+    # Symm = .T.: the half steps exp(g T / 2), exp(-g T / 2) (dense, mat_1D2/invmat_1D2) or
+    # of every bond (checkerboard), for Hop_mod_Symm before the measurements
+    symm: bool = False
+    mat_half: np.ndarray = None
+    invmat_half: np.ndarray = None
+    cb_Eh: np.ndarray = None
+    cb_Eih: np.ndarray = None
 
     @property
     def n_op(self):
@@ -129,6 +136,19 @@ class OracleModel:
         if self.cb_p is None:
             return self.mat @ M
         return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p)), True)
+
+    def hop_symm(self, G):
+        """Hop_mod_Symm (Hop_mod.F90:279-299): for bonds nc = Ncheck..1, G <- E_nc^1/2 G E_nc^-1/2
+        (RealExpOpT/CmplxExpOpT adjointaction with mat_1D2 and invmat_1D2, OpTTypes_mod.F90:135-147,
+        236-248); dense: exp(-dtau T/2) G exp(dtau T/2)."""
+        if self.cb_p is None:
+            return self.mat_half @ G @ self.invmat_half
+        M = G.copy()
+        for b in range(len(self.cb_p) - 1, -1, -1):
+            r = [int(self.cb_p[b, 0]), int(self.cb_p[b, 1])]
+            M[r, :] = self.cb_Eh[b] @ M[r, :]
+            M[:, r] = M[:, r] @ self.cb_Eih[b]
+        return M
 
     def e_exp(self, n, sp):
         """E_exp(1, sp) of operator n (Operator_mod.F90:436-466)."""
@@ -566,12 +586,13 @@ class OracleSweep:
         self.G = G
 
     def _measure(self, ntau1):
-        """ham%Obser(GR, PHASE, NTAU1, 1.0) on the measured slices (Symm = .F.)."""
+        """ham%Obser(GR, PHASE, NTAU1, 1.0) on the measured slices, on Hop_mod_Symm(GR) with
+        Symm = .T. (main.F90:757-772, 789-802)."""
         if self.obs is None:
             return
         st, en = self.lobs if self.lobs is not None else (1, self.m.ltrot)
         if st <= ntau1 <= en:
-            self.obs.measure(self.G, self.phase)
+            self.obs.measure(self.m.hop_symm(self.G) if self.m.symm else self.G, self.phase)
 
     def sweep(self):
         """One NSW iteration of the sequential sweep (main.F90:714-887)."""
@@ -714,6 +735,12 @@ def model_from_ham(ham) -> OracleModel:
     cb = ham.extra.get("checkerboard")
     cbkw = {} if cb is None else dict(cb_p=np.stack([cb.p1, cb.p2], axis=1), cb_E=np.array(cb.E),
                                       cb_Einv=np.array(cb.Einv))
+    if ham.extra.get("symm"):
+        cbkw["symm"] = True
+        if cb is None:
+            cbkw.update(mat_half=np.array(ham.Op_T.mat_1D2), invmat_half=np.array(ham.Op_T.invmat_1D2))
+        else:
+            cbkw.update(cb_Eh=np.array(cb.Eh), cb_Eih=np.array(cb.Eih))
     return OracleModel(**cbkw,
         ndim=ham.Ndim, ltrot=ham.Ltrot, nwrap=ham.Nwrap, n_sun=ham.N_SUN,
         mat=np.array(ham.Op_T.mat), invmat=np.array(ham.Op_T.invmat),

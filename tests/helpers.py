@@ -6,9 +6,9 @@ from oracle.dqmc_oracle import OracleSweep, Tape, model_from_ham
 
 
 def hubbard(L1=4, L2=4, beta=1.0, U=4.0, nwrap=5, lattice="Square", n_phi=0, dtau=0.1, chem=0.0,
-            checkerboard=False):
+            checkerboard=False, symm=False):
     p = HubbardParams(Lattice_type=lattice, L1=L1, L2=L2, Beta=beta, Ham_U=U, Nwrap=nwrap, N_Phi=n_phi,
-                      Dtau=dtau, Ham_chem=chem, Checkerboard=checkerboard)
+                      Dtau=dtau, Ham_chem=chem, Checkerboard=checkerboard, Symm=symm)
     return hubbard_ham_set(p)
 
 

@@ -10,6 +10,7 @@ from scipy.linalg import expm
 
 import oracle.dqmc_oracle as O
 from alf_amd.host import hoppings
+from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
 from helpers import fields_for, hubbard, oracle, rel, tape
 
 CASES = [("Square", 4, 0, 0.0), ("Square", 6, 0, 0.3), ("Honeycomb", 3, 1, 0.0), ("Honeycomb", 4, 2, 0.2)]
@@ -120,3 +121,63 @@ def test_checkerboard_sweep_wraps_track_the_fresh_green():
     assert s.ctrl.XMAXG < 1e-9 and s.ctrl.XMAXP < 1e-9
     Gb, _ = O.brute_force_green(s.m, s.f, 0, dps=30)
     assert rel(s.G, Gb) < 1e-10
+
+
+def test_symmetrised_families_follow_the_reference_rule():
+    """Symmetrize_Families (Predefined_Hop_mod.F90:1422-1495): families 1..N-1, N, N-1..1 with
+    time steps 1/2 and 1 in the middle; the longest family goes to the middle."""
+    from alf_amd.host.hoppings import symmetrize_families
+    # equal lengths (the predefined square and honeycomb families): the first family is the
+    # longest, so it takes the middle and the last family both ends
+    out, prop = symmetrize_families([["a"], ["b"], ["c"], ["d"]])
+    assert out == [["d"], ["b"], ["c"], ["a"], ["c"], ["b"], ["d"]]
+    assert prop == [0.5, 0.5, 0.5, 1.0, 0.5, 0.5, 0.5]
+    out, prop = symmetrize_families([["a"], ["b"], ["c"]])
+    assert out == [["c"], ["b"], ["a"], ["b"], ["c"]]
+    # the longest family last: the plain palindrome
+    out, prop = symmetrize_families([["a"], ["b"], ["c", "c"]])
+    assert out == [["a"], ["b"], ["c", "c"], ["b"], ["a"]] and prop == [0.5, 0.5, 1.0, 0.5, 0.5]
+    # unequal lengths with the longest elsewhere: the reference's placement overwrites the first
+    # and the last entries, so the longest family appears three times and the first one not at all
+    out, _ = symmetrize_families([["a"], ["b", "b"], ["c"]])
+    assert out == [["c"], ["b", "b"], ["b", "b"], ["b", "b"], ["c"]]
+
+
+@pytest.mark.parametrize("lattice,L,nphi", [("Square", 6, 0), ("Honeycomb", 3, 0), ("Honeycomb", 4, 2)])
+def test_symmetric_splitting_is_second_order(lattice, L, nphi):
+    """With Symm the checkerboard product P = E_1 ... E_N of the symmetrised families and the
+    half-step product Q of Hop_mod_Symm approach exp(-dtau T) and exp(-dtau T/2) with an error
+    of order dtau^3 per step (falls about 8x per halving of dtau), against first order without
+    Symm; Q Q^-1 = 1 and Q^dagger = Q for a palindromic product of Hermitian bond steps."""
+    import scipy.linalg as sla
+    errs, errq, err1 = [], [], []
+    for dtau in (0.2, 0.1, 0.05):
+        p = HubbardParams(Lattice_type=lattice, L1=L, L2=L, N_Phi=nphi, Beta=1.0, Dtau=dtau, Checkerboard=True,
+                          Symm=True)
+        ham = hubbard_ham_set(p)
+        T = ham.hop_matrix
+        errs.append(np.abs(ham.Op_T.mat - sla.expm(-dtau * T)).max())
+        errq.append(np.abs(ham.Op_T.mat_1D2 - sla.expm(-dtau / 2 * T)).max())
+        Q, Qi = ham.Op_T.mat_1D2, ham.Op_T.invmat_1D2
+        assert np.abs(Q @ Qi - np.eye(ham.Ndim)).max() < 1e-13
+        assert np.abs(Q - Q.conj().T).max() < 1e-13
+        p1 = HubbardParams(Lattice_type=lattice, L1=L, L2=L, N_Phi=nphi, Beta=1.0, Dtau=dtau, Checkerboard=True)
+        h1 = hubbard_ham_set(p1)
+        err1.append(np.abs(h1.Op_T.mat - sla.expm(-dtau * T)).max())
+    for e in (errs, errq):
+        assert 6.0 < e[0] / e[1] < 10.0 and 6.0 < e[1] / e[2] < 10.0, e
+    assert 3.0 < err1[0] / err1[1] < 5.0, err1
+
+
+def test_dense_symm_half_steps():
+    """Dense Op_T with Symm: mat_1D2 = exp(-dtau T/2), invmat_1D2 its inverse (the oracle's own
+    expm of the geometric hopping matrix), and the oracle's Hop_mod_Symm is their similarity."""
+    from oracle import model_setup as M
+    from oracle.dqmc_oracle import model_from_ham
+    ham = hubbard_ham_set(HubbardParams(L1=4, L2=4, Beta=1.0, Dtau=0.1, Symm=True))
+    T = M.hopping_matrix("Square", 4, 4)
+    h, hi = M.exp_hopping(T, 0.05)
+    assert np.abs(ham.Op_T.mat_1D2 - h).max() < 1e-13 and np.abs(ham.Op_T.invmat_1D2 - hi).max() < 1e-13
+    m = model_from_ham(ham)
+    G = np.random.default_rng(0).standard_normal((16, 16)) + 0j
+    assert np.abs(m.hop_symm(G) - h @ G @ hi).max() < 1e-13

@@ -102,3 +102,33 @@ def test_batched_measurements_match_single_chains(rt):
         assert rel(bat["scal"], one["scal"]) < TOL and rel(bat["latt"], one["latt"]) < TOL
         e.close()
     b.close()
+
+
+@pytest.mark.parametrize("lattice,L,n_phi,checkerboard", [
+    ("Square", 4, 0, False), ("Square", 6, 0, True), ("Honeycomb", 3, 1, False), ("Honeycomb", 4, 2, True),
+])
+def test_symm_measurements_match_oracle(rt, lattice, L, n_phi, checkerboard):
+    """Symm = .T.: the measurements see Hop_mod_Symm(G) = exp(-dtau T/2) G exp(dtau T/2), dense
+    or bond by bond over the symmetrised checkerboard families; sums equal the oracle's."""
+    ham = hubbard(L1=L, L2=L, beta=1.0, nwrap=5, lattice=lattice, n_phi=n_phi, checkerboard=checkerboard,
+                  symm=True)
+    f = fields_for(ham, 6)
+    u = tape(2_000_000, seed=3)
+    s = oracle(ham, f, u)
+    assert s.m.symm
+    s.obs = obs_oracle(ham)
+    e = rt.Engine(ham, delay=8)
+    e.set_fields(f)
+    e.set_tape(u)
+    e.obs_setup()
+    e.obs_enable(True)
+    s.init()
+    e.init()
+    for _ in range(2):
+        s.sweep()
+        e.sweep(1)
+    assert np.array_equal(e.get_fields(), s.f)
+    check_sums(e.obs_get(), s.obs)
+    with pytest.raises(rt.AlfqError):
+        e.tau_enable(True)
+    e.close()
