@@ -1,0 +1,86 @@
+"""Observables of full runs against the numbers the reference publishes for its default model.
+
+The reference's Trotter study (Documentation/stabilization.tex:124-134, data in
+Documentation/Figures/Dtau/*/Ener.dat) gives the total energy of the 6 x 6 square Hubbard
+model at U/t = 4, half filling, beta t = 5 (the model of Scripts_and_Parameters_files/Start/
+parameters:8-26) for the four combinations of Symm and Checkerboard; at dtau = 0.1 (line 2 of
+each file):
+
+    Symm=.F. Checkerboard=.F.  -30.453687 +- 0.003541   (NoSymNoCheck/Ener.dat:2)
+    Symm=.F. Checkerboard=.T.  -30.667618 +- 0.004828   (NoSymCheck/Ener.dat:2)
+    Symm=.T. Checkerboard=.F.  -29.777810 +- 0.003834   (SymNoCheck/Ener.dat:2)
+    Symm=.T. Checkerboard=.T.  -29.774338 +- 0.005003   (SymCheck/Ener.dat:2)
+
+Those runs couple the HS field to the z-component of spin; this engine couples it to the
+density.  Both write the same interaction operator up to a constant, so the Trotterised
+Hamiltonian (hence the dtau dependence these anchors measure) is the same, and the Ener
+estimator Kin + U sum_i <n_up n_dn> (Hamiltonian_Hubbard_smod.F90:735-771) is too.  The
+four-state HS transformation itself is exact up to O(dtau^4) per slice in either channel.
+
+Each case runs a batch of 256 independent chains (seeds 0..255) for 100 thermalising and 200
+measured sweeps, measuring on every slice; the chains' means are independent samples, so the
+standard error is their standard deviation over sqrt(256).  The test asks the mean to lie
+within two combined standard errors of the published value and prints it with the double
+occupancy beside it."""
+import math
+
+import numpy as np
+import pytest
+
+from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
+
+pytestmark = pytest.mark.gpu
+
+ANCHORS = {
+    (False, False): (-30.453687, 0.003541),
+    (False, True): (-30.667618, 0.004828),
+    (True, False): (-29.777810, 0.003834),
+    (True, True): (-29.774338, 0.005003),
+}
+CHAINS, THERM, MEASURED = 256, 100, 200
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from alf_amd import runtime
+    runtime.load_library()
+    return runtime
+
+
+def run_energy(rt, symm, checkerboard):
+    p = HubbardParams(Lattice_type="Square", L1=6, L2=6, Ham_U=4.0, Beta=5.0, Dtau=0.1, Nwrap=10,
+                      Symm=symm, Checkerboard=checkerboard)
+    ham = hubbard_ham_set(p)
+    b = rt.BatchEngine([ham] * CHAINS, delay=32)
+    for z in range(CHAINS):
+        ham.nsigma.set_uniform_four_state(z)
+        b.chain(z).set_fields(ham.nsigma.f.copy())
+        b.chain(z).set_philox(seed=z, stream=7)
+    b.obs_setup()
+    b.init()
+    b.sweep(THERM)
+    b.obs_enable(True)
+    b.obs_reset()
+    b.sweep(MEASURED)
+    E, D, XG = [], [], []
+    for z in range(CHAINS):
+        s = b.chain(z).obs_get()
+        assert s["n_scal"] == MEASURED * (2 * ham.Ltrot - 1)
+        E.append((s["scal"][3] / s["sign_scal"]).real)
+        D.append((s["scal"][1] / s["sign_scal"]).real / (4.0 * ham.Ndim))
+        XG.append(b.chain(z).control()["XMAXG"])
+    b.close()
+    E, D = np.array(E), np.array(D)
+    return E.mean(), E.std(ddof=1) / math.sqrt(CHAINS), D.mean(), D.std(ddof=1) / math.sqrt(CHAINS), max(XG)
+
+
+@pytest.mark.parametrize("symm,checkerboard", list(ANCHORS))
+def test_energy_matches_published_value(rt, symm, checkerboard):
+    e, de, d, dd, xg = run_energy(rt, symm, checkerboard)
+    ref, dref = ANCHORS[(symm, checkerboard)]
+    sig = math.sqrt(de ** 2 + dref ** 2)
+    print(f"\nSymm={symm} Checkerboard={checkerboard}: E = {e:.6f} +- {de:.6f} (reference {ref} +- {dref}, "
+          f"{(e - ref) / sig:+.2f} combined sigma); double occupancy {d:.6f} +- {dd:.6f}; Precision Green max {xg:.1e}",
+          flush=True)
+    assert de < 2.0 * dref                       # a resolution comparable to the reference's
+    assert abs(e - ref) < 2.0 * sig
