@@ -8,7 +8,9 @@ random stream.  One step = one sweep of ALF's sequential finite-temperature
 update (up pass, down pass, slice-0 refresh), all on the device.
 
 Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle (numpy /
-LAPACK restatement of the same sweep) timed on one host core for one sweep.
+LAPACK restatement of the same sweep) on the host cores: one single-threaded chain per
+core (at most 16), all at once, timed before the GPU is touched; the one-core and
+threaded-BLAS single-chain rates stand beside it.
 Beside the single chain, the line reports the aggregate rate of a batch of
 independent chains of the same workload (distinct seeds) advancing in one
 context, every kernel launch covering all of them.
@@ -48,22 +50,50 @@ def attainable_sweep_seconds(n, L, S, acc, complex_t=False, peak=78.6e12, bw=8.0
     return t_wrap + t_rank1 + t_qr
 
 
-def cpu_baseline(ham, fields, seconds_budget=30.0):
-    """Time the oracle on one core over one sweep of the same workload."""
+def cpu_baseline_run(procs, sweeps=2, latency_sweeps=1):
+    """The FP64 oracle on the host cores, in child processes started before this process
+    touches the GPU and finished before it does: throughput mode (one single-threaded chain of
+    the workload per core, all at once), one single-threaded chain alone, and one chain with
+    threaded BLAS on all of them.  Returns the parsed report or an error note."""
+    import subprocess
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--procs", str(procs), "--sweeps", str(sweeps),
+           "--latency-sweeps", str(latency_sweeps)]
+    r = subprocess.run(cmd, cwd=HERE, stdout=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"error": f"oracle.cpu_baseline exited with {r.returncode}"}
+    rep = json.loads(r.stdout.strip().splitlines()[-1])
+    thr = rep["throughput"]
+    out = {"value": thr["aggregate_sweeps_per_s"], "unit": "sweeps/s", "cores": procs, "kind": "port",
+           "sample": (f"throughput mode: {procs} single-threaded chains of the 16x16 U=4 beta=10 workload "
+                      f"(seeds 0..{procs - 1}) x {sweeps} sweeps each, started together on {procs} host cores, "
+                      "run by the numpy/LAPACK FP64 oracle (ZGEQP3 pivoting, ZGERU rank-1 updates, real "
+                      "exp(-dtau T)); value = total sweeps / wall time of the timed window"),
+           "median_sweep_s_throughput": thr["median_sweep_s"],
+           "nproc": rep["nproc"], "cpu_model": rep["cpu_model"], "lapack": rep["lapack"]}
+    if "single_core" in rep:
+        out["single_core_sweeps_per_s"] = 1.0 / rep["single_core"]["median_sweep_s"]
+    if "latency" in rep:
+        out["latency_mode_sweeps_per_s"] = 1.0 / rep["latency"]["median_sweep_s"]
+        out["latency_mode_threads"] = procs
+    return out
+
+
+def cpu_oracle_rate(ham, fields, sweeps=3):
+    """Median sweeps/s of the oracle on one host thread for a small model (the 4x4 case)."""
     from threadpoolctl import threadpool_limits
 
     from oracle.dqmc_oracle import OracleSweep, Tape, model_from_ham
     m = model_from_ham(ham)
-    rng = Tape(np.random.default_rng(12345).random(4 * m.n_op * m.ltrot * 2 + 16))
+    rng = Tape(np.random.default_rng(12345).random(3 * m.n_op * m.ltrot * 2 * sweeps + 16))
+    times = []
     with threadpool_limits(1):
-        s = OracleSweep(m, fields, rng, presort=False)
+        s = OracleSweep(m, fields, rng)
         s.init()
-        t0 = time.perf_counter()
-        s.sweep()
-        dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
-            "sample": "1 sweep of the 16x16 U=4 beta=10 workload by the numpy/LAPACK FP64 oracle "
-                      f"(ZGEQP3 pivoting), one thread, {dt:.1f} s"}
+        for _ in range(sweeps):
+            t0 = time.perf_counter()
+            s.sweep()
+            times.append(time.perf_counter() - t0)
+    return 1.0 / float(np.median(times))
 
 
 def batched_rate(chains, steps, prec=0, checkerboard=False, params=None):
@@ -189,7 +219,7 @@ def config_rates(delay, cpu=True):
             r = single_rate(ham, f, delay, 5 if ham.Ndim > 64 else 20, 2, 0)
             row["checkerboard" if cb else "dense"] = {k: r[k] for k in ("sweeps_per_s", "precision_green_max")}
             if cpu and not cb and ham.Ndim <= 16:
-                row["cpu_oracle_sweeps_per_s"] = cpu_baseline(ham, f)["value"]
+                row["cpu_oracle_sweeps_per_s"] = cpu_oracle_rate(ham, f)
         if kw["L1"] * kw["L2"] <= 16:
             # a lattice this small leaves a single chain latency-bound: many chains per launch
             r = batched_rate(256, 5, params=kw)
@@ -246,6 +276,9 @@ def main():
         print("bench.py: this engine runs one Markov chain per GPU and this build measures one GPU only; "
               "independent chains shard across GPUs without communication, so use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the host-core baseline runs first, in child processes, before this process touches the GPU
+    cpu = None if args.no_cpu_baseline else cpu_baseline_run(min(16, os.cpu_count() or 1))
 
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import Engine
@@ -343,8 +376,8 @@ def main():
     if not args.only_headline:
         result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
                                             "fp64_vector_fma": mfma_rate(2)}
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(ham, fields)
+    if cpu is not None:
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
 
 

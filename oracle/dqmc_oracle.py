@@ -34,12 +34,13 @@ the field tables.  A function without such a check says so in its docstring.
 """
 from __future__ import annotations
 
+import cmath
 import math
 from dataclasses import dataclass, field
 
 import numpy as np
 import scipy.linalg as sla
-from scipy.linalg import lapack
+from scipy.linalg import blas, lapack
 
 _S6 = math.sqrt(6.0)
 PHI2 = {-2: -math.sqrt(2.0 * (3.0 + _S6)), -1: -math.sqrt(2.0 * (3.0 - _S6)),
@@ -60,6 +61,29 @@ def nranf(u, n):
     x = u * n + 0.5
     k = int(math.floor(x + 0.5))
     return min(max(k, 1), n)
+
+
+def lmul(A, M):
+    """A @ M for a square complex M; a real A (RealExpOpT, Operator_mod.F90:1103-1120 keeps
+    real T real) multiplies the real and imaginary parts with one real GEMM of 2n columns,
+    as ZDSLSYMM does (Mat_subroutines_mod.F90:518-640)."""
+    if np.iscomplexobj(A):
+        return A @ M
+    Mc = np.ascontiguousarray(M, dtype=complex)
+    return (A @ Mc.view(np.float64)).view(complex).reshape(A.shape[0], Mc.shape[1])
+
+
+def rmul(M, A):
+    """M @ A for a square complex M and a real or complex A (the right-sided ZDSLSYMM)."""
+    if np.iscomplexobj(A):
+        return M @ A
+    return np.ascontiguousarray(lmul(A.T, np.ascontiguousarray(M.T)).T)
+
+
+def _real_if_real(a):
+    if a is not None and np.iscomplexobj(a) and not np.any(a.imag):
+        return np.ascontiguousarray(a.real)
+    return a
 
 
 @dataclass
@@ -93,6 +117,20 @@ class OracleModel:
     cb_Eh: np.ndarray = None
     cb_Eih: np.ndarray = None
 
+    def __post_init__(self):
+        # real symmetric exponentials stay real, so the products take half the flops
+        for k in ("mat", "invmat", "mat_half", "invmat_half"):
+            setattr(self, k, _real_if_real(getattr(self, k)))
+        self.scalars = [(int(self.op_P[n]), int(self.op_type[n]), complex(self.op_g[n]), float(self.op_E[n]),
+                         complex(self.op_alpha[n])) for n in range(len(self.op_P))]
+        # E_exp(1, sp) of every operator for sp in {-2, -1, 1, 2} (column sp + 2)
+        self._ee = np.zeros((len(self.op_P), 5), dtype=complex)
+        for n in range(len(self.op_P)):
+            for sp in (1, 2) if self.op_type[n] == 2 else (1,):
+                z = np.exp(self.op_g[n] * self.op_E[n] * phi(sp, self.op_type[n]))
+                self._ee[n, 2 + sp] = z
+                self._ee[n, 2 - sp] = 1.0 / z
+
     @property
     def n_op(self):
         return len(self.op_P)
@@ -109,32 +147,32 @@ class OracleModel:
     def mmthr(self, M):
         """Hop_mod_mmthr (Hop_mod.F90:143-160): exp(-dtau T) M, bonds Ncheck..1."""
         if self.cb_p is None:
-            return self.mat @ M
+            return lmul(self.mat, M)
         return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p) - 1, -1, -1), True)
 
     def mmthr_m1(self, M):
         """Hop_mod_mmthr_m1 (Hop_mod.F90:162-181): exp(dtau T) M, bonds 1..Ncheck."""
         if self.cb_p is None:
-            return self.invmat @ M
+            return lmul(self.invmat, M)
         return self._bonds(M.copy(), self.cb_Einv, range(len(self.cb_p)), True)
 
     def mmthl(self, M):
         """Hop_mod_mmthl (Hop_mod.F90:185-204): M exp(-dtau T), bonds 1..Ncheck."""
         if self.cb_p is None:
-            return M @ self.mat
+            return rmul(M, self.mat)
         return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p)), False)
 
     def mmthl_m1(self, M):
         """Hop_mod_mmthl_m1 (Hop_mod.F90:231-250): M exp(dtau T), bonds Ncheck..1."""
         if self.cb_p is None:
-            return M @ self.invmat
+            return rmul(M, self.invmat)
         return self._bonds(M.copy(), self.cb_Einv, range(len(self.cb_p) - 1, -1, -1), False)
 
     def mmthlc(self, M):
         """Hop_mod_mmthlc (Hop_mod.F90:208-227): lmult with bonds 1..Ncheck, i.e.
         exp(-dtau T)^+ M (for the dense Hermitian exponential the same as mmthr)."""
         if self.cb_p is None:
-            return self.mat @ M
+            return lmul(self.mat, M)
         return self._bonds(M.copy(), self.cb_E, range(len(self.cb_p)), True)
 
     def hop_symm(self, G):
@@ -142,7 +180,7 @@ class OracleModel:
         (RealExpOpT/CmplxExpOpT adjointaction with mat_1D2 and invmat_1D2, OpTTypes_mod.F90:135-147,
         236-248); dense: exp(-dtau T/2) G exp(dtau T/2)."""
         if self.cb_p is None:
-            return self.mat_half @ G @ self.invmat_half
+            return rmul(lmul(self.mat_half, G), self.invmat_half)
         M = G.copy()
         for b in range(len(self.cb_p) - 1, -1, -1):
             r = [int(self.cb_p[b, 0]), int(self.cb_p[b, 1])]
@@ -152,9 +190,7 @@ class OracleModel:
 
     def e_exp(self, n, sp):
         """E_exp(1, sp) of operator n (Operator_mod.F90:436-466)."""
-        s = abs(int(sp))
-        z = np.exp(self.op_g[n] * self.op_E[n] * phi(s, self.op_type[n]))
-        return z if sp > 0 else 1.0 / z
+        return self._ee[n, 2 + int(sp)]
 
     def stab_nt(self):
         """Stab_nt(0:NSTM) (main.F90:446-457)."""
@@ -465,12 +501,17 @@ class OracleSweep:
 
     # ---- per-slice pieces ------------------------------------------------
     def op_phase(self, z):
+        """Op_phase (Operator_mod.F90:160-181): z times exp(i Im(g alpha phi(f))) over all
+        operators and slices."""
         m = self.m
-        for n in range(m.n_op):
-            for nt in range(m.ltrot):
-                ang = (m.op_g[n] * m.op_alpha[n] * phi(self.f[n, nt], m.op_type[n])).imag
-                z = z * complex(math.cos(ang), math.sin(ang))
-        return z
+        if m.n_op == 0:
+            return z
+        phis = np.array([[phi(s, int(m.op_type[n])) if s in (-1, 1) or m.op_type[n] == 2 else 0.0
+                          for s in (-2, -1, 1, 2)] for n in range(m.n_op)])
+        phis = np.insert(phis, 2, 0.0, axis=1)          # column f + 2
+        ph = phis[np.arange(m.n_op)[:, None], self.f + 2]
+        ang = (m.op_g[:, None] * m.op_alpha[:, None] * ph).imag
+        return z * np.prod(np.cos(ang) + 1j * np.sin(ang))
 
     def flip(self, n, nt):
         typ = self.m.op_type[n]
@@ -483,24 +524,28 @@ class OracleSweep:
         """Upgrade2 for a 1x1 operator (upgrade_mod.F90:105-302), mode 'Final',
         S0_ratio = T0_proposal_ratio = 1."""
         m = self.m
-        P = int(m.op_P[n])
-        typ = m.op_type[n]
+        P, typ, g, E, alpha = m.scalars[n]       # Python scalars: the per-visit arithmetic
         s_old = int(self.f[n, nt])
-        Z1 = m.op_g[n] * (phi(hs_new, typ) - phi(s_old, typ))
-        myexp = np.exp(Z1 * m.op_E[n])
+        Z1 = g * (phi(hs_new, typ) - phi(s_old, typ))
+        myexp = cmath.exp(Z1 * E)
         Z = myexp - 1.0
-        D_mat = myexp + (-Z * G[P, P])
-        ratio = D_mat * np.exp(Z1 * m.op_alpha[n])
+        D_mat = myexp + (-Z * complex(G[P, P]))
+        ratio = D_mat * cmath.exp(Z1 * alpha)
         ratiotot = ratio ** float(m.n_sun) * gama(hs_new, typ) / gama(s_old, typ)
-        weight = abs((self.phase * ratiotot).real / self.phase.real)
+        phase = complex(self.phase)
+        weight = abs((phase * ratiotot).real / phase.real)
         toggle = weight > self.rng()
         if toggle:
-            self.phase = self.phase * ratiotot / np.sqrt(ratiotot * np.conj(ratiotot))
+            self.phase = phase * ratiotot / cmath.sqrt(ratiotot * ratiotot.conjugate())
             v = -G[P, :].copy()
             v[P] = 1.0 - G[P, P]
             x = Z / (1.0 + v[P] * Z)
             xp = G[:, P].copy()
-            G -= x * np.outer(xp, v)
+            if G.flags.c_contiguous:
+                # ZGERU (upgrade_mod.F90:269) in place: G^T is the column-major view of G
+                blas.zgeru(-x, v, xp, a=G.T, overwrite_a=1)
+            else:
+                G -= x * np.outer(xp, v)
             self.f[n, nt] = hs_new
         self.ctrl.upgrade(toggle)
         return toggle
