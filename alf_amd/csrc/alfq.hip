@@ -2126,6 +2126,83 @@ int alfq_kernel_timing_get(alfq_ctx* c, double* total_ms, int64_t* launches) {
   return ST_NONE;
 }
 
+}  // extern "C"
+
+// Staging memory of alfq_gemm_noalloc: one fixed array per device, three regions of
+// ALFQ_NOALLOC_OPERAND_BYTES (A, B, C).  Host code reaches it only through the device
+// address hipGetSymbolAddress returns, fetched once per device.
+static __device__ __align__(256) unsigned char scratch[3 * (size_t)ALFQ_NOALLOC_OPERAND_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's staging array (looked up on its first use there).
+int noalloc_scratch(char** out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) { alfq_set_error("gemm_noalloc: device index out of range"); return ST_GENERIC; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) { alfq_set_error("gemm_noalloc: no device address for the staging array"); return ST_GENERIC; }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (char*)g_scratch_ptr[dev];
+  return ST_NONE;
+}
+
+// The copy-in, product and copy-out of alfq_gemm on device buffers dA, dB, dC.
+int gemm_staged(int32_t M, int32_t N, int32_t K, int oa, int32_t a_is_real, const double* A, int32_t lda, size_t asz,
+                int ob, int32_t b_is_real, const double* B, int32_t ldb, size_t bsz, const double* alpha,
+                const double* beta, double* Cm, int32_t ldc, size_t csz, int32_t prec, void* dA, void* dB, void* dC) {
+  HIP_OK(hipMemcpy(dA, A, asz, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dB, B, bsz, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dC, Cm, csz, hipMemcpyHostToDevice));
+  GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
+                         make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
+  g.prec = prec == ALFQ_PREC_FP32_WRAPS ? 1 : 0;
+  int st = gemm_launch(g, 0);
+  if (!st) {
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(Cm, dC, csz, hipMemcpyDeviceToHost));
+  }
+  return st;
+}
+}  // namespace
+
+extern "C" {
+
+int alfq_gemm_noalloc(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
+                      char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
+                      const double* beta, double* Cm, int32_t ldc, int32_t prec) {
+  NULL_STREAM_LOCK;
+  if (M < 0 || N < 0 || K < 0 || !A || !B || !Cm || !alpha || !beta) {
+    alfq_set_error("gemm_noalloc: bad arguments");
+    return ST_GENERIC;
+  }
+  auto op = [](char o) { return (o == 'T' || o == 't') ? OP_T : ((o == 'C' || o == 'c') ? OP_C : OP_N); };
+  const int oa = op(opa), ob = op(opb);
+  const long long acols = (oa == OP_N) ? K : M, bcols = (ob == OP_N) ? N : K;
+  const size_t asz = (size_t)lda * acols * (a_is_real ? sizeof(double) : sizeof(cplx));
+  const size_t bsz = (size_t)ldb * bcols * (b_is_real ? sizeof(double) : sizeof(cplx));
+  const size_t csz = (size_t)ldc * N * sizeof(cplx);
+  const size_t cap = (size_t)ALFQ_NOALLOC_OPERAND_BYTES;
+  if (asz > cap || bsz > cap || csz > cap) {
+    alfq_set_error("gemm_noalloc: an operand exceeds the " + std::to_string(cap) +
+                   "-byte staging limit (A " + std::to_string(asz) + ", B " + std::to_string(bsz) + ", C " +
+                   std::to_string(csz) + " bytes)");
+    return ST_GENERIC;
+  }
+  char* base = nullptr;
+  int st = noalloc_scratch(&base);
+  if (st) return st;
+  return gemm_staged(M, N, K, oa, a_is_real, A, lda, asz, ob, b_is_real, B, ldb, bsz, alpha, beta, Cm, ldc, csz,
+                     prec, base, base + cap, base + 2 * cap);
+}
+
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha, const double* beta,
               double* Cm, int32_t ldc, int32_t prec) {
@@ -2138,20 +2215,15 @@ int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, cons
   const size_t bsz = (size_t)ldb * bcols * (b_is_real ? sizeof(double) : sizeof(cplx));
   const size_t csz = (size_t)ldc * N * sizeof(cplx);
   void *dA = nullptr, *dB = nullptr, *dC = nullptr;
-  HIP_OK(hipMalloc(&dA, asz ? asz : 16));
-  HIP_OK(hipMalloc(&dB, bsz ? bsz : 16));
-  HIP_OK(hipMalloc(&dC, csz ? csz : 16));
-  HIP_OK(hipMemcpy(dA, A, asz, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dB, B, bsz, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dC, Cm, csz, hipMemcpyHostToDevice));
-  GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
-                         make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
-  g.prec = prec == ALFQ_PREC_FP32_WRAPS ? 1 : 0;
-  int st = gemm_launch(g, 0);
-  if (!st) {
-    HIP_OK(hipDeviceSynchronize());
-    HIP_OK(hipMemcpy(Cm, dC, csz, hipMemcpyDeviceToHost));
+  if (hipMalloc(&dA, asz ? asz : 16) || hipMalloc(&dB, bsz ? bsz : 16) || hipMalloc(&dC, csz ? csz : 16)) {
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dC);
+    alfq_set_error("gemm: hipMalloc");
+    return ST_GENERIC;
   }
+  int st = gemm_staged(M, N, K, oa, a_is_real, A, lda, asz, ob, b_is_real, B, ldb, bsz, alpha, beta, Cm, ldc, csz,
+                       prec, dA, dB, dC);
   hipFree(dA);
   hipFree(dB);
   hipFree(dC);

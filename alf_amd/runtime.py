@@ -36,7 +36,7 @@ EXPORTS = [
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
-    "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get",
+    "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc",
 ]
 
 
@@ -149,6 +149,8 @@ def load_library(path: str = LIB_PATH):
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
                       C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32],
+        "alfq_gemm_noalloc": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
+                              C.c_char, C.c_int32, dp, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

@@ -177,6 +177,18 @@ const char* alfq_last_error(void);
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
               const double* beta, double* C, int32_t ldc, int32_t prec);
+/* alfq_gemm without any device allocation (no counterpart in the reference): for callers
+ * that cannot allocate device memory (serving loops and harnesses that forbid hipMalloc).
+ * Same arguments and result as alfq_gemm; A, B and C are staged through a fixed array of
+ * 3 x ALFQ_NOALLOC_OPERAND_BYTES per device that the library holds until the process ends.
+ * An operand larger than ALFQ_NOALLOC_OPERAND_BYTES (lda x columns x element size) is
+ * rejected with ERROR_GENERIC.  Every call on one device shares that device's array: calls
+ * that overlap on one device (from other processes or from code that bypasses this
+ * library's lock) race. */
+#define ALFQ_NOALLOC_OPERAND_BYTES (8 << 20)
+int alfq_gemm_noalloc(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A,
+                      int32_t lda, char opb, int32_t b_is_real, const double* B, int32_t ldb,
+                      const double* alpha, const double* beta, double* C, int32_t ldc, int32_t prec);
 /* Per-launch timing of the Metropolis chain kernel (start/stop events recorded
  * with each launch on the context's stream).  Enabling resets the record;
  * *total_ms is the summed kernel time of the recorded launches. */
