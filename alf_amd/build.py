@@ -7,7 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["alfq.hip", "gemm.hip", "qr.hip", "chain.hip", "rate.hip", "obs.hip", "checkerboard.hip", "tau.hip"]
+SOURCES = ["alfq.hip", "gemm.hip", "qr.hip", "chain.hip", "rate.hip", "obs.hip", "checkerboard.hip", "tau.hip",
+           "ozaki.hip"]
 OUT = os.path.join(HERE, "libalfq.so")
 
 HIPCC_FLAGS = [

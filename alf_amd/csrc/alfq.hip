@@ -25,6 +25,7 @@
 #include "alfq_cb.h"
 #include "alfq_chain.h"
 #include "alfq_obs.h"
+#include "alfq_oz.h"
 #include "alfq_qr.h"
 #include "alfq_tau.h"
 
@@ -166,6 +167,11 @@ struct alfq_ctx {
   bool t_real = true;
   int K = 32;
   int prec = ALFQ_PREC_FP64;             // arithmetic of the hopping wraps of G
+  // ALFQ_PREC_INT8_SPLIT: digit planes of exp(-dtau T) and exp(dtau T) as left operands (by
+  // row) and right operands (by column), and of the wrapped matrix, oz_cs bytes per chain
+  char* oz_buf = nullptr;
+  long long oz_cs = 0;
+  OzOperand oz_e_row, oz_e_col, oz_ei_row, oz_ei_col, oz_x;
   bool symm = false;                     // Symm = .T.: measurements on Hop_mod_Symm(G)
   // equal-time measurements
   ObsDev obs;
@@ -229,6 +235,7 @@ struct alfq_ctx {
   int* uoff_up = nullptr;                // uniforms consumed before visit v (ops ascending)
   int* uoff_dn = nullptr;                // the same for the descending visit order
   std::vector<int> h_op_type;
+  std::vector<int> h_op_site;
   // state
   int8_t* f = nullptr;
   cplx* G = nullptr;
@@ -499,6 +506,67 @@ int hop_ready(const alfq_ctx* c) {
   return ST_NONE;
 }
 
+// ALFQ_PREC_INT8_SPLIT: one digit-plane buffer per chain (oz_cs bytes) holding exp(-dtau T)
+// and exp(dtau T) split by row (left operands) and by column (right operands), split once
+// here since a chain's hopping is fixed, and the planes of the matrix being wrapped.
+int oz_setup(alfq_ctx* c) {
+  const int n = c->n, pe = c->t_real ? 1 : 3;
+  const long long kp = oz_pitch(n);
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t o = off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_d[5], o_e[5];
+  for (int q = 0; q < 5; ++q) {
+    o_d[q] = take(oz_bytes(n, n, q < 4 ? pe : 2));
+    o_e[q] = take((size_t)n * sizeof(int));
+  }
+  c->oz_cs = (long long)((off + 4095) & ~(size_t)4095);
+  if (hipMalloc(&c->oz_buf, (size_t)c->oz_cs * c->B) != hipSuccess) {
+    c->oz_buf = nullptr;
+    alfq_set_error("precision: hipMalloc of the digit planes");
+    return ST_GENERIC;
+  }
+  OzOperand* ops[5] = {&c->oz_e_row, &c->oz_e_col, &c->oz_ei_row, &c->oz_ei_col, &c->oz_x};
+  for (int q = 0; q < 5; ++q) {
+    ops[q]->d = (int8_t*)(c->oz_buf + o_d[q]);
+    ops[q]->e = (int*)(c->oz_buf + o_e[q]);
+    ops[q]->rows = n;
+    ops[q]->planes = q < 4 ? pe : 2;
+    ops[q]->kp = kp;
+    ops[q]->stride = c->oz_cs;
+  }
+  int st;
+  if ((st = oz_split(c->expT, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_e_row, c->B, c->s)) ||
+      (st = oz_split(c->expT, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_e_col, c->B, c->s)) ||
+      (st = oz_split(c->expTinv, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_ei_row, c->B, c->s)) ||
+      (st = oz_split(c->expTinv, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_ei_col, c->B, c->s))) {
+    hipStreamSynchronize(c->s);
+    hipFree(c->oz_buf);
+    c->oz_buf = nullptr;
+    return st;
+  }
+  return ST_NONE;
+}
+
+// out = rs (E X) cs with E split by row, X (n x n complex) split by column here
+int oz_left(alfq_ctx* c, const OzOperand& E, const cplx* X, cplx* out, const cplx* rs, const cplx* cs) {
+  const int n = c->n;
+  int st = oz_split(X, true, n, n, n, false, nullptr, c->cs, c->oz_x, c->B, c->s);
+  return st ? st : oz_gemm(n, n, n, E, c->oz_x, out, n, ONE, ZERO, rs, cs, c->cs, c->B, c->s);
+}
+
+// out = rs (X K E) cs with X (n x n complex) split by row here, its columns scaled by the
+// diagonal K (kscale, may be null), and E split by column
+int oz_right(alfq_ctx* c, const cplx* X, const cplx* kscale, const OzOperand& E, cplx* out, const cplx* rs,
+             const cplx* cs) {
+  const int n = c->n;
+  int st = oz_split(X, true, n, n, n, true, kscale, c->cs, c->oz_x, c->B, c->s);
+  return st ? st : oz_gemm(n, n, n, c->oz_x, E, out, n, ONE, ZERO, rs, cs, c->cs, c->B, c->s);
+}
+
 int wrapgrup_dev(alfq_ctx* c, int ntau) {
   int st;
   if ((st = hop_ready(c))) return st;
@@ -513,6 +581,10 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
     if ((st = cb_right(c->Gtmp, c->H, c->n, nullptr, nullptr, c->dv_d, c->dv_dinv, c->cb_inv, 1, c->B, c->cs,
                        c->s)))
       return st;
+  } else if (c->prec == ALFQ_PREC_INT8_SPLIT) {
+    // H = exp(-dtau T) G;  G = D (H exp(dtau T)) D^-1, both on the int8 split products
+    if ((st = oz_left(c, c->oz_e_row, c->G, c->H, nullptr, nullptr))) return st;
+    if ((st = oz_right(c, c->H, nullptr, c->oz_ei_col, c->Gtmp, c->dv_d, c->dv_dinv))) return st;
   } else {
     // H = exp(-dtau T) G
     GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
@@ -547,6 +619,10 @@ int wrapgrdo_dev(alfq_ctx* c, int ntau) {
     if ((st = cb_right(c->H, c->G, c->n, c->dv_dinv, c->dv_d, nullptr, nullptr, c->cb, 0, c->B, c->cs, c->s)))
       return st;
     if ((st = cb_left(c->Gtmp, c->H, c->n, nullptr, nullptr, c->cb_inv, 0, c->B, c->cs, c->s))) return st;
+  } else if (c->prec == ALFQ_PREC_INT8_SPLIT) {
+    // H = D^-1 (G D) exp(-dtau T);  G = exp(dtau T) H, both on the int8 split products
+    if ((st = oz_right(c, c->G, c->dv_d, c->oz_e_col, c->H, c->dv_dinv, nullptr))) return st;
+    if ((st = oz_left(c, c->oz_ei_row, c->H, c->Gtmp, nullptr, nullptr))) return st;
   } else {
     // H = D^-1 G D exp(-dtau T)   (Op_Wrapdo type 1 with the new fields, then Hop_mod_mmthl)
     GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->G, c->n, OP_N, true, c->expT, c->n, OP_N, !c->t_real, c->H, c->n,
@@ -1194,6 +1270,7 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
     for (int i = 0; i < no; ++i) sops[fillp[site[i]]++] = i;
   }
   c->h_op_type = typ;
+  c->h_op_site = site;
   c->per_visit_sum = 0;
   for (int i = 0; i < no; ++i) c->per_visit_sum += (typ[i] == 2) ? 3 : 2;
   c->u_cap = c->u_arena_cap = 2LL * c->L * c->per_visit_sum;
@@ -1290,6 +1367,64 @@ int alfq_chain_bytes(const alfq_model_desc* d, int64_t* bytes) {
   return ST_NONE;
 }
 
+int alfq_chain_load(alfq_ctx* dst, int32_t dst_chain, alfq_ctx* src, int32_t src_chain) {
+  NULL_STREAM_LOCK;
+  if (!dst || !src || dst == src) { alfq_set_error("chain_load: two distinct contexts are needed"); return ST_GENERIC; }
+  if (dst_chain < 0 || dst_chain >= dst->B || src_chain < 0 || src_chain >= src->B) {
+    alfq_set_error("chain_load: chain index outside the batch");
+    return ST_GENERIC;
+  }
+  if (dst->n != src->n || dst->L != src->L || dst->W != src->W || dst->n_op != src->n_op ||
+      dst->n_sun != src->n_sun || dst->t_real != src->t_real || dst->symm != src->symm || dst->cs != src->cs ||
+      dst->pool.size() != src->pool.size() || dst->h_op_type != src->h_op_type || dst->h_op_site != src->h_op_site) {
+    alfq_set_error("chain_load: the contexts differ in shape or operator structure");
+    return ST_HAMILTONIAN;
+  }
+  if (dst->obs.buf || src->obs.buf || dst->tau.mem || src->tau.mem || dst->cb_buf || src->cb_buf) {
+    alfq_set_error("chain_load: contexts with measurements or a checkerboard Op_T cannot exchange chains");
+    return ST_GENERIC;
+  }
+  if (dst->rng_mode != 2 || src->rng_mode != 2) {
+    alfq_set_error("chain_load: both contexts need Philox streams");
+    return ST_GENERIC;
+  }
+  HIP_OK(hipStreamSynchronize(src->s));
+  // the chain's whole arena (model tables, fields, G, UDV slots, chain state, scalars, log)
+  char* d0 = dst->arena + (long long)dst_chain * dst->cs;
+  const char* s0 = src->arena + (long long)src_chain * src->cs;
+  HIP_OK(hipMemcpyAsync(d0, s0, (size_t)dst->cs, hipMemcpyDeviceToDevice, dst->s));
+  // the host's buffer and slot bindings may differ between the contexts: the current G and
+  // every bound UDV variable go where the destination's bindings point
+  auto rel = [](const alfq_ctx* c, const void* p) { return (long long)((const char*)p - c->arena); };
+  const size_t gbytes = (size_t)dst->n * dst->n * sizeof(cplx);
+  if (rel(dst, dst->G) != rel(src, src->G))
+    HIP_OK(hipMemcpyAsync(d0 + rel(dst, dst->G), s0 + rel(src, src->G), gbytes, hipMemcpyDeviceToDevice, dst->s));
+  auto move_var = [&](int vd, int vs) -> int {
+    if (vd < 0 || vs < 0 || vd == vs) return ST_NONE;
+    const UDVSlot &a = dst->pool[vd], &b = src->pool[vs];
+    HIP_OK(hipMemcpyAsync(d0 + rel(dst, a.U), s0 + rel(src, b.U), gbytes, hipMemcpyDeviceToDevice, dst->s));
+    HIP_OK(hipMemcpyAsync(d0 + rel(dst, a.V), s0 + rel(src, b.V), gbytes, hipMemcpyDeviceToDevice, dst->s));
+    HIP_OK(hipMemcpyAsync(d0 + rel(dst, a.D), s0 + rel(src, b.D), dst->n * sizeof(cplx), hipMemcpyDeviceToDevice,
+                          dst->s));
+    HIP_OK(hipMemcpyAsync(d0 + rel(dst, a.sc), s0 + rel(src, b.sc), 2 * sizeof(cplx), hipMemcpyDeviceToDevice, dst->s));
+    return ST_NONE;
+  };
+  int st;
+  if ((st = move_var(dst->var_r, src->var_r)) || (st = move_var(dst->var_l, src->var_l))) return st;
+  for (size_t k = 0; k < dst->var_st.size() && k < src->var_st.size(); ++k)
+    if ((st = move_var(dst->var_st[k], src->var_st[k]))) return st;
+  if (dst->oz_buf) {
+    // the destination's digit planes of exp(-dtau T) and exp(dtau T), re-split for every chain
+    const int n = dst->n;
+    if ((st = oz_split(dst->expT, !dst->t_real, n, n, n, true, nullptr, dst->cs, dst->oz_e_row, dst->B, dst->s)) ||
+        (st = oz_split(dst->expT, !dst->t_real, n, n, n, false, nullptr, dst->cs, dst->oz_e_col, dst->B, dst->s)) ||
+        (st = oz_split(dst->expTinv, !dst->t_real, n, n, n, true, nullptr, dst->cs, dst->oz_ei_row, dst->B, dst->s)) ||
+        (st = oz_split(dst->expTinv, !dst->t_real, n, n, n, false, nullptr, dst->cs, dst->oz_ei_col, dst->B, dst->s)))
+      return st;
+  }
+  return ST_NONE;
+}
+
 int alfq_set_chain(alfq_ctx* c, int32_t chain) {
   if (!c) return ST_GENERIC;
   if (chain < 0 || chain >= c->B) { alfq_set_error("chain index outside the batch"); return ST_GENERIC; }
@@ -1314,6 +1449,7 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (c->tau.mem) hipFree(c->tau.mem);
   if (c->tau.acc) hipFree(c->tau.acc);
   if (c->cb_sites) hipFree(c->cb_sites);
+  if (c->oz_buf) hipFree(c->oz_buf);
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape,
                   c->stat_dev};
   for (void* p : ptrs)
@@ -1336,9 +1472,14 @@ int alfq_set_delay(alfq_ctx* c, int32_t k) {
 
 int alfq_set_precision(alfq_ctx* c, int32_t mode) {
   if (!c) return ST_GENERIC;
-  if (mode != ALFQ_PREC_FP64 && mode != ALFQ_PREC_FP32_WRAPS) {
+  if (mode != ALFQ_PREC_FP64 && mode != ALFQ_PREC_FP32_WRAPS && mode != ALFQ_PREC_INT8_SPLIT) {
     alfq_set_error("precision: unknown mode");
     return ST_GENERIC;
+  }
+  if (mode == ALFQ_PREC_INT8_SPLIT && !c->oz_buf) {
+    NULL_STREAM_LOCK;
+    int st = oz_setup(c);
+    if (st) return st;
   }
   c->prec = mode;
   return ST_NONE;
@@ -2154,6 +2295,51 @@ int noalloc_scratch(char** out) {
   return ST_NONE;
 }
 
+// C = alpha A B + beta C on the int8 split products, with digit planes allocated here
+// (op(A) = A and op(B) = B; at least one operand complex).
+int gemm_split_dev(int32_t M, int32_t N, int32_t K, int oa, int32_t a_is_real, const void* dA, int32_t lda, int ob,
+                   int32_t b_is_real, const void* dB, int32_t ldb, cplx alpha, cplx beta, cplx* dC, int32_t ldc) {
+  if (oa != OP_N || ob != OP_N || (a_is_real && b_is_real)) {
+    alfq_set_error("gemm: the int8 split products take op = 'N' and at least one complex operand");
+    return ST_GENERIC;
+  }
+  if (M <= 0 || N <= 0) return ST_NONE;
+  OzOperand a, b;
+  a.rows = M;
+  a.planes = a_is_real ? 1 : (b_is_real ? 2 : 3);
+  b.rows = N;
+  b.planes = b_is_real ? 1 : 2;
+  a.kp = b.kp = oz_pitch(K);
+  const size_t ad = oz_bytes(M, K, a.planes), bd = oz_bytes(N, K, b.planes);
+  char* buf = nullptr;
+  const size_t aoff = 0, aeoff = (ad + 255) & ~(size_t)255, boff = (aeoff + M * sizeof(int) + 255) & ~(size_t)255,
+               beoff = (boff + bd + 255) & ~(size_t)255;
+  if (hipMalloc(&buf, beoff + N * sizeof(int)) != hipSuccess) {
+    alfq_set_error("gemm: hipMalloc of the digit planes");
+    return ST_GENERIC;
+  }
+  a.d = (int8_t*)(buf + aoff);
+  a.e = (int*)(buf + aeoff);
+  b.d = (int8_t*)(buf + boff);
+  b.e = (int*)(buf + beoff);
+  int st = 0;
+  if (K > 0) {
+    if (!(st = oz_split(dA, !a_is_real, lda, M, K, true, nullptr, 0, a, 1, 0)) &&
+        !(st = oz_split(dB, !b_is_real, ldb, N, K, false, nullptr, 0, b, 1, 0)))
+      st = oz_gemm(M, N, K, a, b, dC, ldc, alpha, beta, nullptr, nullptr, 0, 1, 0);
+  } else {
+    hipMemset(a.e, 0, M * sizeof(int));
+    hipMemset(b.e, 0, N * sizeof(int));
+    st = oz_gemm(M, N, 0, a, b, dC, ldc, alpha, beta, nullptr, nullptr, 0, 1, 0);
+  }
+  if (hipDeviceSynchronize() != hipSuccess && !st) {
+    alfq_set_error("gemm: split product failed");
+    st = ST_GENERIC;
+  }
+  hipFree(buf);
+  return st;
+}
+
 // The copy-in, product and copy-out of alfq_gemm on device buffers dA, dB, dC.
 int gemm_staged(int32_t M, int32_t N, int32_t K, int oa, int32_t a_is_real, const double* A, int32_t lda, size_t asz,
                 int ob, int32_t b_is_real, const double* B, int32_t ldb, size_t bsz, const double* alpha,
@@ -2161,10 +2347,16 @@ int gemm_staged(int32_t M, int32_t N, int32_t K, int oa, int32_t a_is_real, cons
   HIP_OK(hipMemcpy(dA, A, asz, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dB, B, bsz, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dC, Cm, csz, hipMemcpyHostToDevice));
-  GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
-                         make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
-  g.prec = prec == ALFQ_PREC_FP32_WRAPS ? 1 : 0;
-  int st = gemm_launch(g, 0);
+  int st;
+  if (prec == ALFQ_PREC_INT8_SPLIT) {
+    st = gemm_split_dev(M, N, K, oa, a_is_real, dA, lda, ob, b_is_real, dB, ldb, make_double2(alpha[0], alpha[1]),
+                        make_double2(beta[0], beta[1]), (cplx*)dC, ldc);
+  } else {
+    GemmArgs g = gemm_args(M, N, K, dA, lda, oa, !a_is_real, dB, ldb, ob, !b_is_real, dC, ldc,
+                           make_double2(alpha[0], alpha[1]), make_double2(beta[0], beta[1]));
+    g.prec = prec == ALFQ_PREC_FP32_WRAPS ? 1 : 0;
+    st = gemm_launch(g, 0);
+  }
   if (!st) {
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(Cm, dC, csz, hipMemcpyDeviceToHost));
@@ -2189,6 +2381,10 @@ int alfq_gemm_noalloc(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_re
   const size_t asz = (size_t)lda * acols * (a_is_real ? sizeof(double) : sizeof(cplx));
   const size_t bsz = (size_t)ldb * bcols * (b_is_real ? sizeof(double) : sizeof(cplx));
   const size_t csz = (size_t)ldc * N * sizeof(cplx);
+  if (prec == ALFQ_PREC_INT8_SPLIT) {
+    alfq_set_error("gemm_noalloc: the int8 split products need digit-plane workspace; use alfq_gemm");
+    return ST_GENERIC;
+  }
   const size_t cap = (size_t)ALFQ_NOALLOC_OPERAND_BYTES;
   if (asz > cap || bsz > cap || csz > cap) {
     alfq_set_error("gemm_noalloc: an operand exceeds the " + std::to_string(cap) +

@@ -32,6 +32,11 @@ program issues a bounded number of dispatches (about 5000 per sweep of one chain
 
     python3 -m alf_amd.kernelbench counters [--batch B] [--checkerboard]
 
+The ``traffic`` mode turns the two counter passes and the two calibration passes (their
+rocprofv3 counter_collection.csv files) into the per-kernel table of bytes per dispatch:
+
+    python3 -m alf_amd.kernelbench traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv
+
 The ``shim`` mode times the up-pass slices of the same workload called one at a time, as
 main.F90 calls WRAPGRUP: with G kept on the device, and with G uploaded and downloaded
 around every call (the alfq_WRAPGRUP_shim form, whose rate includes PCIe):
@@ -60,6 +65,16 @@ def measured_traffic(path: str = TRAFFIC_SUMMARY):
     return float(m.group(1)) if m else None
 
 
+def measured_traffic_counters(path: str = TRAFFIC_SUMMARY):
+    """Which counters the committed traffic figure sums ("FETCH_SIZE only" or both)."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_counters:\s*(.+)", text)
+    return m.group(1).strip() if m else None
+
+
 def measure_dominant(eng, sweeps: int = 1) -> dict:
     c0 = eng.control()
     eng.kernel_timing(True)
@@ -80,6 +95,7 @@ def measure_dominant(eng, sweeps: int = 1) -> dict:
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(),
+        "traffic_counters": measured_traffic_counters(),
         "kernel": "chain_block_kernel (Metropolis decisions + delayed rank-1 updates)",
         "avg_launch_us": avg_s * 1e6,
         "launches": launches,
@@ -195,6 +211,50 @@ def counter_run(batch: int, checkerboard: bool) -> None:
         b.close()
 
 
+def _counter_rows(path: str):
+    import csv
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            yield row["Kernel_Name"], row["Counter_Name"], float(row["Counter_Value"])
+
+
+def traffic_summary(fetch_csv: str, write_csv, copy_fetch_csv: str, copy_write_csv, copy_mib: int) -> str:
+    """Per-kernel HBM bytes per dispatch from two counter passes (FETCH_SIZE, WRITE_SIZE, in KiB
+    per dispatch), each scaled by what the same counter reads for a device-to-device copy of
+    copy_mib MiB (which reads and writes exactly that many bytes).  write_csv / copy_write_csv
+    may be "-" when only the FETCH_SIZE pass exists."""
+    if write_csv == "-" or copy_write_csv == "-":
+        write_csv = copy_write_csv = None
+    def calib(path, counter):
+        vals = [v for k, c, v in _counter_rows(path) if c == counter and "copyBuffer" in k]
+        if not vals:
+            raise RuntimeError(f"no copy dispatch in {path}")
+        return (copy_mib << 20) / (max(vals) * 1024.0)     # bytes per counted KiB
+    f_scale = calib(copy_fetch_csv, "FETCH_SIZE")
+    w_scale = calib(copy_write_csv, "WRITE_SIZE") if write_csv else float("nan")
+    agg = {}
+    passes = [(fetch_csv, "FETCH_SIZE", f_scale)] + ([(write_csv, "WRITE_SIZE", w_scale)] if write_csv else [])
+    for path, counter, scale in passes:
+        for k, c, v in _counter_rows(path):
+            if c != counter:
+                continue
+            name = k.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0].strip()
+            a = agg.setdefault(name, {"FETCH_SIZE": [], "WRITE_SIZE": []})
+            a[counter].append(v * 1024.0 * scale)
+    lines = [f"copy calibration: FETCH_SIZE x {f_scale:.4f}, WRITE_SIZE x {w_scale:.4f} "
+             f"(bytes per counted KiB, from a {copy_mib} MiB device-to-device copy)", "",
+             "| Kernel | Dispatches | Read bytes / dispatch | Written bytes / dispatch |", "|---|---|---|---|"]
+    for name, a in sorted(agg.items(), key=lambda kv: -sum(kv[1]["FETCH_SIZE"])):
+        nf, nw = len(a["FETCH_SIZE"]), len(a["WRITE_SIZE"])
+        rf = sum(a["FETCH_SIZE"]) / max(nf, 1)
+        rw = sum(a["WRITE_SIZE"]) / nw if nw else float("nan")
+        lines.append(f"| `{name}` | {max(nf, nw)} | {rf:.4g} | {rw:.4g} |")
+        if name.startswith("chain_block_kernel"):
+            lines.insert(0, f"traffic_bytes_per_launch: {rf + (rw if nw else 0.0):.6g}")
+            lines.insert(1, f"traffic_counters: {'FETCH_SIZE + WRITE_SIZE' if nw else 'FETCH_SIZE only'}")
+    return "\n".join(lines)
+
+
 def shim_slices() -> None:
     """Per-slice time of alfq_wrapgrup with G on the device and with G crossing PCIe both
     ways around every call (PHASE is returned by every call in both forms)."""
@@ -242,6 +302,10 @@ if __name__ == "__main__":
     elif len(sys.argv) >= 2 and sys.argv[1] == "counters":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
         counter_run(arg("--batch", 64), "--checkerboard" in sys.argv)
+    elif len(sys.argv) >= 2 and sys.argv[1] == "traffic":
+        # traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv [--mib N]
+        mib = int(sys.argv[sys.argv.index("--mib") + 1]) if "--mib" in sys.argv else 256
+        print(traffic_summary(*sys.argv[2:6], copy_mib=mib))
     elif len(sys.argv) >= 2 and sys.argv[1] == "shim":
         shim_slices()
     else:

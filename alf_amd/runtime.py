@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libalfq.so")
 ALFQ_UDVR = -1
 ALFQ_UDVL = -2
 
-PREC_FP64, PREC_FP32_WRAPS = 0, 1     # alfq_set_precision modes
+PREC_FP64, PREC_FP32_WRAPS, PREC_INT8_SPLIT = 0, 1, 2     # alfq_set_precision modes
 
 ERROR_NAMES = {0: "ERROR_NONE", 1: "ERROR_GENERIC", 2: "ERROR_FILE_NOT_FOUND", 3: "ERROR_UNSTABLE_MATRIX",
                4: "ERROR_MISSING_OBS", 5: "ERROR_FIELDS", 6: "ERROR_HAMILTONIAN"}
@@ -36,7 +36,7 @@ EXPORTS = [
     "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
-    "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc",
+    "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc", "alfq_chain_load",
 ]
 
 
@@ -145,6 +145,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_sweep_graphs": [P, C.c_int32],
         "alfq_ctx_create_batch": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(P)],
         "alfq_set_chain": [P, C.c_int32],
+        "alfq_chain_load": [P, C.c_int32, P, C.c_int32],
         "alfq_batch_size": [P, C.POINTER(C.c_int32)],
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
@@ -340,7 +341,8 @@ class Engine:
                 "den0": den0, "sign_eq": s.sign_eq, "n_eq": s.n_eq}
 
     def set_precision(self, mode):
-        """PREC_FP64 (drop-in) or PREC_FP32_WRAPS (hopping wraps of G on FP32 MFMA)."""
+        """PREC_FP64 (drop-in), PREC_FP32_WRAPS (hopping wraps of G on FP32 MFMA) or
+        PREC_INT8_SPLIT (the same wraps as exact int8 digit products summed in FP64)."""
         self._c(self.lib.alfq_set_precision(self.ctx, int(mode)))
 
     def set_fields(self, f):
@@ -536,6 +538,12 @@ class BatchEngine(Engine):
         """Select chain z for the per-chain accessors; returns self for chaining calls."""
         self._c(self.lib.alfq_set_chain(self.ctx, int(z)))
         return self
+
+    def load_chain(self, z, src, src_chain=0):
+        """Chain src_chain of the context `src` (same shape, initialised) replaces chain z
+        (alfq_chain_load): a new chain joins the running batch at a sweep boundary."""
+        self._c(self.lib.alfq_chain_load(self.ctx, int(z), src.ctx, int(src_chain)))
+        self.models[z] = src.models[src_chain] if hasattr(src, "models") else src.model
 
 
 def sweep_many(engines, nsweep=1):

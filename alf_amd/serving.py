@@ -2,24 +2,29 @@
 
 ALF itself runs one parameter set per job and one Markov chain per MPI rank (no
 counterpart in the reference).  Here a request asks for ``nsweeps`` sweeps of a Hubbard
-model (lattice, linear size, beta, U, seed) and the loop serves a stream of them:
+model (lattice, linear size, beta, U, seed) and the loop serves a stream of them with
+continuous batching:
 
-* Requests of the same shape (lattice, size, Ltrot, Nwrap, real or complex hopping) can
-  share a batched context (one ``BatchEngine``: every kernel launch covers all its
-  chains); couplings and seeds differ per chain.
-* A queued request is admitted as soon as the device memory its chains need
-  (``alfq_chain_bytes`` per chain) fits in the budget beside the batches already running.
-  The queue is served in arrival order; the head request takes every queued request of its
-  shape (up to ``max_batch``) into one new batch.  Requests that arrive while others run
-  therefore start at once in a batch of their own, and their sweeps overlap on the device
-  with the running batches: each batch is driven from a host thread of its own (model
-  set-up, init, sweeps, download) on its own HIP stream, so batches progress independently.
-* A batch whose sweeps are done returns each chain's fields, G, phase and precision
-  statistics to the host and frees its memory.
+* Requests of one shape (lattice, size, Ltrot, Nwrap, real or complex hopping) run in one
+  lane: a batched context (``BatchEngine``: every kernel launch covers all its slots) driven
+  by a host thread of its own on its own HIP stream, sweeping one sweep at a time; couplings
+  and seeds differ per slot.  Lanes of different shapes overlap on the device.
+* A request joins its shape's running lane at the next sweep boundary as soon as a slot is
+  free: it is initialised in a small context of its own (model set-up, fields, Philox stream,
+  the UDV stack and G at tau = 0) and moved into the free slot (``alfq_chain_load``); the
+  other slots keep sweeping.  A slot whose request has its sweeps returns the chain's fields,
+  G, phase and precision statistics to the host and becomes free.  Free slots keep a copy of
+  a chain and sweep it unread until a request takes them.
+* A shape without a lane gets a new one as soon as the device memory of its slots
+  (``alfq_chain_bytes`` per slot) fits in the budget beside the lanes already running; its
+  slot count is the number of its requests waiting then (at least ``min_slots``, at most
+  ``max_batch``).  A lane with no request in its slots and none waiting closes and frees its
+  memory.
 
 Time-to-result of a request = its arrival -> its result on the host.  A chain's trajectory
-does not depend on the batch it runs in (chains are independent and seeded per request), so
-a request's result is the same alone and in any mix.
+does not depend on the lane, the slot or the other chains (chains are independent, seeded
+per request, and move between contexts with their whole state), so a request's result is the
+same alone and in any mix.
 """
 from __future__ import annotations
 
@@ -81,9 +86,6 @@ def device_free_bytes() -> int:
     return int(free.value)
 
 
-GRAPH_MIN_SWEEPS = 32        # requests with fewer sweeps run them eagerly
-
-
 class _HubbardBackend:
     """Builds models and batched engines of the gfx950 sweep engine."""
 
@@ -110,24 +112,35 @@ class _HubbardBackend:
     def chain_bytes(self, r: Request) -> int:
         return self.rt.chain_bytes(self.model(r))
 
-    def start(self, reqs):
-        hams = [self.model(r) for r in reqs]
+    def prepare(self, r: Request):
+        """A single-chain context of the request, initialised (G and the UDV stack at tau = 0)."""
+        h = self.model(r)
+        eng = self.rt.Engine(h, delay=self.delay)
+        with self._lock:
+            h.nsigma.set_uniform_four_state(r.seed)
+            f = h.nsigma.f.copy()
+        eng.set_fields(f)
+        eng.set_philox(seed=r.seed, stream=0)
+        eng.init()
+        return eng
+
+    def lane(self, reqs, nslot: int):
+        """A batched context of nslot slots; the requests take the first slots, the rest hold
+        copies of the first request's chain."""
+        fill = [reqs[min(z, len(reqs) - 1)] for z in range(nslot)]
+        hams = [self.model(r) for r in fill]
         eng = self.rt.BatchEngine(hams, delay=self.delay)
-        for z, (r, h) in enumerate(zip(reqs, hams)):
+        for z, r in enumerate(fill):
             with self._lock:
-                h.nsigma.set_uniform_four_state(r.seed)
-                f = h.nsigma.f.copy()
+                hams[z].nsigma.set_uniform_four_state(r.seed)
+                f = hams[z].nsigma.f.copy()
             eng.chain(z).set_fields(f)
             eng.chain(z).set_philox(seed=r.seed, stream=0)
         eng.init()
         return eng
 
-    def run(self, eng, nsweeps: int):
-        # capturing a whole-sweep graph (two parities) costs more than a few eager sweeps
-        # save, and captures of concurrent batches take turns
-        if nsweeps < GRAPH_MIN_SWEEPS:
-            eng.sweep_graphs(False)
-        eng.sweep(nsweeps)
+    def sweep(self, eng):
+        eng.sweep(1)
         eng.sync()
 
     def collect(self, eng, z: int):
@@ -135,100 +148,172 @@ class _HubbardBackend:
         return eng.get_fields(), eng.get_green(), eng.get_phase(), eng.control()
 
 
-@dataclass
-class _Batch:
-    reqs: list
-    nbytes: int
-    admitted: float
+class _Lane:
+    """One shape's batched context and the host thread that drives it."""
+
+    def __init__(self, server, key, reqs, nslot, nbytes):
+        self.server, self.key, self.nslot, self.nbytes = server, key, nslot, nbytes
+        self.waiting = list(reqs)           # requests of this shape not yet in a slot
+        self.cv = threading.Condition()
+        self.closed = False
+        self.thread = threading.Thread(target=self._run, daemon=True)
+
+    def offer(self, r) -> bool:
+        """Queue a request for this lane; False when the lane has already closed."""
+        with self.cv:
+            if self.closed:
+                return False
+            self.waiting.append(r)
+            return True
+
+    def _run(self):
+        srv, be = self.server, self.server.backend
+        try:
+            with self.cv:
+                first = self.waiting[:self.nslot]
+                del self.waiting[:len(first)]
+            eng = be.lane(first, self.nslot)
+            t_adm = srv.clock() - srv.t0
+            slots = {z: [r, r.nsweeps, t_adm] for z, r in enumerate(first)}
+            try:
+                while True:
+                    if slots:
+                        be.sweep(eng)
+                    done = []
+                    for z, s in list(slots.items()):
+                        s[1] -= 1
+                        if s[1] <= 0:
+                            f, g, ph, ctl = be.collect(eng, z)
+                            r = s[0]
+                            done.append(Result(r.rid, r.arrival, s[2], srv.clock() - srv.t0, r.nsweeps,
+                                               self.nslot, f, g, ph, ctl))
+                            del slots[z]
+                    if done:
+                        srv.done.put(("results", self, done))
+                    # refill free slots at this sweep boundary
+                    while True:
+                        with self.cv:
+                            free = [z for z in range(self.nslot) if z not in slots]
+                            if not free or not self.waiting:
+                                if not slots and not self.waiting:
+                                    self.closed = True
+                                break
+                            r = self.waiting.pop(0)
+                        src = be.prepare(r)
+                        try:
+                            eng.load_chain(free[0], src)
+                        finally:
+                            src.close()
+                        slots[free[0]] = [r, r.nsweeps, srv.clock() - srv.t0]
+                    if self.closed:
+                        break
+                    srv.lane_peak = max(srv.lane_peak, len(slots))
+            finally:
+                eng.close()
+            srv.done.put(("closed", self, None))
+        except Exception as exc:          # reported by the loop
+            with self.cv:
+                self.closed = True
+            srv.done.put(("error", self, exc))
 
 
 class Server:
     """Continuous-batching loop over a list of timed requests (see the module docstring)."""
 
-    def __init__(self, memory_budget: int | None = None, max_batch: int = 64, backend=None, clock=time.perf_counter):
+    def __init__(self, memory_budget: int | None = None, max_batch: int = 64, backend=None, clock=time.perf_counter,
+                 min_slots: int = 2):
         self.backend = backend if backend is not None else _HubbardBackend()
         self.budget = memory_budget if memory_budget is not None else int(0.8 * device_free_bytes())
         self.max_batch = max_batch
+        self.min_slots = min_slots
         self.clock = clock
-        self.peak_batches = 0
+        self.peak_batches = 0              # lanes running at once
+        self.lane_peak = 0                 # occupied slots of one lane at a sweep boundary
+        self.joined = 0                    # requests that joined a lane already running
         self.wall = 0.0
-
-    def _work(self, b: _Batch, t0: float, done):
-        """One batch from admission to results: models, context, init, sweeps, download."""
-        try:
-            eng = self.backend.start(b.reqs)
-            try:
-                self.backend.run(eng, b.reqs[0].nsweeps)
-                out = []
-                for z, r in enumerate(b.reqs):
-                    f, g, ph, ctl = self.backend.collect(eng, z)
-                    out.append(Result(r.rid, r.arrival, b.admitted, self.clock() - t0, r.nsweeps, len(b.reqs),
-                                      f, g, ph, ctl))
-            finally:
-                eng.close()
-            done.put((b, out, None))
-        except Exception as exc:          # reported by the loop
-            done.put((b, None, exc))
 
     def run(self, requests) -> list:
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        queue, results, workers = [], [], {}
-        done = _queue.Queue()
+        queue, results, lanes = [], [], {}
+        self.done = _queue.Queue()
         used = 0
         per_chain = {}
-        t0 = self.clock()
-        while pending or queue or workers:
-            now = self.clock() - t0
+        self.t0 = self.clock()
+        while pending or queue or lanes:
+            now = self.clock() - self.t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.pop(0))
-            # admission in arrival order; the head request takes the queued requests of its shape
+            # a request of a shape with a running lane joins it; the others open lanes by memory
+            rest = []
+            for r in queue:
+                ln = lanes.get(r.shape())
+                if ln is not None and ln.offer(r):
+                    self.joined += 1
+                else:
+                    rest.append(r)
+            queue = rest
             while queue:
                 head = queue[0]
-                same = [r for r in queue if r.shape() == head.shape() and r.nsweeps == head.nsweeps]
-                same = same[:self.max_batch]
                 key = head.shape()
+                if key in lanes:
+                    break                   # its lane is closing: wait for it
+                same = [r for r in queue if r.shape() == key]
                 if key not in per_chain:
                     per_chain[key] = self.backend.chain_bytes(head)
-                need = per_chain[key] * len(same)
+                nslot = min(self.max_batch, max(len(same), self.min_slots))
+                if used + per_chain[key] * nslot > self.budget:
+                    nslot = max(len(same), 1)
+                need = per_chain[key] * nslot
                 if used + need > self.budget:
-                    if workers:
+                    if lanes:
                         break               # wait for memory to free up
                     if per_chain[key] > self.budget:
                         raise RuntimeError(f"request {head.rid} needs {per_chain[key]} B, above the budget")
-                    same = same[:max(1, self.budget // per_chain[key])]
-                    need = per_chain[key] * len(same)
-                b = _Batch(same, need, self.clock() - t0)
+                    nslot = max(1, self.budget // per_chain[key])
+                    need = per_chain[key] * nslot
+                # the lane takes every waiting request of its shape: the first nslot start at once,
+                # the others join as slots free up
+                ln = _Lane(self, key, same, nslot, need)
+                lanes[key] = ln
                 used += need
-                th = threading.Thread(target=self._work, args=(b, t0, done), daemon=True)
-                workers[id(b)] = th
-                th.start()
+                self.joined += max(0, len(same) - nslot)
                 ids = {r.rid for r in same}
                 queue = [r for r in queue if r.rid not in ids]
-            self.peak_batches = max(self.peak_batches, len(workers))
-            # wait for a batch to finish or for the next arrival
+                ln.thread.start()
+            self.peak_batches = max(self.peak_batches, len(lanes))
             timeout = None
             if pending:
-                timeout = max(0.0, pending[0].arrival - (self.clock() - t0))
-            elif not workers:
+                timeout = max(0.0, pending[0].arrival - (self.clock() - self.t0))
+            elif not lanes:
                 continue
             try:
-                item = done.get(timeout=timeout)
+                item = self.done.get(timeout=timeout)
             except _queue.Empty:
                 continue
             while True:
-                b, out, exc = item
-                workers.pop(id(b)).join()
-                used -= b.nbytes
-                if exc is not None:
-                    for th in workers.values():
-                        th.join()
-                    raise exc
-                results.extend(out)
+                kind, ln, payload = item
+                if kind == "results":
+                    results.extend(payload)
+                elif kind == "closed":
+                    ln.thread.join()
+                    if lanes.get(ln.key) is ln:
+                        del lanes[ln.key]
+                    used -= ln.nbytes
+                    with ln.cv:                 # requests offered after the lane's last check
+                        queue.extend(ln.waiting)
+                        ln.waiting = []
+                else:
+                    for other in lanes.values():
+                        with other.cv:
+                            other.waiting = []
+                    for other in lanes.values():
+                        other.thread.join()
+                    raise payload
                 try:
-                    item = done.get_nowait()
+                    item = self.done.get_nowait()
                 except _queue.Empty:
                     break
-        self.wall = self.clock() - t0
+        self.wall = self.clock() - self.t0
         return sorted(results, key=lambda x: x.rid)
 
 

@@ -125,13 +125,13 @@ def serving_rate(nreq, nsweeps, rate, seed=0):
     served by the continuous-batching loop (alf_amd.serving)."""
     from alf_amd.serving import Server, latency_summary, poisson_requests
     reqs = poisson_requests(nreq, rate, seed=seed, nsweeps=nsweeps)
-    srv = Server()
+    srv = Server(min_slots=4)
     res = srv.run(reqs)
     out = latency_summary(res, srv.wall)
     out.update({"sample": f"{nreq} requests x {nsweeps} sweeps (the serving configuration asks 500 sweeps each), "
                           f"Poisson arrivals at {rate}/s, square L in {{8,12,16,24,32}} weighted 4:3:2:1:1 "
                           "(Ndim 64..1024), beta in {4,8,16} (Ltrot 40..160), U in {2,4,6}, Nwrap 10, FP64",
-                "peak_concurrent_batches": srv.peak_batches,
+                "peak_concurrent_batches": srv.peak_batches, "joined_running_lanes": srv.joined,
                 "largest_batch": max(r.batch_size for r in res),
                 "precision_green_max": max(r.control["XMAXG"] for r in res)})
     return out
@@ -146,7 +146,7 @@ def c4_mix_rate(chains, nsweeps, seed=0):
     rng = np.random.default_rng(seed)
     reqs = [Request(rid=i, L=int(rng.choice(np.arange(6, 21, 2))), beta=float(rng.uniform(2.0, 12.0)),
                     U=float(rng.uniform(2.0, 8.0)), seed=i, nsweeps=nsweeps, arrival=0.0) for i in range(chains)]
-    srv = Server()
+    srv = Server(min_slots=1)
     res = srv.run(reqs)
     return {"chains": chains, "sweeps_per_chain": nsweeps, "distinct_shapes": len({r.shape() for r in reqs}),
             "aggregate_sweeps_per_s": chains * nsweeps / srv.wall, "wall_s": srv.wall,
@@ -229,6 +229,29 @@ def config_rates(delay, cpu=True):
         row["ltrot"] = ham.Ltrot
         out[name] = row
     return out
+
+
+def paired_error(ham, fields, delay, sweeps=1):
+    """The split-product chain and the FP64 drop-in on the same random tape from the same fields:
+    both rebuild the same G from the same fields at every stabilisation point, so there
+    |G_split - G_FP64| <= (Precision Green of the split run) + (that of the FP64 run)."""
+    from alf_amd.runtime import PREC_FP64, PREC_INT8_SPLIT, Engine
+    u = np.random.default_rng(77).random(3 * 2 * ham.Ndim * ham.Ltrot * sweeps + 64)
+    logs, fl = [], []
+    for prec in (PREC_FP64, PREC_INT8_SPLIT):
+        e = Engine(ham, delay=delay)
+        e.set_precision(prec)
+        e.set_fields(fields)
+        e.set_tape(u)
+        e.init()
+        e.sweep(sweeps)
+        logs.append(e.precision_log())
+        fl.append(e.get_fields())
+        e.close()
+    a, b = logs
+    return {"sweeps": sweeps, "stabilisations": int(len(b)), "same_fields": bool(np.array_equal(fl[0], fl[1])),
+            "precision_green_max_fp64": float(a[:, 0].max()), "precision_green_max_split": float(b[:, 0].max()),
+            "g_err_bound_max": float((a[:, 0] + b[:, 0]).max()) if len(a) == len(b) else float("inf")}
 
 
 def single_rate(ham, fields, delay, steps, warmup, prec):
@@ -344,17 +367,27 @@ def main():
         agg = result["batched"]["aggregate_sweeps_per_s"]
         result["batched"]["sweep_roofline_fraction"] = attainable_sweep_seconds(n, L, S, acc) * agg
         result["batched"]["sweep_roofline_fraction_delayed"] = attainable_sweep_seconds(n, L, S, acc, delay=32) * agg
-    # the chosen precision form: hopping wraps of G on FP32 MFMA, everything else FP64
-    from alf_amd.runtime import PREC_FP32_WRAPS
-    mixed = None if args.no_mixed else {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) with FP32 operands, "
-                     "FP32 MFMA and FP32 accumulation; UDV chain, QR, CGR, Metropolis ratio and delayed "
-                     "updates FP64",
-             "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_FP32_WRAPS)}
-    if mixed is not None:
+    # the chosen precision form: hopping wraps of G as exact int8 digit products on the matrix
+    # cores (8 digits of 7 bits per operand), everything else FP64; FP32 wraps beside it
+    from alf_amd.runtime import PREC_FP32_WRAPS, PREC_INT8_SPLIT
+    mixed = None
+    if not args.no_mixed:
+        mixed = {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) as int8 split products: each "
+                         "operand row/column = 2^e x 8 truncated base-128 int8 digits, digit pairs with s+t <= 9 "
+                         "multiplied exactly on v_mfma_i32_32x32x32_i8 (int32 sums), combined in FP64; UDV chain, "
+                         "QR, CGR, Metropolis ratio and delayed updates FP64",
+                 "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_INT8_SPLIT),
+                 "vs_fp64_same_fields": paired_error(ham, fields, args.delay)}
         if args.batch > 0:
-            mixed["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_FP32_WRAPS)
+            mixed["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_INT8_SPLIT)
         mixed["meets_green_1e-8"] = bool(mixed["single"]["precision_green_max"] <= 1e-8 and
+                                         mixed["vs_fp64_same_fields"]["g_err_bound_max"] <= 1e-8 and
                                          mixed.get("batched", {}).get("precision_green_max", 0.0) <= 1e-8)
+        fp32 = {"form": "hopping wraps of G with FP32 operands, FP32 MFMA and FP32 accumulation",
+                "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_FP32_WRAPS)}
+        if args.batch > 0:
+            fp32["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), PREC_FP32_WRAPS)
+        mixed["fp32_wraps"] = fp32
         result["mixed"] = mixed
     if not args.no_checkerboard:
         # the Hubbard plugin's default hopping: the checkerboard product of bond exponentials

@@ -88,6 +88,15 @@ int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx
  * HBM footprint (no counterpart in the reference). */
 int alfq_chain_bytes(const alfq_model_desc* desc, int64_t* bytes);
 int alfq_set_chain(alfq_ctx* ctx, int32_t chain);
+/* Continuous batching (no counterpart in the reference): chain src_chain of src replaces
+ * chain dst_chain of dst.  Both contexts need the same shape (Ndim, Ltrot, Nwrap, N_SUN,
+ * operator sites and types, type of T, Symm) and Philox streams; the chain's couplings and
+ * exp(-dtau T), fields, G, PHASE, UDV stack, random stream and statistics move with it, so a
+ * chain initialised in a small context (alfq_init) joins a running batch at a sweep
+ * boundary and continues exactly as it would have alone.  Enqueued on dst's stream after
+ * its earlier work (src is synchronised first).  Contexts with measurements or a
+ * checkerboard Op_T are refused. */
+int alfq_chain_load(alfq_ctx* dst, int32_t dst_chain, alfq_ctx* src, int32_t src_chain);
 int alfq_batch_size(alfq_ctx* ctx, int32_t* nchain);
 /* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */
 int alfq_set_delay(alfq_ctx* ctx, int32_t k);
@@ -100,6 +109,13 @@ int alfq_set_delay(alfq_ctx* ctx, int32_t k);
  * drift the FP32 wraps accumulated since the last one. */
 #define ALFQ_PREC_FP64 0
 #define ALFQ_PREC_FP32_WRAPS 1
+/* ALFQ_PREC_INT8_SPLIT (no counterpart in the reference): the same products as exact
+ * integer sums on the int8 matrix cores.  Each operand row (left) or column (right) is
+ * 2^e times 8 truncated base-128 int8 digits (56 bits of the row's range); the digit
+ * pairs with s + t <= 9 are multiplied exactly (int32 sums) and combined in FP64.  The
+ * product's error is about 2^-56 of |row| |column|, against 2^-24 for FP32 wraps.  The
+ * first call allocates the digit planes and splits exp(-dtau T) and exp(dtau T) once. */
+#define ALFQ_PREC_INT8_SPLIT 2
 int alfq_set_precision(alfq_ctx* ctx, int32_t mode);
 
 /* ---- state movement ------------------------------------------------------ */
@@ -173,7 +189,8 @@ const char* alfq_last_error(void);
 /* The engine's MFMA product on host arrays (copied in and out), for checking the
  * kernels in isolation: C = alpha op(A) op(B) + beta C with op in 'N', 'T', 'C'; A
  * and/or B may be real (a_is_real/b_is_real), C is complex; prec is an ALFQ_PREC_*
- * arithmetic (ALFQ_PREC_FP32_WRAPS: FP32 operands, products and sums). */
+ * arithmetic (ALFQ_PREC_FP32_WRAPS: FP32 operands, products and sums; ALFQ_PREC_INT8_SPLIT:
+ * op 'N' only, at least one complex operand). */
 int alfq_gemm(int32_t M, int32_t N, int32_t K, char opa, int32_t a_is_real, const double* A, int32_t lda,
               char opb, int32_t b_is_real, const double* B, int32_t ldb, const double* alpha,
               const double* beta, double* C, int32_t ldc, int32_t prec);

@@ -347,6 +347,57 @@ def test_batched_chains_match_single_chains(rt):
     b.close()
 
 
+def test_chain_loaded_into_a_running_batch_continues_as_alone(rt):
+    """alfq_chain_load: a chain initialised in its own context joins a batch that has already
+    swept (and replays sweep graphs), at a sweep boundary; it then follows the trajectory it
+    follows alone, and the batch's other chains are undisturbed.  A context of another shape
+    is refused."""
+    Us = (4.0, 2.5, 6.0)
+    hams = [hubbard(L1=6, L2=6, beta=2.0, nwrap=10, U=U) for U in Us]
+    b = rt.BatchEngine(hams, delay=16)
+    for z, h in enumerate(hams):
+        b.chain(z).set_fields(fields_for(h, 40 + z))
+        b.chain(z).set_philox(200 + z, 0)
+    b.init()
+    b.sweep(3)
+    new = hubbard(L1=6, L2=6, beta=2.0, nwrap=10, U=3.3)
+    src = rt.Engine(new, delay=16)
+    src.set_fields(fields_for(new, 77))
+    src.set_philox(777, 0)
+    src.init()
+    b.load_chain(1, src)
+    src.close()
+    b.sweep(4)
+    alone = rt.Engine(new, delay=16)
+    alone.set_fields(fields_for(new, 77))
+    alone.set_philox(777, 0)
+    alone.init()
+    alone.sweep(4)
+    b.chain(1)
+    assert np.array_equal(b.get_fields(), alone.get_fields())
+    assert rel(b.get_green(), alone.get_green()) < TOL
+    assert abs(b.get_phase() - alone.get_phase()) < TOL
+    cb, ca = b.control(), alone.control()
+    assert cb["ACC_up"] == ca["ACC_up"] and cb["NC_up"] == ca["NC_up"] and cb["NCG"] == ca["NCG"]
+    alone.close()
+    for z in (0, 2):                     # the other chains: 7 sweeps as if nothing happened
+        e = rt.Engine(hams[z], delay=16)
+        e.set_fields(fields_for(hams[z], 40 + z))
+        e.set_philox(200 + z, 0)
+        e.init()
+        e.sweep(7)
+        b.chain(z)
+        assert np.array_equal(b.get_fields(), e.get_fields())
+        assert rel(b.get_green(), e.get_green()) < TOL
+        e.close()
+    other = rt.Engine(hubbard(L1=4, L2=4, beta=2.0, nwrap=10), delay=16)
+    other.set_philox(1, 0)
+    with pytest.raises(RuntimeError, match="shape"):
+        b.load_chain(0, other)
+    other.close()
+    b.close()
+
+
 def test_batched_chains_at_full_tile_grids(rt):
     """32 chains of Ndim = 256: every full-size product of the batch runs on the 64 x 64
     tiling (>= 512 tiles per launch).  Chains 0, 13 and 31 follow the trajectories of the
@@ -508,23 +559,26 @@ def test_ndim_1024_sweep_is_self_consistent(rt):
 
 
 def test_serving_results_same_alone_and_in_mix(rt):
-    """Requests served in a mix (shared batches, batches overlapping on the device, a late
-    arrival admitted while others run) give the same result as each request served alone."""
+    """Requests served in a mix (lanes of two slots; request 3 joins the running lane of its
+    shape in the slot request 1 frees after its two sweeps; lanes of other shapes overlapping
+    on the device; a late arrival) give the same result as each request served alone."""
     from alf_amd.serving import Request, Server
-    reqs = [Request(0, 6, 2.0, 4.0, seed=3, nsweeps=3, arrival=0.0),
-            Request(1, 6, 2.0, 2.5, seed=4, nsweeps=3, arrival=0.0),
+    reqs = [Request(0, 6, 2.0, 4.0, seed=3, nsweeps=6, arrival=0.0),
+            Request(1, 6, 2.0, 2.5, seed=4, nsweeps=2, arrival=0.0),
             Request(2, 4, 1.0, 4.0, seed=5, nsweeps=3, arrival=0.0),
-            Request(3, 6, 2.0, 6.0, seed=6, nsweeps=3, arrival=0.05),
+            Request(3, 6, 2.0, 6.0, seed=6, nsweeps=3, arrival=0.0),
             Request(4, 8, 1.5, 3.0, seed=7, nsweeps=2, arrival=0.1)]
-    srv = Server(memory_budget=8 << 30)
+    srv = Server(memory_budget=8 << 30, max_batch=2)
     mix = srv.run(reqs)
     assert [r.rid for r in mix] == [0, 1, 2, 3, 4]
-    assert mix[0].batch_size == 2 and srv.peak_batches >= 2
+    assert mix[0].batch_size == 2 and srv.peak_batches >= 2 and srv.joined >= 1
+    assert mix[3].admitted >= mix[1].done - 1e-3          # took request 1's slot
     for r in (reqs[1], reqs[3], reqs[4]):
-        solo = Server(memory_budget=8 << 30).run([Request(**{**r.__dict__, "arrival": 0.0})])[0]
+        solo = Server(memory_budget=8 << 30, min_slots=1).run([Request(**{**r.__dict__, "arrival": 0.0})])[0]
         m = mix[r.rid]
         assert solo.batch_size == 1
         assert np.array_equal(solo.fields, m.fields)
         assert rel(solo.green, m.green) < TOL
         assert abs(solo.phase - m.phase) < TOL
         assert m.control["XMAXG"] < 1e-8
+        assert solo.control["ACC_up"] == m.control["ACC_up"] and solo.control["NC_up"] == m.control["NC_up"]

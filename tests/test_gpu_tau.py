@@ -36,6 +36,9 @@ def tau_oracle(ham):
     ("Honeycomb", 3, 1.0, 4, 1, 0.0, False),
     ("Square", 4, 1.5, 5, 0, 0.2, True),
     ("Honeycomb", 3, 1.0, 5, 1, 0.1, True),
+    # Ndim 144: CGR2_2 factorises a 288-row matrix, its first panel on the LDS panel kernel
+    # and the later ones (at most 256 rows) on the register panel
+    ("Square", 12, 0.5, 5, 0, 0.0, False),
 ])
 def test_tau_measurements_match_oracle(rt, lattice, L, beta, nwrap, n_phi, chem, cb):
     ham = hubbard(L1=L, L2=L, beta=beta, nwrap=nwrap, lattice=lattice, n_phi=n_phi, chem=chem, checkerboard=cb)
