@@ -166,6 +166,7 @@ struct alfq_ctx {
   int n = 0, L = 0, W = 0, nstm = 0, n_op = 0, n_sun = 0;
   bool t_real = true;
   int K = 32;
+  int gemm_policy = 0;                   // GEMM tiling policy (alfq_set_gemm_policy)
   int prec = ALFQ_PREC_FP64;             // arithmetic of the hopping wraps of G
   // ALFQ_PREC_INT8_SPLIT: digit planes of exp(-dtau T) and exp(dtau T) as left operands (by
   // row) and right operands (by column), and of the wrapped matrix, oz_cs bytes per chain
@@ -288,12 +289,13 @@ struct alfq_ctx {
   // sweeps replayed as HIP graphs, keyed by the host state they start from
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
-    int var_r, var_l, rounds, K, rng_mode, prec;
+    int var_r, var_l, rounds, K, rng_mode, prec, gemm_policy;
     bool obs_on, cb_on, tau_on;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
-             K == o.K && rng_mode == o.rng_mode && prec == o.prec && obs_on == o.obs_on && cb_on == o.cb_on &&
+             K == o.K && rng_mode == o.rng_mode && prec == o.prec && gemm_policy == o.gemm_policy &&
+             obs_on == o.obs_on && cb_on == o.cb_on &&
              tau_on == o.tau_on &&
              var_st == o.var_st && refs == o.refs;
     }
@@ -316,6 +318,7 @@ namespace {
 int gemm_c(alfq_ctx* c, GemmArgs g) {
   g.nchain = c->B;
   g.cstride = c->cs;
+  g.policy = c->gemm_policy;
   return gemm_launch(g, c->s);
 }
 
@@ -416,6 +419,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   gx.amap = c->amapk;
   gx.kdev = &c->st->m;
   gx.nchain = c->B;
+  gx.policy = c->gemm_policy;
   a.yhat_external = reg_kernel && gemm_direct_path(gx) ? 1 : 0;
   if (a.yhat_external) gx.yhat = c->Y;
   for (int r = 0; r < rounds; ++r) {
@@ -960,6 +964,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   h.G = c->G; h.Gtmp = c->Gtmp; h.u = c->u;
   h.var_r = c->var_r; h.var_l = c->var_l; h.rounds = c->rounds; h.K = c->K; h.rng_mode = c->rng_mode;
   h.prec = c->prec;
+  h.gemm_policy = c->gemm_policy;
   h.obs_on = c->obs_on;
   h.cb_on = c->cb_on;
   h.tau_on = c->tau_on;
@@ -1470,6 +1475,15 @@ int alfq_set_delay(alfq_ctx* c, int32_t k) {
   return ST_NONE;
 }
 
+int alfq_set_gemm_policy(alfq_ctx* c, int32_t policy) {
+  if (!c) return ST_GENERIC;
+  if (policy < 0 || policy >= GEMM_POLICIES) { alfq_set_error("gemm policy outside 0..3"); return ST_GENERIC; }
+  c->gemm_policy = policy;
+  c->qr.policy = policy;
+  c->tau.w2.policy = policy;
+  return ST_NONE;
+}
+
 int alfq_set_precision(alfq_ctx* c, int32_t mode) {
   if (!c) return ST_GENERIC;
   if (mode != ALFQ_PREC_FP64 && mode != ALFQ_PREC_FP32_WRAPS && mode != ALFQ_PREC_INT8_SPLIT) {
@@ -1723,6 +1737,7 @@ int tau_alloc(alfq_ctx* c) {
   t.regions = regions;
   t.w2.nchain = c->B;
   t.w2.cs = c->cs;
+  t.w2.policy = c->gemm_policy;
   // sums: head, Den0[norb], [3][Ltrot + 1][ncell][norb][norb]
   const int nc = c->obs.ncell, no = c->obs.norb;
   auto r256 = [](long long b) { return (b + 255) & ~255LL; };

@@ -132,6 +132,8 @@ struct GemmArgs {
   int nchain;               // chains of the batch (grid z = batch * nchain)
   long long cstride;        // byte stride between the chains' arenas (applies to every pointer)
   int prec;                 // 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA and accumulation
+  int policy;               // tiling policy (GEMM_POLICIES): the grid sizes at which the launcher
+                            // switches to the direct-load and to the 64 x 64 kernels
   // direct-load kernel only (gemm_direct_path): with amap set and a complex opA = A (OP_N),
   // also write yhat(r, j) = delta(r, amap[j]) - A(amap[j], r) for r < M, j < K (ld M), the
   // rows amap of A read before any later kernel changes A
@@ -142,6 +144,10 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
                    const void* B, int ldb, int opB, bool b_cplx, void* C, int ldc,
                    cplx alpha, cplx beta);
 int gemm_launch(const GemmArgs& g, hipStream_t s);
+// Tiling policies a context may pick (its autotuner measures them): 0 the estimate-based
+// default, 1 the 64 x 64 kernel from one tile per CU, 2 never the 64 x 64 kernel, 3 the
+// direct-load kernel up to 512 tiles of 32 x 32.
+constexpr int GEMM_POLICIES = 4;
 // true when gemm_launch runs g on the direct-load kernel (the only one that honours yhat)
 bool gemm_direct_path(const GemmArgs& g);
 // G += X Y^T (all n x n / n x m, column-major, ld n) with the rank m read from *mdev and

@@ -6,6 +6,7 @@ struct QRWork {
   int n = 0, nb = 0, nblk = 0;
   int nchain = 1;         // chains of the batch; every buffer below is per chain
   long long cs = 0;       // byte stride between the chains' arenas
+  int policy = 0;         // GEMM tiling policy of the context
   bool lds_panel = false; // panel factorisation in LDS (n <= 2048)
   cplx* A = nullptr;      // n x n factorisation workspace
   cplx* Vw = nullptr;     // explicit Householder panels (n x n)

@@ -737,6 +737,7 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
   g.nchain = 1;
   g.cstride = 0;
   g.prec = 0;
+  g.policy = 0;
   return g;
 }
 
@@ -757,8 +758,17 @@ static int gemm_tile_shape(int M, int N, int K, int batch) {
   return opts[best].code;
 }
 
+// grid sizes of the tiling policies: direct-load kernel below direct_below tiles of 32 x 32,
+// 64 x 64 kernel from big_from tiles of 64 x 64 (with 3/4 of their area inside the product)
+struct TilePolicy { long long direct_below, big_from; };
+constexpr TilePolicy kTilePolicy[GEMM_POLICIES] = {{256, 512}, {256, 256}, {256, 1LL << 62}, {512, 512}};
+
+static const TilePolicy& tile_policy(const GemmArgs& g) {
+  return kTilePolicy[(g.policy >= 0 && g.policy < GEMM_POLICIES) ? g.policy : 0];
+}
+
 bool gemm_direct_path(const GemmArgs& g) {
-  return g.M > 0 && g.N > 0 && cdivl(g.M, 32) * cdivl(g.N, 32) * (long long)gemm_zdim(g) < 256;
+  return g.M > 0 && g.N > 0 && cdivl(g.M, 32) * cdivl(g.N, 32) * (long long)gemm_zdim(g) < tile_policy(g).direct_below;
 }
 
 int gemm_launch(const GemmArgs& g, hipStream_t s) {
@@ -782,7 +792,7 @@ int gemm_launch(const GemmArgs& g, hipStream_t s) {
   {
     // two or more 64 x 64 tiles per CU, at least 3/4 of each tile's area inside the product
     const long long tm = cdivl(g.M, 64), tn = cdivl(g.N, 64);
-    if (tm * tn * b >= 512 && 4LL * g.M * g.N >= 3LL * (64 * tm) * (64 * tn)) {
+    if (tm * tn * b >= tile_policy(g).big_from && 4LL * g.M * g.N >= 3LL * (64 * tm) * (64 * tn)) {
       launch_big(g, s);
       HIP_OK(hipGetLastError());
       return ST_NONE;

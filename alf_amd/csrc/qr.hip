@@ -888,6 +888,7 @@ void qr_work_place(QRWork& w, int n, ArenaLayout& ar) {
 static int gemm_w(const QRWork& w, GemmArgs g, hipStream_t s) {
   g.nchain = w.nchain;
   g.cstride = w.cs;
+  g.policy = w.policy;
   return gemm_launch(g, s);
 }
 

@@ -21,7 +21,7 @@ is committed beside the other profiles and read here.
 The ``sweep`` mode runs sweeps of the 16 x 16 U = 4 beta = 10 Hubbard workload, one chain
 and then a batch of chains, for per-kernel profiles of either hopping form:
 
-    python3 -m alf_amd.kernelbench sweep [--checkerboard] [--tau] [--batch B] [--steps K]
+    python3 -m alf_amd.kernelbench sweep [--checkerboard] [--tau] [--batch B] [--steps K] [--prec P]
 
 The ``counters`` mode is the program for hardware-counter passes (rocprofv3 --pmc): one
 chain and then a batch of chains of the same workload, each initialised and swept once with
@@ -129,15 +129,17 @@ def copy_calibration(nbytes: int, repeats: int = 3) -> None:
     print(f"copied {nbytes} bytes device to device x {repeats}", flush=True)
 
 
-def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False) -> None:
+def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False, prec: int = 0) -> None:
     """`steps` sweeps of one chain, then of `batch` chains in one batched context; with tau,
-    the equal-time and time-displaced measurements on (TAU_M after every sweep)."""
+    the equal-time and time-displaced measurements on (TAU_M after every sweep); prec is the
+    arithmetic of the hopping wraps (alfq_set_precision)."""
     from alf_amd.host.hamiltonian import HubbardParams, hubbard_ham_set
     from alf_amd.runtime import BatchEngine, Engine
     ham = hubbard_ham_set(HubbardParams(Lattice_type="Square", L1=16, L2=16, Ham_U=4.0, Beta=10.0, Dtau=0.1,
                                         Nwrap=10, Checkerboard=checkerboard))
     ham.nsigma.set_uniform_four_state(0)
     eng = Engine(ham, delay=32)
+    eng.set_precision(prec)
     eng.set_fields(ham.nsigma.f.copy())
     eng.set_philox(seed=0, stream=0)
     if tau:
@@ -150,6 +152,7 @@ def profile_sweeps(checkerboard: bool, batch: int, steps: int, tau: bool = False
     print(f"single chain: {steps} sweeps, {ms / steps:.2f} ms per sweep", flush=True)
     if batch > 0:
         b = BatchEngine([ham] * batch, delay=32)
+        b.set_precision(prec)
         for z in range(batch):
             ham.nsigma.set_uniform_four_state(z)
             b.chain(z).set_fields(ham.nsigma.f.copy())
@@ -298,7 +301,8 @@ if __name__ == "__main__":
         copy_calibration(mib << 20)
     elif len(sys.argv) >= 2 and sys.argv[1] == "sweep":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
-        profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3), "--tau" in sys.argv)
+        profile_sweeps("--checkerboard" in sys.argv, arg("--batch", 256), arg("--steps", 3), "--tau" in sys.argv,
+                       arg("--prec", 0))
     elif len(sys.argv) >= 2 and sys.argv[1] == "counters":
         arg = lambda k, d: int(sys.argv[sys.argv.index(k) + 1]) if k in sys.argv else d
         counter_run(arg("--batch", 64), "--checkerboard" in sys.argv)

@@ -37,6 +37,7 @@ EXPORTS = [
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
     "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc", "alfq_chain_load",
+    "alfq_set_gemm_policy",
 ]
 
 
@@ -146,6 +147,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_ctx_create_batch": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(P)],
         "alfq_set_chain": [P, C.c_int32],
         "alfq_chain_load": [P, C.c_int32, P, C.c_int32],
+        "alfq_set_gemm_policy": [P, C.c_int32],
         "alfq_batch_size": [P, C.POINTER(C.c_int32)],
         "alfq_kernel_timing_get": [P, dp, C.POINTER(C.c_int64)],
         "alfq_gemm": [C.c_int32, C.c_int32, C.c_int32, C.c_char, C.c_int32, dp, C.c_int32,
@@ -248,6 +250,9 @@ def obs_desc(model, lobs=None):
     return d, keep
 
 
+_TUNED = {}     # autotuner results per (Ndim, Ltrot, chains, real T) for this process
+
+
 class Engine:
     """One Markov chain on the GPU behind the C ABI.
 
@@ -293,6 +298,54 @@ class Engine:
     # ---- state ----
     def set_delay(self, k):
         self._c(self.lib.alfq_set_delay(self.ctx, int(k)))
+        self.delay = int(k)
+
+    def set_gemm_policy(self, policy):
+        self._c(self.lib.alfq_set_gemm_policy(self.ctx, int(policy)))
+        self.gemm_policy = int(policy)
+
+    def autotune(self, delays=(8, 16, 32), policies=(0, 1, 2, 3), timed=2):
+        """Measured choice of the delayed-update depth k and the GEMM tiling policy for this
+        context's shape (Ndim, Ltrot, chains), cached per shape for the process.
+
+        Every trial runs real sweeps of the chains (the trajectory does not depend on k, and
+        the tiling changes only the rounding of the products), so tuning costs no wasted
+        work: per candidate two sweeps capture the sweep graphs of both buffer parities, then
+        `timed` replayed sweeps are timed with HIP events.  k is tuned first at policy 0, then
+        the policy at the best k.  Needs a context after init with a random stream."""
+        key = (self.model.Ndim, self.model.Ltrot, getattr(self, "nchain", 1), bool(self.model.Op_T.is_real))
+        hit = _TUNED.get(key)
+        if hit is not None:
+            self.set_delay(hit["delay"])
+            self.set_gemm_policy(hit["policy"])
+            return hit
+
+        def trial(k, pol):
+            self.set_delay(k)
+            self.set_gemm_policy(pol)
+            self.sweep(2)
+            return self.sweep_timed(timed) / timed
+
+        table = {}
+        best_k, best_t = None, float("inf")
+        for k in delays:
+            t = trial(k, 0)
+            table[f"k={k},policy=0"] = t
+            if t < best_t:
+                best_k, best_t = k, t
+        best_p = 0
+        for pol in policies:
+            if pol == 0:
+                continue
+            t = trial(best_k, pol)
+            table[f"k={best_k},policy={pol}"] = t
+            if t < best_t:
+                best_p, best_t = pol, t
+        self.set_delay(best_k)
+        self.set_gemm_policy(best_p)
+        hit = {"delay": best_k, "policy": best_p, "ms_per_sweep": best_t, "trials_ms_per_sweep": table}
+        _TUNED[key] = hit
+        return hit
 
     # ---- equal-time measurements ----
     def obs_setup(self, model=None, lobs=None):

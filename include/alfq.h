@@ -100,6 +100,11 @@ int alfq_chain_load(alfq_ctx* dst, int32_t dst_chain, alfq_ctx* src, int32_t src
 int alfq_batch_size(alfq_ctx* ctx, int32_t* nchain);
 /* Delayed-update depth k (1..64) of the Sherman-Morrison chain. */
 int alfq_set_delay(alfq_ctx* ctx, int32_t k);
+/* GEMM tiling policy of the context (no counterpart in the reference; picked by the
+ * autotuner of alf_amd/runtime.py): 0 the estimate-based default, 1 the 64 x 64 kernel from
+ * one output tile per CU, 2 never the 64 x 64 kernel, 3 the direct-load kernel up to 512
+ * tiles of 32 x 32.  Results agree to rounding across policies. */
+int alfq_set_gemm_policy(alfq_ctx* ctx, int32_t policy);
 /* Arithmetic of the hopping wraps of G in WRAPGRUP/WRAPGRDO (Hop_mod_mmthr/mmthl and
  * their inverses, Prog/Hop_mod.F90:143-250).  ALFQ_PREC_FP64 (the default) is the drop-in;
  * ALFQ_PREC_FP32_WRAPS rounds both operands of those products to FP32 and multiplies and

@@ -398,6 +398,39 @@ def test_chain_loaded_into_a_running_batch_continues_as_alone(rt):
     b.close()
 
 
+def test_gemm_policies_and_autotuner(rt):
+    """The four GEMM tiling policies give the same trajectory (fields and counters exact, G
+    to rounding) on a batch of 32 chains of 16 x 16 (grids around the policies' switch
+    points), and the autotuner picks a measured (k, policy) and reuses it for the shape."""
+    ham = hubbard(L1=16, L2=16, beta=1.0, nwrap=10)
+    res = []
+    for pol in range(4):
+        b = rt.BatchEngine([ham] * 32, delay=16)
+        b.set_gemm_policy(pol)
+        for z in range(32):
+            b.chain(z).set_fields(fields_for(ham, 60 + z))
+            b.chain(z).set_philox(900 + z, 0)
+        b.init()
+        b.sweep(2)
+        res.append([(b.chain(z).get_fields(), b.get_green(), b.control()["ACC_up"]) for z in (0, 13, 31)])
+        b.close()
+    for r in res[1:]:
+        for (f0, g0, a0), (f1, g1, a1) in zip(res[0], r):
+            assert np.array_equal(f0, f1) and a0 == a1
+            assert rel(g1, g0) < TOL
+    b = rt.BatchEngine([ham] * 32)
+    for z in range(32):
+        b.chain(z).set_fields(fields_for(ham, z))
+        b.chain(z).set_philox(z, 0)
+    b.init()
+    t = b.autotune(delays=(8, 32), policies=(0, 1))
+    print(f"\nautotuner: {t}", flush=True)
+    assert t["delay"] in (8, 32) and t["policy"] in (0, 1) and len(t["trials_ms_per_sweep"]) == 3
+    assert b.autotune(delays=(8, 32), policies=(0, 1)) is t
+    assert b.control()["XMAXG"] < 1e-8
+    b.close()
+
+
 def test_batched_chains_at_full_tile_grids(rt):
     """32 chains of Ndim = 256: every full-size product of the batch runs on the 64 x 64
     tiling (>= 512 tiles per launch).  Chains 0, 13 and 31 follow the trajectories of the
