@@ -237,6 +237,12 @@ struct alfq_ctx {
   int* uoff_dn = nullptr;                // the same for the descending visit order
   std::vector<int> h_op_type;
   std::vector<int> h_op_site;
+  // batches of uneven Ltrot: chain z has h_lz[z] <= L slices (L is the batch's largest); its
+  // later slices hold no fields and apply the identity (lz on the device, ident an identity of
+  // exp(-dtau T)'s type); empty when every chain has L slices
+  std::vector<int> h_lz;
+  int* lz = nullptr;
+  void* ident = nullptr;
   // state
   int8_t* f = nullptr;
   cplx* G = nullptr;
@@ -400,6 +406,7 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   a.u = c->u; a.st = c->st;
   a.W = c->Wk; a.amap = c->amapk;
   a.cs = c->cs; a.nchain = c->B;
+  a.lz = c->lz;
   a.uoff = dir > 0 ? c->uoff_up : c->uoff_dn;
   const int worst = c->n_op / c->K + 1;
   const bool reg_kernel = chain_max_delay(c->n) >= c->K;
@@ -571,6 +578,16 @@ int oz_right(alfq_ctx* c, const cplx* X, const cplx* kscale, const OzOperand& E,
   return st ? st : oz_gemm(n, n, n, c->oz_x, E, out, n, ONE, ZERO, rs, cs, c->cs, c->B, c->s);
 }
 
+// In a batch of uneven Ltrot, operand `side` (1: A, 2: B) of a hopping product of slice
+// `slice` (0-based) becomes the identity for the chains that have no such slice.
+void hop_mask(const alfq_ctx* c, GemmArgs& g, int slice, int side) {
+  if (!c->lz) return;
+  g.lz = c->lz;
+  g.slice = slice;
+  g.hop_side = side;
+  g.ident = c->ident;
+}
+
 int wrapgrup_dev(alfq_ctx* c, int ntau) {
   int st;
   if ((st = hop_ready(c))) return st;
@@ -594,6 +611,7 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
     GemmArgs g1 = gemm_args(c->n, c->n, c->n, c->expT, c->n, OP_N, !c->t_real, c->G, c->n, OP_N, true, c->H, c->n,
                             ONE, ZERO);
     g1.prec = c->prec;
+    hop_mask(c, g1, slice, 1);
     if ((st = gemm_c(c, g1))) return st;
     // G = D (H exp(dtau T)) D^-1  (Op_Wrapup type 1 of every operator, old fields)
     GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->H, c->n, OP_N, true, c->expTinv, c->n, OP_N, !c->t_real, c->Gtmp,
@@ -601,6 +619,7 @@ int wrapgrup_dev(alfq_ctx* c, int ntau) {
     g2.rs = c->dv_d;
     g2.cs = c->dv_dinv;
     g2.prec = c->prec;
+    hop_mask(c, g2, slice, 2);
     if ((st = gemm_c(c, g2))) return st;
   }
   std::swap(c->G, c->Gtmp);
@@ -634,11 +653,13 @@ int wrapgrdo_dev(alfq_ctx* c, int ntau) {
     g1.rs = c->dv_dinv;
     g1.bs = c->dv_d;
     g1.prec = c->prec;
+    hop_mask(c, g1, slice, 2);
     if ((st = gemm_c(c, g1))) return st;
     // G = exp(dtau T) H   (Hop_mod_mmthr_m1)
     GemmArgs g2 = gemm_args(c->n, c->n, c->n, c->expTinv, c->n, OP_N, !c->t_real, c->H, c->n, OP_N, true, c->Gtmp,
                             c->n, ONE, ZERO);
     g2.prec = c->prec;
+    hop_mask(c, g2, slice, 1);
     if ((st = gemm_c(c, g2))) return st;
   }
   std::swap(c->G, c->Gtmp);
@@ -675,6 +696,7 @@ int wrap_udv_dev(alfq_ctx* c, int* var, int nt_lo, int nt_hi, bool left) {
       GemmArgs g = gemm_args(n, n, n, c->expT, n, OP_N, !c->t_real, cur, n, OP_N, true, bufs[which], n, ONE, ZERO);
       if (left) g.bs = c->dv_dconj + (long long)sl * n;
       else g.rs = c->dv_d + (long long)sl * n;
+      hop_mask(c, g, nt_lo + sl, 1);
       if ((st = gemm_c(c, g))) return st;
     }
     cur = bufs[which];
@@ -1035,6 +1057,9 @@ int sweep_graph(alfq_ctx* c) {
   return ST_NONE;
 }
 
+// Ltrot of chain z (the batch's Ltrot unless the batch is uneven)
+int chain_ltrot(const alfq_ctx* c, int z) { return c->h_lz.empty() ? c->L : c->h_lz[z]; }
+
 // chain z's copy of a per-chain buffer
 template <class T>
 T* at_chain(const alfq_ctx* c, T* p, int z) {
@@ -1105,9 +1130,13 @@ int check_desc(const alfq_model_desc* d, const alfq_model_desc* d0) {
       alfq_set_error("batch: the chains differ in Symm");
       return ST_HAMILTONIAN;
     }
-    if (d->ndim != d0->ndim || d->ltrot != d0->ltrot || d->nwrap != d0->nwrap || d->n_sun != d0->n_sun ||
-        d->n_op != d0->n_op || d->t_is_real != d0->t_is_real) {
-      alfq_set_error("batch: the chains differ in Ndim, Ltrot, Nwrap, N_SUN, the operator count or the type of T");
+    if (d->ndim != d0->ndim || d->nwrap != d0->nwrap || d->n_sun != d0->n_sun || d->n_op != d0->n_op ||
+        d->t_is_real != d0->t_is_real) {
+      alfq_set_error("batch: the chains differ in Ndim, Nwrap, N_SUN, the operator count or the type of T");
+      return ST_HAMILTONIAN;
+    }
+    if (d->ltrot != d0->ltrot && d->expT_half != nullptr) {
+      alfq_set_error("batch: chains of uneven Ltrot cannot use Symm");
       return ST_HAMILTONIAN;
     }
     for (int i = 0; i < d->n_op; ++i)
@@ -1248,6 +1277,13 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
   alfq_ctx* c = new alfq_ctx();
   c->B = nchain;
   c->n = d->ndim; c->L = d->ltrot; c->W = d->nwrap; c->n_op = d->n_op; c->n_sun = d->n_sun;
+  for (int z = 1; z < nchain; ++z) c->L = std::max(c->L, (int)descs[z].ltrot);
+  for (int z = 0; z < nchain; ++z)
+    if (descs[z].ltrot != c->L) {
+      c->h_lz.resize(nchain);
+      for (int y = 0; y < nchain; ++y) c->h_lz[y] = descs[y].ltrot;
+      break;
+    }
   c->t_real = d->t_is_real != 0;
   c->symm = d->expT_half != nullptr;
   c->nstm = (c->L % c->W == 0) ? c->L / c->W : c->L / c->W + 1;
@@ -1297,6 +1333,15 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
                  hipMemcpy(c->op_type, typ.data(), no * sizeof(int), hipMemcpyHostToDevice) ||
                  hipMemcpy(c->site_ops, sops.data(), no * sizeof(int), hipMemcpyHostToDevice)))
     return fail("copy tables");
+  if (!c->h_lz.empty()) {
+    const size_t esz = c->t_real ? sizeof(double) : sizeof(cplx);
+    std::vector<char> id((size_t)n * n * esz, 0);
+    for (int i = 0; i < n; ++i) *(double*)(id.data() + ((size_t)i * n + i) * esz) = 1.0;
+    if (hipMalloc(&c->lz, nchain * sizeof(int)) || hipMalloc(&c->ident, id.size()) ||
+        hipMemcpy(c->lz, c->h_lz.data(), nchain * sizeof(int), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->ident, id.data(), id.size(), hipMemcpyHostToDevice))
+      return fail("uneven Ltrot tables");
+  }
   // the chains' arenas: one allocation, chain z at z * cs bytes
   const int nslot = c->nstm + 6;
   ArenaLayout sizing;
@@ -1322,7 +1367,9 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
     DevScalars ds{};
     ds.z_cgr = make_double2(1.0, 0.0);
     ds.opph = make_double2(1.0, 0.0);
-    std::vector<int8_t> f1((size_t)std::max(no, 1) * c->L, 1);
+    // fields +1 on the chain's own slices, none (0: identity exponentials) beyond them
+    std::vector<int8_t> f1((size_t)std::max(no, 1) * c->L, 0);
+    std::fill(f1.begin(), f1.begin() + (size_t)std::max(no, 1) * dz->ltrot, (int8_t)1);
     if (c->symm && (put_chain(c, z, (char*)c->expTh, (const char*)dz->expT_half, nn * tsz) ||
                     put_chain(c, z, (char*)c->expThinv, (const char*)dz->expT_half_inv, nn * tsz)))
       return fail("copy half-step exponentials");
@@ -1384,6 +1431,10 @@ int alfq_chain_load(alfq_ctx* dst, int32_t dst_chain, alfq_ctx* src, int32_t src
       dst->pool.size() != src->pool.size() || dst->h_op_type != src->h_op_type || dst->h_op_site != src->h_op_site) {
     alfq_set_error("chain_load: the contexts differ in shape or operator structure");
     return ST_HAMILTONIAN;
+  }
+  if (dst->lz || src->lz) {
+    alfq_set_error("chain_load: batches of uneven Ltrot cannot exchange chains");
+    return ST_GENERIC;
   }
   if (dst->obs.buf || src->obs.buf || dst->tau.mem || src->tau.mem || dst->cb_buf || src->cb_buf) {
     alfq_set_error("chain_load: contexts with measurements or a checkerboard Op_T cannot exchange chains");
@@ -1455,6 +1506,8 @@ int alfq_ctx_destroy(alfq_ctx* c) {
   if (c->tau.acc) hipFree(c->tau.acc);
   if (c->cb_sites) hipFree(c->cb_sites);
   if (c->oz_buf) hipFree(c->oz_buf);
+  if (c->lz) hipFree(c->lz);
+  if (c->ident) hipFree(c->ident);
   void* ptrs[] = {c->uoff_up, c->uoff_dn, c->op_site, c->op_type, c->site_ptr, c->site_ops, c->arena, c->u_tape,
                   c->stat_dev};
   for (void* p : ptrs)
@@ -1490,6 +1543,10 @@ int alfq_set_precision(alfq_ctx* c, int32_t mode) {
     alfq_set_error("precision: unknown mode");
     return ST_GENERIC;
   }
+  if (mode == ALFQ_PREC_INT8_SPLIT && c->lz) {
+    alfq_set_error("precision: the int8 split products need chains of one Ltrot");
+    return ST_GENERIC;
+  }
   if (mode == ALFQ_PREC_INT8_SPLIT && !c->oz_buf) {
     NULL_STREAM_LOCK;
     int st = oz_setup(c);
@@ -1505,6 +1562,7 @@ int alfq_obs_setup(alfq_ctx* c, const alfq_obs_desc* d) {
     alfq_set_error("obs_setup: null argument");
     return ST_GENERIC;
   }
+  if (c->lz) { alfq_set_error("obs_setup: measurements need chains of one Ltrot"); return ST_GENERIC; }
   const int nc = d->ncell, no = d->norb, n = c->n, nnz = d->nnz;
   if (nc < 1 || no < 1 || (long long)nc * no != n) {
     alfq_set_error("obs_setup: ncell x norb must equal Ndim");
@@ -1588,6 +1646,7 @@ int alfq_set_checkerboard(alfq_ctx* c, const alfq_checkerboard_desc* d) {
     alfq_set_error("set_checkerboard: null argument");
     return ST_GENERIC;
   }
+  if (c->lz) { alfq_set_error("set_checkerboard: a checkerboard Op_T needs chains of one Ltrot"); return ST_HAMILTONIAN; }
   const int n = c->n, nb = d->nbond, nf = d->nfam;
   if (nb < 1 || nf < 1 || nf > CB_MAXFAM) {
     alfq_set_error("set_checkerboard: need 1..16 families and at least one bond");
@@ -1847,7 +1906,7 @@ int alfq_obs_get(alfq_ctx* c, alfq_obs_sums* sums, double* latt, double* den0) {
 int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) {
   NULL_STREAM_LOCK;
   if (!c || !f) return ST_GENERIC;
-  if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
+  if (n_op != c->n_op || ltrot != chain_ltrot(c, c->cur)) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
   std::vector<int8_t> h((size_t)n_op * ltrot);
   for (size_t i = 0; i < h.size(); ++i) {
     const int v = f[i];
@@ -1866,7 +1925,7 @@ int alfq_fields_set(alfq_ctx* c, const int32_t* f, int32_t n_op, int32_t ltrot) 
 int alfq_fields_get(alfq_ctx* c, int32_t* f, int32_t n_op, int32_t ltrot) {
   NULL_STREAM_LOCK;
   if (!c || !f) return ST_GENERIC;
-  if (n_op != c->n_op || ltrot != c->L) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
+  if (n_op != c->n_op || ltrot != chain_ltrot(c, c->cur)) { alfq_set_error("fields: shape mismatch"); return ST_FIELDS; }
   std::vector<int8_t> h((size_t)n_op * ltrot);
   HIP_OK(hipStreamSynchronize(c->s));
   HIP_OK(hipMemcpy(h.data(), at(c, c->f), h.size(), hipMemcpyDeviceToHost));

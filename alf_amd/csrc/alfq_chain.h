@@ -51,6 +51,9 @@ struct ChainArgs {
   // op_type and uoff are shared
   long long cs;
   int nchain;
+  // batches of uneven Ltrot: chain z has lz[z] slices; slices at or beyond it hold no fields
+  // (null: every chain has the batch's Ltrot)
+  const int* lz;
 };
 
 // Optional per-launch timing of the chain kernel: start/stop events recorded by the

@@ -134,6 +134,13 @@ struct GemmArgs {
   int prec;                 // 0: FP64 MFMA; 1: operands rounded to FP32, FP32 MFMA and accumulation
   int policy;               // tiling policy (GEMM_POLICIES): the grid sizes at which the launcher
                             // switches to the direct-load and to the 64 x 64 kernels
+  // batches of uneven Ltrot: for a chain z with lz[z] <= slice, operand hop_side (1: A, 2: B),
+  // the hopping exponential, is replaced by ident (an identity of the same type, shared): the
+  // slice is the identity for that chain
+  const int* lz;
+  int slice;
+  int hop_side;
+  const void* ident;
   // direct-load kernel only (gemm_direct_path): with amap set and a complex opA = A (OP_N),
   // also write yhat(r, j) = delta(r, amap[j]) - A(amap[j], r) for r < M, j < K (ld M), the
   // rows amap of A read before any later kernel changes A

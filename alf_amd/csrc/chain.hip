@@ -295,6 +295,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   a.W = chain_ptr(a.W, a.cs);
   a.amap = chain_ptr(a.amap, a.cs);
   ChainState* st = a.st;
+  if (a.lz && a.slice >= a.lz[blockIdx.z]) {
+    // a slice beyond this chain's Ltrot: no visits, no random numbers, nothing to flush
+    if (tid == 0) st->m = 0;
+    return;
+  }
   if (tid == 0) {
     s_pos0 = a.first_round ? 0 : st->pos;
     s_base = a.first_round ? st->rng_pos : st->rng_slice_base;

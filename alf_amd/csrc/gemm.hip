@@ -56,6 +56,10 @@ __device__ __forceinline__ void chain_operands(GemmArgs& g) {
   g.rs = mv(g.rs); g.cs = mv(g.cs); g.bs = mv(g.bs); g.rs_real = mv(g.rs_real);
   g.amap = mv(g.amap); g.bmap = mv(g.bmap); g.kdev = mv(g.kdev);
   g.yhat = mv(g.yhat);
+  if (g.lz && g.slice >= g.lz[blockIdx.z / (g.batch > 1 ? g.batch : 1)]) {
+    if (g.hop_side == 1) g.A = g.ident;
+    else if (g.hop_side == 2) g.B = g.ident;
+  }
 }
 
 // One K stage of an LDS-tiled kernel's operands, held in registers from its global loads to
@@ -738,6 +742,10 @@ GemmArgs gemm_args(int M, int N, int K, const void* A, int lda, int opA, bool a_
   g.cstride = 0;
   g.prec = 0;
   g.policy = 0;
+  g.lz = nullptr;
+  g.slice = 0;
+  g.hop_side = 0;
+  g.ident = nullptr;
   return g;
 }
 

@@ -9,8 +9,8 @@
 //
 // Product (oz_gemm_kernel): a workgroup of 8 waves owns a 64 x 64 complex output tile;
 // waves 0-3 the real plane, 4-7 the imaginary one, each a 32 x 32 block.  K advances in
-// stages of 64 (32 for complex x complex) through LDS holding every digit plane of both
-// operands for the stage.  For each k-step of 32 a wave keeps the OZ_S digit fragments of
+// stages of 32 through LDS holding every digit plane of both operands for the stage, the
+// next stage's planes loaded into registers while the current one is multiplied.  For each k-step of 32 a wave keeps the OZ_S digit fragments of
 // its B plane(s) in registers and, for each digit s of A, issues v_mfma_i32_32x32x32_i8 into the int32 accumulator of
 // s + t for every t with s + t <= OZ_S + 1: the sums of one s + t are exact.  At the end
 // the OZ_S sums are weighted by 2^(-7 (s + t)) and added in FP64 (smallest first), the two
@@ -173,9 +173,9 @@ template <int V>
 __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
   typedef OzVar<V> T;
   constexpr int NTILE = (T::PA + T::PB) * OZ_S;
-  // contraction bytes per LDS stage: 64 (two k-steps) when every digit plane of the stage
-  // fits in 128 KB, else 32
-  constexpr int KC = NTILE * OZ_T * 64 <= 131072 ? 64 : 32;
+  // contraction bytes per LDS stage: 32 (one k-step), so that the next stage's segments fit in
+  // registers beside the accumulators
+  constexpr int KC = 32;
   constexpr int SEGS = KC / 16;
   constexpr int STAGE = NTILE * OZ_T * KC;
   constexpr int RES = 2 * OZ_T * (OZ_T + 1) * (int)sizeof(double);
@@ -200,21 +200,43 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[s][r] = 0;
 
-  for (int k0 = 0; k0 < g.K; k0 += KC) {
-    // stage: every digit plane of A's 64 rows and B's 64 columns for k0 .. k0 + KC - 1
-    for (int seg = tid; seg < NTILE * OZ_T * SEGS; seg += OZ_NT) {
+  // a stage: every digit plane of A's 64 rows and B's 64 columns for k0 .. k0 + KC - 1, NPRE
+  // 16-byte segments per thread; the next stage's segments are loaded into registers while
+  // the current one is multiplied, and stored to LDS after the barrier that ends it
+  constexpr int NSEG = NTILE * OZ_T * SEGS, NPRE = (NSEG + OZ_NT - 1) / OZ_NT;
+  v4i pre[NPRE];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < NPRE; ++p) {
+      const int seg = tid + p * OZ_NT;
       const int tile = seg / (OZ_T * SEGS), rem = seg % (OZ_T * SEGS), row = rem / SEGS, sg = rem % SEGS;
       v4i v = {0, 0, 0, 0};
-      if (tile < T::PA * OZ_S) {
-        const int gi = i0 + row;
-        if (gi < g.M) v = *(const v4i*)(A + ((long long)tile * g.arows + gi) * g.akp + k0 + 16 * sg);
-      } else {
-        const int gj = j0 + row;
-        if (gj < g.N) v = *(const v4i*)(B + ((long long)(tile - T::PA * OZ_S) * g.brows + gj) * g.bkp + k0 + 16 * sg);
+      if (seg < NSEG) {
+        if (tile < T::PA * OZ_S) {
+          const int gi = i0 + row;
+          if (gi < g.M) v = *(const v4i*)(A + ((long long)tile * g.arows + gi) * g.akp + k0 + 16 * sg);
+        } else {
+          const int gj = j0 + row;
+          if (gj < g.N)
+            v = *(const v4i*)(B + ((long long)(tile - T::PA * OZ_S) * g.brows + gj) * g.bkp + k0 + 16 * sg);
+        }
       }
-      *(v4i*)(lds + lds_off<KC>(tile, row, sg)) = v;
+      pre[p] = v;
     }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int p = 0; p < NPRE; ++p) {
+      const int seg = tid + p * OZ_NT;
+      const int tile = seg / (OZ_T * SEGS), rem = seg % (OZ_T * SEGS), row = rem / SEGS, sg = rem % SEGS;
+      if (seg < NSEG) *(v4i*)(lds + lds_off<KC>(tile, row, sg)) = pre[p];
+    }
+  };
+  if (g.K > 0) gload(0);
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    lstore();
     __syncthreads();
+    if (k0 + KC < g.K) gload(k0 + KC);
 #pragma unroll
     for (int kk = 0; kk < KC / 32; ++kk) {
       if (q == 0) oz_kstep<V, 0, KC>(lds, bi, bj, lane, kk, acc);

@@ -590,7 +590,15 @@ class BatchEngine(Engine):
     def chain(self, z):
         """Select chain z for the per-chain accessors; returns self for chaining calls."""
         self._c(self.lib.alfq_set_chain(self.ctx, int(z)))
+        self._cur = int(z)
         return self
+
+    def get_fields(self):
+        """The selected chain's fields, n_op x its own Ltrot (chains of a batch may differ in Ltrot)."""
+        m = self.models[getattr(self, "_cur", 0)]
+        a = np.zeros((len(m.Op_V), m.Ltrot), dtype=np.int32, order="F")
+        self._c(self.lib.alfq_fields_get(self.ctx, _iptr(a), a.shape[0], a.shape[1]))
+        return a
 
     def load_chain(self, z, src, src_chain=0):
         """Chain src_chain of the context `src` (same shape, initialised) replaces chain z

@@ -317,6 +317,77 @@ class Server:
         return sorted(results, key=lambda x: x.rid)
 
 
+def uneven_batches(requests, max_ratio: float | None = None, max_batch: int = 256):
+    """Groups for batches of uneven Ltrot: requests of one lattice, size, Nwrap, hopping type and
+    sweep count, sorted by Ltrot; with max_ratio a group is cut where its largest Ltrot would
+    exceed max_ratio times its smallest (less padding, more batches)."""
+    groups = {}
+    for r in requests:
+        groups.setdefault((r.lattice, r.L, r.nwrap, r.n_phi != 0, r.nsweeps), []).append(r)
+    out = []
+    for key in sorted(groups):
+        rs = sorted(groups[key], key=lambda r: (r.ltrot, r.rid))
+        cur = []
+        for r in rs:
+            if cur and (len(cur) >= max_batch or (max_ratio is not None and r.ltrot > max_ratio * cur[0].ltrot)):
+                out.append(cur)
+                cur = []
+            cur.append(r)
+        if cur:
+            out.append(cur)
+    return out
+
+
+def run_uneven(requests, max_ratio: float | None = None, max_batch: int = 256, delay: int = 32,
+               clock=time.perf_counter, backend=None):
+    """Every request's chain in a batch of uneven Ltrot (uneven_batches), the batches advancing
+    concurrently from one host thread each on its own stream; returns (results by rid, wall s,
+    number of batches).  A chain whose Ltrot is below its batch's largest sweeps the largest
+    one with identity slices beyond its own (alfq_ctx_create_batch).  A backend passed in keeps
+    its host models between calls."""
+    be = backend if backend is not None else _HubbardBackend(delay=delay)
+    batches = uneven_batches(requests, max_ratio, max_batch)
+    results, errors = [], []
+    lock = threading.Lock()
+    t0 = clock()
+
+    def work(reqs):
+        try:
+            hams = [be.model(r) for r in reqs]
+            eng = be.rt.BatchEngine(hams, delay=delay)
+            try:
+                for z, (r, h) in enumerate(zip(reqs, hams)):
+                    with be._lock:
+                        h.nsigma.set_uniform_four_state(r.seed)
+                        f = h.nsigma.f.copy()
+                    eng.chain(z).set_fields(f)
+                    eng.chain(z).set_philox(seed=r.seed, stream=0)
+                eng.init()
+                t_adm = clock() - t0
+                eng.sweep(reqs[0].nsweeps)
+                eng.sync()
+                out = []
+                for z, r in enumerate(reqs):
+                    f, g, ph, ctl = be.collect(eng, z)
+                    out.append(Result(r.rid, r.arrival, t_adm, clock() - t0, r.nsweeps, len(reqs), f, g, ph, ctl))
+            finally:
+                eng.close()
+            with lock:
+                results.extend(out)
+        except Exception as exc:          # reported after the join
+            with lock:
+                errors.append(exc)
+
+    threads = [threading.Thread(target=work, args=(b,), daemon=True) for b in batches]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    if errors:
+        raise errors[0]
+    return sorted(results, key=lambda x: x.rid), clock() - t0, len(batches)
+
+
 def poisson_requests(n: int, rate: float, seed: int = 0, nsweeps: int = 500, sizes=(8, 12, 16, 24, 32),
                      weights=(4, 3, 2, 1, 1), betas=(4.0, 8.0, 16.0), us=(2.0, 4.0, 6.0)):
     """A stream of Hubbard requests: Poisson arrivals at `rate` per second, square lattices of

@@ -141,20 +141,33 @@ def serving_rate(nreq, nsweeps, rate, seed=0):
 def c4_mix_rate(chains, nsweeps, seed=0):
     """The batched-chains mix: `chains` independent Hubbard chains, square L uniform in
     {6, 8, ..., 20} (Ndim 36..400), U uniform in [2, 8], beta uniform in [2, 12] (Ltrot =
-    nint(beta / 0.1)), Nwrap 10, chain i seeded with i, all submitted at once to the serving
-    loop: chains of one shape (L, Ltrot) share a batch, the batches run concurrently."""
-    from alf_amd.serving import Request, Server
+    nint(beta / 0.1)), Nwrap 10, chain i seeded with i, all at once: one batch of uneven Ltrot
+    per lattice size (every kernel launch covers the size's chains; a chain sweeps its batch's
+    largest Ltrot with identity slices beyond its own), the batches concurrent on their own
+    streams.  The steady rate is the extra sweeps of a second run with more sweeps per chain
+    over its extra wall time, so model set-up, context creation, init and download cancel."""
+    from alf_amd.serving import Request, _HubbardBackend, run_uneven
     rng = np.random.default_rng(seed)
-    reqs = [Request(rid=i, L=int(rng.choice(np.arange(6, 21, 2))), beta=float(rng.uniform(2.0, 12.0)),
-                    U=float(rng.uniform(2.0, 8.0)), seed=i, nsweeps=nsweeps, arrival=0.0) for i in range(chains)]
-    srv = Server(min_slots=1)
-    res = srv.run(reqs)
-    return {"chains": chains, "sweeps_per_chain": nsweeps, "distinct_shapes": len({r.shape() for r in reqs}),
-            "aggregate_sweeps_per_s": chains * nsweeps / srv.wall, "wall_s": srv.wall,
-            "peak_concurrent_batches": srv.peak_batches, "largest_batch": max(r.batch_size for r in res),
-            "precision_green_max": max(r.control["XMAXG"] for r in res),
-            "form": "per-shape batches on concurrent streams (a batch shares Ndim and Ltrot); wall includes "
-                    "each chain's model set-up, context creation, init and download"}
+    draws = [(int(rng.choice(np.arange(6, 21, 2))), float(rng.uniform(2.0, 12.0)), float(rng.uniform(2.0, 8.0)))
+             for _ in range(chains)]
+    be = _HubbardBackend()
+    for i, (L, b, U) in enumerate(draws):          # host model set-up once, outside the timed runs
+        be.model(Request(rid=i, L=L, beta=b, U=U, seed=i, nsweeps=1))
+    out = {"chains": chains, "distinct_shapes": len({(L, int(round(b / 0.1))) for L, b, _ in draws})}
+    for label, ratio in (("one_batch_per_size", None), ("batches_cut_at_ltrot_ratio_2", 2.0)):
+        walls, nb, worst = {}, 0, 0.0
+        for sw in (nsweeps, 3 * nsweeps):
+            reqs = [Request(rid=i, L=L, beta=b, U=U, seed=i, nsweeps=sw) for i, (L, b, U) in enumerate(draws)]
+            res, wall, nb = run_uneven(reqs, max_ratio=ratio, backend=be)
+            walls[sw] = wall
+            worst = max(worst, max(r.control["XMAXG"] for r in res))
+        steady = chains * 2 * nsweeps / (walls[3 * nsweeps] - walls[nsweeps])
+        out[label] = {"batches": nb, "aggregate_sweeps_per_s": steady,
+                      "wall_s": walls, "with_setup_sweeps_per_s": chains * nsweeps / walls[nsweeps],
+                      "precision_green_max": worst}
+    out["aggregate_sweeps_per_s"] = max(out[k]["aggregate_sweeps_per_s"] for k in
+                                        ("one_batch_per_size", "batches_cut_at_ltrot_ratio_2"))
+    return out
 
 
 def measured_rate(ham, fields, delay, steps):

@@ -77,10 +77,16 @@ int alfq_ctx_create(const alfq_model_desc* desc, alfq_ctx** out);
 int alfq_ctx_destroy(alfq_ctx* ctx);
 /* Batches (no counterpart in the reference, which runs one chain per MPI rank): nchain
  * independent Markov chains that advance in lockstep, every kernel launch covering all
- * of them.  descs[z] describes chain z; Ndim, Ltrot, Nwrap, N_SUN, the operators' sites
- * and field types and the type of T must agree, the couplings (U, ...) and exp(-dtau T)
- * may differ.  alfq_ctx_create(desc) is the batch of one.  Sweeps, wraps and CGR act on
- * every chain; the accessors (fields, G, PHASE, random stream, UDV reads, control,
+ * of them.  descs[z] describes chain z; Ndim, Nwrap, N_SUN, the operators' sites and field
+ * types and the type of T must agree, the couplings (U, ...), exp(-dtau T) and Ltrot may
+ * differ.  With uneven Ltrot the batch sweeps the largest one, Lmax, with the stabilisation
+ * schedule of Lmax: a chain of Ltrot < Lmax treats the slices beyond its own as identities
+ * (no fields, no visits, no random numbers, exp(-dtau T) replaced by 1), which leaves its
+ * Green's functions and weights unchanged, so it follows its own trajectory up to rounding
+ * (its stabilisation points and NVAR follow Lmax); fields are set and read per chain with
+ * the chain's own Ltrot.  Uneven batches take no measurements, checkerboard Op_T, int8
+ * split products or alfq_chain_load.  alfq_ctx_create(desc) is the batch of one.  Sweeps,
+ * wraps and CGR act on every chain; the accessors (fields, G, PHASE, random stream, UDV reads, control,
  * precision log) act on the chain chosen with alfq_set_chain (chain 0 initially).
  * Random tapes and alfq_udv_set are single-chain only. */
 int alfq_ctx_create_batch(const alfq_model_desc* descs, int32_t nchain, alfq_ctx** out);

@@ -5,9 +5,10 @@
   oracle on the same random tape (fields and counters bit-exact, G and PHASE within 1e-10).
 * Uneven-size chains: 256 independent Hubbard chains, square L uniform in {6, ..., 20}
   (Ndim 36..400), U uniform in [2, 8], beta uniform in [2, 12], Nwrap 10, chain i seeded
-  with i, all submitted at once: every chain's mean Precision Green stays below 1e-8 (its
-  largest single value below 1e-6), and sampled chains (the smallest, the largest and two
-  others) equal their single-chain runs.
+  with i, run as one batch of uneven Ltrot per lattice size: every chain's mean Precision
+  Green stays below 1e-8 (its largest single value below 1e-6), and sampled chains (the
+  smallest, the largest and two others) take the decisions of their single-chain runs, with
+  G equal to rounding.
 * Served requests at the largest sizes of the serving stream (Ndim 576 at beta 16, Ndim 1024)
   mixed with small ones: each result equals the request served alone, Precision Green below
   1e-8.
@@ -61,13 +62,14 @@ def uneven_mix(nchain, nsweeps, seed=0):
 
 
 def test_uneven_mix_of_256_chains(rt):
-    from alf_amd.serving import Request, Server
+    from alf_amd.serving import run_uneven
     reqs = uneven_mix(256, 2)
-    res = Server(min_slots=1).run(reqs)
-    assert [r.rid for r in res] == list(range(256))
+    res, wall, nb = run_uneven(reqs)
+    assert [r.rid for r in res] == list(range(256)) and nb == 8      # one batch per lattice size
     worst = max(r.control["XMAXG"] for r in res)
     mean = max(r.control["precision_green_mean"] for r in res)
-    print(f"\n256 uneven chains: Precision Green max {worst:.2e}, largest per-chain mean {mean:.2e}", flush=True)
+    print(f"\n256 uneven chains in {nb} batches: Precision Green max {worst:.2e}, largest per-chain mean "
+          f"{mean:.2e}, {wall:.1f} s", flush=True)
     # the reference's criterion is the mean (Documentation/stabilization.tex:225); single
     # stabilisations of FP64 chains at U up to 8 and beta up to 12 reach a few 1e-8
     assert mean < 1e-8
@@ -76,11 +78,18 @@ def test_uneven_mix_of_256_chains(rt):
     n = [r.L * r.L for r in reqs]
     sample = {int(np.argmin(n)), int(np.argmax(n)), 17, 200}
     for i in sorted(sample):
-        solo = Server(min_slots=1).run([Request(**{**reqs[i].__dict__})])[0]
+        r = reqs[i]
+        h = hubbard(L1=r.L, L2=r.L, beta=r.beta, U=r.U, nwrap=r.nwrap)
+        e = rt.Engine(h, delay=32)
+        e.set_fields(fields_for(h, r.seed))
+        e.set_philox(seed=r.seed, stream=0)
+        e.init()
+        e.sweep(r.nsweeps)
         m = res[i]
-        assert np.array_equal(solo.fields, m.fields)
-        assert rel(solo.green, m.green) < TOL
-        assert abs(solo.phase - m.phase) < TOL
+        assert np.array_equal(e.get_fields(), m.fields)
+        assert rel(e.get_green(), m.green) < 1e-9
+        assert abs(e.get_phase() - m.phase) < 1e-9
+        e.close()
 
 
 def test_large_served_requests_same_alone_and_in_mix(rt):

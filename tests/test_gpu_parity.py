@@ -398,6 +398,40 @@ def test_chain_loaded_into_a_running_batch_continues_as_alone(rt):
     b.close()
 
 
+def test_uneven_ltrot_batch_matches_single_chains(rt):
+    """A batch of chains of different Ltrot (and U) sweeps the largest Ltrot with identity
+    slices beyond each chain's own: every chain takes the decisions it takes alone (fields,
+    counters and random-stream positions exact) and its G agrees to rounding (its
+    stabilisation points follow the batch's Ltrot)."""
+    cases = [(1.0, 4.0), (1.7, 2.5), (2.3, 6.0), (0.6, 3.0)]       # (beta, U): Ltrot 10, 17, 23, 6
+    hams = [hubbard(L1=6, L2=6, beta=b, U=U, nwrap=5) for b, U in cases]
+    b = rt.BatchEngine(hams, delay=16)
+    for z, h in enumerate(hams):
+        b.chain(z).set_fields(fields_for(h, 70 + z))
+        b.chain(z).set_philox(300 + z, 0)
+    b.init()
+    b.sweep(3)
+    for z, h in enumerate(hams):
+        e = rt.Engine(h, delay=16)
+        e.set_fields(fields_for(h, 70 + z))
+        e.set_philox(300 + z, 0)
+        e.init()
+        e.sweep(3)
+        b.chain(z)
+        assert b.get_fields().shape == (h.n_op, h.Ltrot)
+        assert np.array_equal(b.get_fields(), e.get_fields())
+        assert rel(b.get_green(), e.get_green()) < 1e-9
+        assert abs(b.get_phase() - e.get_phase()) < 1e-9
+        cb, ce = b.control(), e.control()
+        assert cb["ACC_up"] == ce["ACC_up"] and cb["NC_up"] == ce["NC_up"]
+        assert b.rng_position() == e.rng_position()
+        assert cb["XMAXG"] < 1e-8
+        e.close()
+    with pytest.raises(RuntimeError, match="one Ltrot"):
+        b.set_precision(rt.PREC_INT8_SPLIT)
+    b.close()
+
+
 def test_gemm_policies_and_autotuner(rt):
     """The four GEMM tiling policies give the same trajectory (fields and counters exact, G
     to rounding) on a batch of 32 chains of 16 x 16 (grids around the policies' switch
