@@ -48,6 +48,29 @@ template <class T>
 __device__ __forceinline__ T* chain_ptr(T* p, long long stride) {
   return p ? (T*)((char*)p + (long long)blockIdx.z * stride) : p;
 }
+template <class T>
+__device__ __forceinline__ T* chain_ptr_at(T* p, long long stride, int z) {
+  return p ? (T*)((char*)p + (long long)z * stride) : p;
+}
+
+// XCD-aware order of a batched grid (tiles in x, y; chains or problems in z).  The device
+// hands workgroups to its 8 XCDs in turn by linear id, so in grid order the tiles of one
+// chain spread over all 8 L2s, each fetching the chain's operands.  When gridDim.z is a
+// multiple of 8 the linear ids are renumbered (a bijection) so that every tile of one z lands
+// on the same XCD: id = 8 i + x runs z = x + 8 (i / tiles), tile i % tiles.  Other grids keep
+// their order.
+__device__ __forceinline__ void xcd_order(int& bx, int& by, int& bz) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  bz = blockIdx.z;
+  if ((gridDim.z & 7) != 0) return;
+  const long long P = (long long)gridDim.x * gridDim.y;
+  const long long L = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+  const long long i = L >> 3, t = i % P;
+  bz = (int)((L & 7) + 8 * (i / P));
+  bx = (int)(t % gridDim.x);
+  by = (int)(t / gridDim.x);
+}
 
 // Two-pass layout of a chain's arena: the first pass (base == nullptr) only sizes it, the
 // second assigns pointers at the same 256-byte aligned offsets.  With cap > 0 the layout

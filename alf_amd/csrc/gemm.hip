@@ -47,16 +47,16 @@ template <> struct Mma<true> {
   __device__ __forceinline__ static int row(int r, int lk) { return 4 * lk + r; }
 };
 
-// moves every operand pointer to the arena of the chain blockIdx.z / batch
-__device__ __forceinline__ void chain_operands(GemmArgs& g) {
+// moves every operand pointer to the arena of the chain bz / batch
+__device__ __forceinline__ void chain_operands(GemmArgs& g, int bz) {
   if (g.nchain <= 1) return;
-  const long long off = (long long)(blockIdx.z / (g.batch > 1 ? g.batch : 1)) * g.cstride;
+  const long long off = (long long)(bz / (g.batch > 1 ? g.batch : 1)) * g.cstride;
   auto mv = [off](auto p) { return p ? (decltype(p))((const char*)p + off) : p; };
   g.A = mv(g.A); g.B = mv(g.B); g.C = (void*)mv((const char*)g.C);
   g.rs = mv(g.rs); g.cs = mv(g.cs); g.bs = mv(g.bs); g.rs_real = mv(g.rs_real);
   g.amap = mv(g.amap); g.bmap = mv(g.bmap); g.kdev = mv(g.kdev);
   g.yhat = mv(g.yhat);
-  if (g.lz && g.slice >= g.lz[blockIdx.z / (g.batch > 1 ? g.batch : 1)]) {
+  if (g.lz && g.slice >= g.lz[bz / (g.batch > 1 ? g.batch : 1)]) {
     if (g.hop_side == 1) g.A = g.ident;
     else if (g.hop_side == 2) g.B = g.ident;
   }
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
   const int j0 = blockIdx.y * TN;
-  chain_operands(g);
+  chain_operands(g, blockIdx.z);
   const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
@@ -303,10 +303,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   __shared__ cplx bsl[BK];
 
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * TM;
-  const int j0 = blockIdx.y * TN;
-  chain_operands(g);
-  const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
+  int bx, by, bz;
+  xcd_order(bx, by, bz);
+  const int i0 = bx * TM;
+  const int j0 = by * TN;
+  chain_operands(g, bz);
+  const int z = bz % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
     const int kd = *g.kdev;
@@ -415,15 +417,17 @@ __global__ __launch_bounds__(NT, 2) void flush_kernel(cplx* G, const cplx* X, co
                                                       int n, int kmax, long long cs) {
   typedef Mma<false> M;
   __shared__ double Xr[FL_K][FL_T + 1], Xi[FL_K][FL_T + 1], Yr[FL_K][FL_T + 1], Yi[FL_K][FL_T + 1];
-  G = chain_ptr(G, cs);
-  X = chain_ptr(X, cs);
-  Y = chain_ptr(Y, cs);
-  mdev = chain_ptr(mdev, cs);
+  int bx, by, bz;
+  xcd_order(bx, by, bz);
+  G = chain_ptr_at(G, cs, bz);
+  X = chain_ptr_at(X, cs, bz);
+  Y = chain_ptr_at(Y, cs, bz);
+  mdev = chain_ptr_at(mdev, cs, bz);
   int m = *mdev;
   m = m < kmax ? m : kmax;
   if (m <= 0) return;
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * FL_T, j0 = blockIdx.y * FL_T;
+  const int i0 = bx * FL_T, j0 = by * FL_T;
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
   const int li = tid & 63, lkb = tid >> 6;   // operand rows: index li, columns k = lkb + 4 p
@@ -557,7 +561,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   const int wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-  chain_operands(g);
+  chain_operands(g, blockIdx.z);
   const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {

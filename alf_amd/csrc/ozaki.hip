@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ 
                                                        const cplx* __restrict__ kscale, long long xs,
                                                        int8_t* __restrict__ out, int* __restrict__ ex, int rows,
                                                        int planes, long long kp, long long os) {
+  constexpr int KL = 16;                   // contraction entries per lane and chunk: one 16-byte store
   const long long z = blockIdx.z;
   X = (const char*)X + z * xs;
   if (kscale) kscale = (const cplx*)((const char*)kscale + z * xs);
@@ -61,11 +62,11 @@ __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ 
   };
 
   double m = 0.0;
-  for (int k0 = 0; k0 < nk; k0 += 64) {
+  for (int k0 = 0; k0 < nk; k0 += 16 * KL) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < KL; ++u) {
       double re, im;
-      load(k0 + 4 * kl + u, re, im);
+      load(k0 + KL * kl + u, re, im);
       m = fmax(m, fmax(fabs(re), fabs(im)));
     }
   }
@@ -83,34 +84,33 @@ __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ 
   if (live && kl == 0) ex[idx] = e;
   if (!live) return;
 
-  for (long long k0 = 0; k0 < kp; k0 += 64) {
-    const int k = (int)k0 + 4 * kl;
-    double vr[4], vi[4];
+  for (long long k0 = 0; k0 < kp; k0 += 16 * KL) {
+    const int k = (int)k0 + KL * kl;
+    if (k >= kp) break;                    // kp is a multiple of 64 and of KL
+    double vr[KL], vi[KL];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < KL; ++u) {
       load(k + u, vr[u], vi[u]);
       vr[u] = ldexp(vr[u], -e);
       vi[u] = ldexp(vi[u], -e);
     }
-    for (int p = 0; p < (CPLX ? 2 : 1); ++p) {
-      double r[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = p ? vi[u] : vr[u];
+    for (int p = 0; p < (CPLX ? 2 : 1); ++p) {
+      double* r = p ? vi : vr;
 #pragma unroll
       for (int s = 0; s < OZ_S; ++s) {
-        unsigned w = 0, wn = 0;
+        v4i w = {0, 0, 0, 0}, wn = {0, 0, 0, 0};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < KL; ++u) {
           r[u] *= 128.0;
           const double d = trunc(r[u]);
           r[u] -= d;
           const int di = (int)d;            // |di| <= 127
-          w |= (unsigned)(di & 0xff) << (8 * u);
-          wn |= (unsigned)((-di) & 0xff) << (8 * u);
+          w[u >> 2] |= (int)((unsigned)(di & 0xff) << (8 * (u & 3)));
+          wn[u >> 2] |= (int)((unsigned)((-di) & 0xff) << (8 * (u & 3)));
         }
-        *(unsigned*)(out + ((long long)(p * OZ_S + s) * rows + idx) * kp + k) = w;
-        if (p == 1 && planes == 3)
-          *(unsigned*)(out + ((long long)(2 * OZ_S + s) * rows + idx) * kp + k) = wn;
+        *(v4i*)(out + ((long long)(p * OZ_S + s) * rows + idx) * kp + k) = w;
+        if (p == 1 && planes == 3) *(v4i*)(out + ((long long)(2 * OZ_S + s) * rows + idx) * kp + k) = wn;
       }
     }
   }
@@ -181,7 +181,9 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
   constexpr int RES = 2 * OZ_T * (OZ_T + 1) * (int)sizeof(double);
   __shared__ __align__(16) int8_t lds[STAGE > RES ? STAGE : RES];
 
-  const long long z = blockIdx.z;
+  int bx, by, bzi;
+  xcd_order(bx, by, bzi);
+  const long long z = bzi;
   const int8_t* A = g.A + z * g.as;
   const int8_t* B = g.B + z * g.bs;
   const int* ea = (const int*)((const char*)g.ea + z * g.as);
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int q = wave >> 2, bi = wave & 1, bj = (wave >> 1) & 1;
-  const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T;
+  const int i0 = bx * OZ_T, j0 = by * OZ_T;
 
   v16i acc[OZ_S];
 #pragma unroll

@@ -552,7 +552,9 @@ def test_gemm_fp32_arithmetic(rt, shape):
 def test_fp32_wraps_sweep_precision(rt):
     """With the hopping wraps of G on FP32 MFMA, every stabilisation point still rebuilds G in
     FP64 from the UDV chain (FP64 in both modes), so Precision Green measures the FP32 drift
-    over Nwrap slices: well above FP64 roundoff, and bounded."""
+    over Nwrap slices: well above FP64 roundoff, and bounded.  This form does not meet the
+    1e-8 bar (about 1e-3 at 16 x 16, beta = 10) and is kept as a measured contrast; the bar is
+    asserted on the int8 split products (tests/test_gpu_int8_split.py)."""
     ham = hubbard(L1=6, L2=6, beta=4.0, nwrap=10)
     f = fields_for(ham, 11)
     e = rt.Engine(ham, delay=16)
