@@ -9,17 +9,18 @@ continuous batching:
   lane: a batched context (``BatchEngine``: every kernel launch covers all its slots) driven
   by a host thread of its own on its own HIP stream, sweeping one sweep at a time; couplings
   and seeds differ per slot.  Lanes of different shapes overlap on the device.
-* A request joins its shape's running lane at the next sweep boundary as soon as a slot is
-  free: it is initialised in a small context of its own (model set-up, fields, Philox stream,
-  the UDV stack and G at tau = 0) and moved into the free slot (``alfq_chain_load``); the
-  other slots keep sweeping.  A slot whose request has its sweeps returns the chain's fields,
-  G, phase and precision statistics to the host and becomes free.  Free slots keep a copy of
-  a chain and sweep it unread until a request takes them.
-* A shape without a lane gets a new one as soon as the device memory of its slots
-  (``alfq_chain_bytes`` per slot) fits in the budget beside the lanes already running; its
-  slot count is the number of its requests waiting then (at least ``min_slots``, at most
-  ``max_batch``).  A lane with no request in its slots and none waiting closes and frees its
-  memory.
+* A request joins a running lane of its shape that has a slot free at its next sweep
+  boundary: it is initialised in a small context of its own (model set-up, fields, Philox
+  stream, the UDV stack and G at tau = 0) and moved into the free slot (``alfq_chain_load``);
+  the other slots keep sweeping.  A slot whose request has its sweeps returns the chain's
+  fields, G, phase and precision statistics to the host and becomes free.  Free slots keep a
+  copy of a chain and sweep it unread until a request takes them.
+* A request without such a slot opens a new lane as soon as the device memory of its slots
+  (``alfq_chain_bytes`` per slot) fits in the budget beside the lanes already running; the
+  slot count is the number of requests of its shape waiting then (at least ``min_slots``, at
+  most ``max_batch``).  Without the memory it waits behind a running lane of its shape (and
+  joins it when a slot frees) or, with no such lane, for memory.  A lane with no request in
+  its slots and none waiting closes and frees its memory.
 
 Time-to-result of a request = its arrival -> its result on the host.  A chain's trajectory
 does not depend on the lane, the slot or the other chains (chains are independent, seeded
@@ -154,14 +155,17 @@ class _Lane:
     def __init__(self, server, key, reqs, nslot, nbytes):
         self.server, self.key, self.nslot, self.nbytes = server, key, nslot, nbytes
         self.waiting = list(reqs)           # requests of this shape not yet in a slot
+        self.busy = min(len(reqs), nslot)   # occupied slots
         self.cv = threading.Condition()
         self.closed = False
         self.thread = threading.Thread(target=self._run, daemon=True)
 
-    def offer(self, r) -> bool:
-        """Queue a request for this lane; False when the lane has already closed."""
+    def offer(self, r, wait: bool = False) -> bool:
+        """Queue a request for this lane when a slot is free for it at the next sweep boundary
+        (or, with wait, behind the lane's other waiting requests); False when the lane is full
+        or has closed."""
         with self.cv:
-            if self.closed:
+            if self.closed or (not wait and self.busy + len(self.waiting) >= self.nslot):
                 return False
             self.waiting.append(r)
             return True
@@ -189,6 +193,8 @@ class _Lane:
                                                self.nslot, f, g, ph, ctl))
                             del slots[z]
                     if done:
+                        with self.cv:
+                            self.busy = len(slots)
                         srv.done.put(("results", self, done))
                     # refill free slots at this sweep boundary
                     while True:
@@ -205,6 +211,8 @@ class _Lane:
                         finally:
                             src.close()
                         slots[free[0]] = [r, r.nsweeps, srv.clock() - srv.t0]
+                        with self.cv:
+                            self.busy = len(slots)
                     if self.closed:
                         break
                     srv.lane_peak = max(srv.lane_peak, len(slots))
@@ -217,15 +225,20 @@ class _Lane:
             srv.done.put(("error", self, exc))
 
 
+def n_lanes(lanes) -> int:
+    return sum(len(group) for group in lanes.values())
+
+
 class Server:
     """Continuous-batching loop over a list of timed requests (see the module docstring)."""
 
     def __init__(self, memory_budget: int | None = None, max_batch: int = 64, backend=None, clock=time.perf_counter,
-                 min_slots: int = 2):
+                 min_slots: int = 2, progress=None):
         self.backend = backend if backend is not None else _HubbardBackend()
         self.budget = memory_budget if memory_budget is not None else int(0.8 * device_free_bytes())
         self.max_batch = max_batch
         self.min_slots = min_slots
+        self.progress = progress           # called with each list of results as it arrives
         self.clock = clock
         self.peak_batches = 0              # lanes running at once
         self.lane_peak = 0                 # occupied slots of one lane at a sweep boundary
@@ -239,15 +252,16 @@ class Server:
         used = 0
         per_chain = {}
         self.t0 = self.clock()
-        while pending or queue or lanes:
+        while pending or queue or n_lanes(lanes):
             now = self.clock() - self.t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.pop(0))
-            # a request of a shape with a running lane joins it; the others open lanes by memory
+            # a request joins a running lane of its shape that has a slot free at its next sweep
+            # boundary; the others open new lanes as memory allows, and otherwise wait behind
+            # a running lane of their shape
             rest = []
             for r in queue:
-                ln = lanes.get(r.shape())
-                if ln is not None and ln.offer(r):
+                if any(ln.offer(r) for ln in lanes.get(r.shape(), [])):
                     self.joined += 1
                 else:
                     rest.append(r)
@@ -255,17 +269,20 @@ class Server:
             while queue:
                 head = queue[0]
                 key = head.shape()
-                if key in lanes:
-                    break                   # its lane is closing: wait for it
                 same = [r for r in queue if r.shape() == key]
                 if key not in per_chain:
                     per_chain[key] = self.backend.chain_bytes(head)
                 nslot = min(self.max_batch, max(len(same), self.min_slots))
                 if used + per_chain[key] * nslot > self.budget:
-                    nslot = max(len(same), 1)
+                    nslot = min(self.max_batch, max(len(same), 1))
                 need = per_chain[key] * nslot
                 if used + need > self.budget:
-                    if lanes:
+                    running = lanes.get(key, [])
+                    if running and running[0].offer(head, wait=True):
+                        self.joined += 1
+                        queue.pop(0)
+                        continue
+                    if n_lanes(lanes):
                         break               # wait for memory to free up
                     if per_chain[key] > self.budget:
                         raise RuntimeError(f"request {head.rid} needs {per_chain[key]} B, above the budget")
@@ -274,17 +291,17 @@ class Server:
                 # the lane takes every waiting request of its shape: the first nslot start at once,
                 # the others join as slots free up
                 ln = _Lane(self, key, same, nslot, need)
-                lanes[key] = ln
+                lanes.setdefault(key, []).append(ln)
                 used += need
                 self.joined += max(0, len(same) - nslot)
                 ids = {r.rid for r in same}
                 queue = [r for r in queue if r.rid not in ids]
                 ln.thread.start()
-            self.peak_batches = max(self.peak_batches, len(lanes))
+            self.peak_batches = max(self.peak_batches, n_lanes(lanes))
             timeout = None
             if pending:
                 timeout = max(0.0, pending[0].arrival - (self.clock() - self.t0))
-            elif not lanes:
+            elif not n_lanes(lanes):
                 continue
             try:
                 item = self.done.get(timeout=timeout)
@@ -294,19 +311,25 @@ class Server:
                 kind, ln, payload = item
                 if kind == "results":
                     results.extend(payload)
+                    if self.progress is not None:
+                        self.progress(payload)
                 elif kind == "closed":
                     ln.thread.join()
-                    if lanes.get(ln.key) is ln:
-                        del lanes[ln.key]
+                    group = lanes.get(ln.key, [])
+                    if ln in group:
+                        group.remove(ln)
+                    if not group:
+                        lanes.pop(ln.key, None)
                     used -= ln.nbytes
                     with ln.cv:                 # requests offered after the lane's last check
                         queue.extend(ln.waiting)
                         ln.waiting = []
                 else:
-                    for other in lanes.values():
+                    every = [o for group in lanes.values() for o in group]
+                    for other in every:
                         with other.cv:
                             other.waiting = []
-                    for other in lanes.values():
+                    for other in every:
                         other.thread.join()
                     raise payload
                 try:
@@ -408,3 +431,40 @@ def latency_summary(results, wall: float) -> dict:
     sweeps = sum(r.nsweeps for r in results)
     return {"requests": len(results), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "max_s": float(lat.max()), "aggregate_sweeps_per_s": sweeps / wall, "wall_s": wall}
+
+
+def main(argv=None) -> None:
+    """A Poisson stream of serving-configuration requests through the loop, summary as JSON:
+
+        python3 -m alf_amd.serving --requests 16 --sweeps 500 --rate 50 --seed 0
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--sweeps", type=int, default=500)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--min-slots", type=int, default=2)
+    args = ap.parse_args(argv)
+    reqs = poisson_requests(args.requests, args.rate, seed=args.seed, nsweeps=args.sweeps)
+    import sys
+
+    def progress(done):
+        for r in done:
+            print(f"[serving] request {r.rid} done after {r.latency:.1f} s", file=sys.stderr, flush=True)
+    srv = Server(min_slots=args.min_slots, progress=progress)
+    res = srv.run(reqs)
+    out = latency_summary(res, srv.wall)
+    out.update({"sweeps_per_request": args.sweeps, "rate_per_s": args.rate, "seed": args.seed,
+                "shapes": sorted({f"L={r.L} Ltrot={r.ltrot}" for r in reqs}),
+                "peak_lanes": srv.peak_batches, "joined_running_lanes": srv.joined,
+                "precision_green_max": max(r.control["XMAXG"] for r in res),
+                "precision_green_mean_worst": max(r.control["precision_green_mean"] for r in res),
+                "latency_s_by_request": {r.rid: round(r.latency, 3) for r in res}})
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
+

@@ -28,6 +28,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
+def note(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
 WORKLOAD = dict(Lattice_type="Square", L1=16, L2=16, ham_T=1.0, Ham_U=4.0, Beta=10.0, Dtau=0.1, Nwrap=10)
 
 
@@ -126,7 +132,7 @@ def serving_rate(nreq, nsweeps, rate, seed=0):
     served by the continuous-batching loop (alf_amd.serving)."""
     from alf_amd.serving import Server, latency_summary, poisson_requests
     reqs = poisson_requests(nreq, rate, seed=seed, nsweeps=nsweeps)
-    srv = Server(min_slots=4)
+    srv = Server(min_slots=2)
     res = srv.run(reqs)
     out = latency_summary(res, srv.wall)
     out.update({"sample": f"{nreq} requests x {nsweeps} sweeps (the serving configuration asks 500 sweeps each), "
@@ -153,12 +159,15 @@ def c4_mix_rate(chains, nsweeps, seed=0):
     be = _HubbardBackend()
     for i, (L, b, U) in enumerate(draws):          # host model set-up once, outside the timed runs
         be.model(Request(rid=i, L=L, beta=b, U=U, seed=i, nsweeps=1))
+        if i % 32 == 31:
+            note(f"uneven mix: {i + 1} models set up")
     out = {"chains": chains, "distinct_shapes": len({(L, int(round(b / 0.1))) for L, b, _ in draws})}
     for label, ratio in (("one_batch_per_size", None), ("batches_cut_at_ltrot_ratio_2", 2.0)):
         walls, nb, worst = {}, 0, 0.0
         for sw in (nsweeps, 3 * nsweeps):
             reqs = [Request(rid=i, L=L, beta=b, U=U, seed=i, nsweeps=sw) for i, (L, b, U) in enumerate(draws)]
             res, wall, nb = run_uneven(reqs, max_ratio=ratio, backend=be)
+            note(f"uneven mix, {label}: {sw} sweeps per chain in {wall:.1f} s")
             walls[sw] = wall
             worst = max(worst, max(r.control["XMAXG"] for r in res))
         steady = chains * 2 * nsweeps / (walls[3 * nsweeps] - walls[nsweeps])
@@ -304,7 +313,7 @@ def main():
     ap.add_argument("--only-headline", action="store_true",
                     help="the single-chain line and its roofline only (for kernel profiles of that run)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
-    ap.add_argument("--serve-sweeps", type=int, default=10)
+    ap.add_argument("--serve-sweeps", type=int, default=20)
     args = ap.parse_args()
     if args.only_headline:
         args.batch, args.no_mixed, args.no_checkerboard, args.no_configs = 0, True, True, True
@@ -321,6 +330,7 @@ def main():
     from alf_amd.runtime import Engine
     from alf_amd import kernelbench
 
+    note("single chain (headline)")
     ham = hubbard_ham_set(HubbardParams(**WORKLOAD))
     ham.nsigma.set_uniform_four_state(0)
     fields = ham.nsigma.f.copy()
@@ -372,20 +382,22 @@ def main():
     }
     eng.close()
     if not args.only_headline:
+        note("with measurements")
         result["with_measurements"] = measured_rate(ham, fields, args.delay, args.steps)
         result["with_time_displaced"] = tau_rate(ham, fields, args.delay, max(2, args.steps // 2))
     if args.batch > 0:
+        note(f"batch of {args.batch}, FP64")
         result["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)))
-        # above 1: the delayed updates keep the rank-1 traffic out of HBM, which the model charges;
-        # the delayed form prices the rank-32 flushes the engine runs
+        # priced for the rank-32 delayed flushes the engine runs (the per-flip ZGERU streaming of
+        # the reference's form would put the batch above 1, so that form is not reported here)
         agg = result["batched"]["aggregate_sweeps_per_s"]
-        result["batched"]["sweep_roofline_fraction"] = attainable_sweep_seconds(n, L, S, acc) * agg
         result["batched"]["sweep_roofline_fraction_delayed"] = attainable_sweep_seconds(n, L, S, acc, delay=32) * agg
     # the chosen precision form: hopping wraps of G as exact int8 digit products on the matrix
     # cores (8 digits of 7 bits per operand), everything else FP64; FP32 wraps beside it
     from alf_amd.runtime import PREC_FP32_WRAPS, PREC_INT8_SPLIT
     mixed = None
     if not args.no_mixed:
+        note("mixed precision forms")
         mixed = {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) as int8 split products: each "
                          "operand row/column = 2^e x 8 truncated base-128 int8 digits, digit pairs with s+t <= 9 "
                          "multiplied exactly on v_mfma_i32_32x32x32_i8 (int32 sums), combined in FP64; UDV chain, "
@@ -414,6 +426,7 @@ def main():
         mixed["fp32_wraps"] = fp32
         result["mixed"] = mixed
     if not args.no_checkerboard:
+        note("checkerboard")
         # the Hubbard plugin's default hopping: the checkerboard product of bond exponentials
         cb_ham = hubbard_ham_set(HubbardParams(**WORKLOAD, Checkerboard=True))
         cb = {"form": "Checkerboard=.T.: 4 families of 2x2 bond exponentials applied from LDS strips "
@@ -423,10 +436,13 @@ def main():
             cb["batched"] = batched_rate(args.batch, max(2, min(args.steps, 5)), 0, checkerboard=True)
         result["checkerboard"] = cb
     if args.serve_requests > 0:
+        note("serving")
         result["serving"] = serving_rate(args.serve_requests, args.serve_sweeps, 50.0)
     if not args.no_configs:
+        note("other configurations")
         result["other_configs"] = config_rates(args.delay, cpu=not args.no_cpu_baseline)
     if args.c4_chains > 0:
+        note("uneven mix")
         result["uneven_batch_mix"] = c4_mix_rate(args.c4_chains, args.c4_sweeps)
     # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
     from alf_amd.runtime import mfma_rate

@@ -82,12 +82,13 @@ def test_same_shape_requests_share_a_lane():
 
 
 def test_late_request_joins_the_running_lane_at_a_free_slot():
-    # a lane of two slots: request 1 finishes after 2 sweeps, request 2 (arriving meanwhile)
-    # takes its slot at the next sweep boundary while request 0 keeps sweeping
+    # a lane of two slots and no memory for a second lane: request 1 finishes after 2 sweeps,
+    # request 2 (arriving meanwhile) takes its slot at the next sweep boundary while request 0
+    # keeps sweeping
     a = Request(0, 4, 1.0, 4.0, seed=0, nsweeps=12, arrival=0.0)
     b = Request(1, 4, 1.0, 3.0, seed=1, nsweeps=2, arrival=0.0)
     c = Request(2, 4, 1.0, 2.0, seed=2, nsweeps=3, arrival=0.01)
-    srv, be, res = run([a, b, c], budget=1000)
+    srv, be, res = run([a, b, c], budget=20)
     assert len(be.lanes) == 1 and srv.joined == 1
     lane = be.lanes[0]
     assert lane.loads and lane.loads[0][1:] == (1, 2)       # slot 1, request 2
@@ -95,6 +96,15 @@ def test_late_request_joins_the_running_lane_at_a_free_slot():
     ra, rb, rc = res
     assert int(rc.fields[0, 0]) == 2
     assert rc.admitted >= rb.done - 1e-3 and rc.done < ra.done
+
+
+def test_late_request_opens_a_second_lane_when_its_lane_is_full():
+    a = Request(0, 4, 1.0, 4.0, seed=0, nsweeps=12, arrival=0.0)
+    b = Request(1, 4, 1.0, 3.0, seed=1, nsweeps=12, arrival=0.0)
+    c = Request(2, 4, 1.0, 2.0, seed=2, nsweeps=3, arrival=0.01)
+    srv, be, res = run([a, b, c], budget=1000)
+    assert len(be.lanes) == 2 and srv.joined == 0 and srv.peak_batches == 2
+    assert res[2].done < res[0].done                 # c did not wait for a slot of the first lane
 
 
 def test_memory_budget_serialises_lanes_in_arrival_order():
