@@ -173,6 +173,7 @@ struct alfq_ctx {
   char* oz_buf = nullptr;
   long long oz_cs = 0;
   OzOperand oz_e_row, oz_e_col, oz_ei_row, oz_ei_col, oz_x;
+  bool oz_shared = false;                // one copy of the hopping digits for the whole batch
   bool symm = false;                     // Symm = .T.: measurements on Hop_mod_Symm(G)
   // equal-time measurements
   ObsDev obs;
@@ -517,6 +518,50 @@ int hop_ready(const alfq_ctx* c) {
   return ST_NONE;
 }
 
+// flag |= 1 where a chain's words differ from chain 0's (grid y = chains 1 .. B - 1)
+__global__ void chains_differ_kernel(const unsigned long long* __restrict__ x, long long words, long long cs_words,
+                                     int* flag) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long z = (long long)blockIdx.y + 1;
+  if (i < words && x[z * cs_words + i] != x[i]) atomicOr(flag, 1);
+}
+
+// The digit planes of exp(-dtau T) and exp(dtau T).  When every chain of the batch holds the
+// same two matrices (one lattice, one dtau: the U and beta of the chains may differ), only
+// chain 0's are split and the products read them with chain stride 0, so that the four
+// operands (4 x 8 n^2 bytes for real T) stay in each XCD's L2 for the whole batch.
+int oz_split_hopping(alfq_ctx* c) {
+  const int n = c->n;
+  const size_t tbytes = (size_t)n * n * (c->t_real ? sizeof(double) : sizeof(cplx));
+  bool shared = c->B > 1 && c->cs % 8 == 0;
+  if (shared) {
+    int* flag = c->stat_dev;            // a word of the status buffer, read back at once
+    HIP_OK(hipMemsetAsync(flag, 0, sizeof(int), c->s));
+    const long long words = (long long)(tbytes / 8);
+    const dim3 grid((unsigned)((words + 255) / 256), (unsigned)(c->B - 1));
+    for (const void* m : {c->expT, c->expTinv}) {
+      hipLaunchKernelGGL(chains_differ_kernel, grid, dim3(256), 0, c->s, (const unsigned long long*)m, words,
+                         c->cs / 8, flag);
+      HIP_OK(hipGetLastError());
+    }
+    int differ = 1;
+    HIP_OK(hipMemcpyAsync(&differ, flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIP_OK(hipStreamSynchronize(c->s));
+    shared = differ == 0;
+  }
+  const int nz = shared ? 1 : c->B;
+  OzOperand* ops[4] = {&c->oz_e_row, &c->oz_e_col, &c->oz_ei_row, &c->oz_ei_col};
+  for (OzOperand* o : ops) o->stride = shared ? 0 : c->oz_cs;
+  c->oz_shared = shared;
+  int st;
+  if ((st = oz_split(c->expT, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_e_row, nz, c->s)) ||
+      (st = oz_split(c->expT, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_e_col, nz, c->s)) ||
+      (st = oz_split(c->expTinv, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_ei_row, nz, c->s)) ||
+      (st = oz_split(c->expTinv, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_ei_col, nz, c->s)))
+    return st;
+  return ST_NONE;
+}
+
 // ALFQ_PREC_INT8_SPLIT: one digit-plane buffer per chain (oz_cs bytes) holding exp(-dtau T)
 // and exp(dtau T) split by row (left operands) and by column (right operands), split once
 // here since a chain's hopping is fixed, and the planes of the matrix being wrapped.
@@ -550,10 +595,7 @@ int oz_setup(alfq_ctx* c) {
     ops[q]->stride = c->oz_cs;
   }
   int st;
-  if ((st = oz_split(c->expT, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_e_row, c->B, c->s)) ||
-      (st = oz_split(c->expT, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_e_col, c->B, c->s)) ||
-      (st = oz_split(c->expTinv, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_ei_row, c->B, c->s)) ||
-      (st = oz_split(c->expTinv, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_ei_col, c->B, c->s))) {
+  if ((st = oz_split_hopping(c))) {
     hipStreamSynchronize(c->s);
     hipFree(c->oz_buf);
     c->oz_buf = nullptr;
@@ -1470,13 +1512,9 @@ int alfq_chain_load(alfq_ctx* dst, int32_t dst_chain, alfq_ctx* src, int32_t src
   for (size_t k = 0; k < dst->var_st.size() && k < src->var_st.size(); ++k)
     if ((st = move_var(dst->var_st[k], src->var_st[k]))) return st;
   if (dst->oz_buf) {
-    // the destination's digit planes of exp(-dtau T) and exp(dtau T), re-split for every chain
-    const int n = dst->n;
-    if ((st = oz_split(dst->expT, !dst->t_real, n, n, n, true, nullptr, dst->cs, dst->oz_e_row, dst->B, dst->s)) ||
-        (st = oz_split(dst->expT, !dst->t_real, n, n, n, false, nullptr, dst->cs, dst->oz_e_col, dst->B, dst->s)) ||
-        (st = oz_split(dst->expTinv, !dst->t_real, n, n, n, true, nullptr, dst->cs, dst->oz_ei_row, dst->B, dst->s)) ||
-        (st = oz_split(dst->expTinv, !dst->t_real, n, n, n, false, nullptr, dst->cs, dst->oz_ei_col, dst->B, dst->s)))
-      return st;
+    // the destination's digit planes of exp(-dtau T) and exp(dtau T), split again (shared by
+    // the batch's chains only if the loaded chain has the same hopping)
+    if ((st = oz_split_hopping(dst))) return st;
   }
   return ST_NONE;
 }

@@ -1,8 +1,9 @@
 // Error-free split products on the int8 matrix cores (ozaki.hip).
 //
 // A double matrix is written as 2^e_i (sum_{s=1..S} 2^(-7 s) D_s) with one exponent per
-// row (or column) and S int8 digit matrices D_s in [-127, 127] (truncated base-128 digits of
-// the row scaled into (-1, 1)); the remainder is below 2^(-7 S) of the row's largest entry.
+// row (or column) and S int8 digit matrices D_s in [-127, 127] (base-128 digits of the row
+// scaled into (-1, 1), each rounded to nearest and saturated at +-127); the remainder is
+// below 2^(-7 S) of the row's largest entry.
 // A product is the sum over digit pairs with s + t <= S + 1: each pair is an exact integer
 // product (v_mfma_i32_32x32x32_i8, exact int32 sums), the pairs of one s + t are added in
 // int32 and the S sums are combined in FP64 with their power-of-two weights.
@@ -11,9 +12,12 @@
 
 constexpr int OZ_S = 8;              // digits per operand: 56 bits of each row's range
 
-// Digit planes of one operand: planes x OZ_S x rows x kp bytes (row r of digit s of plane p
-// at ((p OZ_S + s) rows + r) kp; k contiguous, zero beyond the contraction length) and one
-// exponent per row.  Plane 0 is the real part, 1 the imaginary part, 2 its negation.
+// Digit planes of one operand: planes x rows x kp x OZ_S bytes and one exponent per row.  Row
+// r of plane p starts at (p rows + r) kp OZ_S and holds its k-blocks of 32 entries in order,
+// each block as OZ_S segments of 32 bytes (digit s of entries 32 b .. 32 b + 31 at
+// (32 b) OZ_S + 32 s); zero beyond the contraction length.  A product stage of 32 entries of
+// one row is then OZ_S x 32 contiguous bytes.  Plane 0 is the real part, 1 the imaginary
+// part, 2 its negation.
 struct OzOperand {
   int8_t* d = nullptr;
   int* e = nullptr;

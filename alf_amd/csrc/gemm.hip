@@ -7,7 +7,8 @@
 // does not depend on scheduling).  Each wave holds a 16 x 16 sub-tile as two
 // (re, im) v_mfma_f64_16x16x4f64 accumulators.  K advances in stages of 16 KS
 // through LDS, where the operands are split into real and imaginary planes so each
-// MFMA consumes one plane; a complex product costs 4 MFMAs per k-step of 4, a
+// MFMA consumes one plane; a complex product costs 4 MFMAs per k-step of 4 (3 in the
+// 64 x 64 batched kernel, which forms (Ar + Ai)(Br + Bi) from the staged planes), a
 // real x complex one 2.  The host picks the tile shape per launch: at Ndim = 256 a
 // 32 x 32 tiling leaves most of the 256 CUs idle, a 16 x 16 tiling with KS = 4 fills
 // them with a quarter of the per-wave K.
@@ -325,13 +326,17 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
   const int lr = lane & 15, lk = lane >> 4;
 
-  typename M::Acc acc_re[2][2], acc_im[2][2];
+  // complex x complex in three real products (3M): acc_re = Ar Br, acc_t2 = Ai Bi,
+  // acc_im = (Ar + Ai)(Br + Bi); re = acc_re - acc_t2, im = acc_im - acc_re - acc_t2
+  constexpr bool M3 = A_CPLX && B_CPLX;
+  typename M::Acc acc_re[2][2], acc_im[2][2], acc_t2[M3 ? 2 : 1][M3 ? 2 : 1];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       acc_re[a][b] = typename M::Acc{0, 0, 0, 0};
       acc_im[a][b] = typename M::Acc{0, 0, 0, 0};
+      if (M3) acc_t2[M3 ? a : 0][M3 ? b : 0] = typename M::Acc{0, 0, 0, 0};
     }
 
   Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
@@ -360,26 +365,47 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
 #pragma unroll
         for (int b = 0; b < 2; ++b) scale_b(sk, b_re[b], b_im[b]);
       }
+      if (M3) {
+        E as[2], bsum[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) as[a] = a_re[a] + a_im[a];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = M::mac(a_re[a], b_re[b], acc_re[a][b]);
-          if (B_CPLX) acc_im[a][b] = M::mac(a_re[a], b_im[b], acc_im[a][b]);
-        }
-      if (A_CPLX) {
+        for (int b = 0; b < 2; ++b) bsum[b] = b_re[b] + b_im[b];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          const E na = -a_im[a];
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            if (B_CPLX) acc_re[a][b] = M::mac(na, b_im[b], acc_re[a][b]);
-            acc_im[a][b] = M::mac(a_im[a], b_re[b], acc_im[a][b]);
+            acc_re[a][b] = M::mac(a_re[a], b_re[b], acc_re[a][b]);
+            acc_t2[M3 ? a : 0][M3 ? b : 0] = M::mac(a_im[a], b_im[b], acc_t2[M3 ? a : 0][M3 ? b : 0]);
+            acc_im[a][b] = M::mac(as[a], bsum[b], acc_im[a][b]);
           }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            acc_re[a][b] = M::mac(a_re[a], b_re[b], acc_re[a][b]);
+            if (B_CPLX) acc_im[a][b] = M::mac(a_re[a], b_im[b], acc_im[a][b]);
+          }
+        if (A_CPLX) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc_im[a][b] = M::mac(a_im[a], b_re[b], acc_im[a][b]);
         }
       }
     }
     __syncthreads();
+  }
+  if (M3) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const typename M::Acc t2 = acc_t2[M3 ? a : 0][M3 ? b : 0];
+        acc_im[a][b] = acc_im[a][b] - acc_re[a][b] - t2;
+        acc_re[a][b] = acc_re[a][b] - t2;
+      }
   }
 
   // register r of a lane holds row M::row(r, lane / 16) of its 16 x 16 block

@@ -2,20 +2,24 @@
 // FP64-class accuracy (ALFQ_PREC_INT8_SPLIT).
 //
 // Split (oz_split_kernel): a workgroup takes 16 rows (or columns) of one chain's operand,
-// finds each one's largest |re|, |im| over the contraction index, and writes OZ_S truncated
-// base-128 digits of every entry scaled by 2^-e into int8 planes whose contraction index is
-// contiguous (zero-padded to a multiple of 64), with the exponent e.  The digits represent
-// the entry exactly up to 2^(-7 OZ_S) of its row's range.
+// finds each one's largest |re|, |im| over the contraction index, and writes OZ_S rounded
+// base-128 digits of every entry scaled by 2^-e into int8 planes laid out by k-block of 32
+// (every digit of a block contiguous; zero-padded to a multiple of 64), with the exponent e.
+// The digits represent the entry exactly up to 2^(-7 OZ_S) of its row's range.  A source row
+// of at most 256 entries is read once.
 //
-// Product (oz_gemm_kernel): a workgroup of 8 waves owns a 64 x 64 complex output tile;
-// waves 0-3 the real plane, 4-7 the imaginary one, each a 32 x 32 block.  K advances in
-// stages of 32 through LDS holding every digit plane of both operands for the stage, the
-// next stage's planes loaded into registers while the current one is multiplied.  For each k-step of 32 a wave keeps the OZ_S digit fragments of
-// its B plane(s) in registers and, for each digit s of A, issues v_mfma_i32_32x32x32_i8 into the int32 accumulator of
-// s + t for every t with s + t <= OZ_S + 1: the sums of one s + t are exact.  At the end
-// the OZ_S sums are weighted by 2^(-7 (s + t)) and added in FP64 (smallest first), the two
-// planes meet in LDS, and every thread applies the exponents, alpha, the row and column
-// scales and beta to its outputs, written column-major and coalesced.
+// Product (oz_gemm_kernel): each wave owns a 32 x 32 block of one output plane (real or
+// imaginary); a real x complex product runs 4 waves on a 64 x 32 or 32 x 64 tile and two
+// workgroups per CU, so that one multiplies while the other waits for its next stage.  K
+// advances in stages of 32 through LDS holding every digit plane of both operands for the
+// stage (whole 256-byte row blocks from global memory, stored conflict-free), the next
+// stage loaded into registers while the current one is multiplied.  A wave keeps the OZ_S
+// digit fragments of its B plane(s) in registers and, for each digit s of A, issues
+// v_mfma_i32_32x32x32_i8 into the int32 accumulator of s + t for every t with
+// s + t <= OZ_S - 1 (0-based): the sums of one s + t are exact.  At the end the OZ_S sums
+// are weighted by 2^(-7 (s + t + 2)) and added in FP64 (smallest first), the two planes
+// meet in LDS, and every thread applies the exponents, alpha, the row and column scales and
+// beta to its outputs, written column-major and coalesced.
 #include "alfq_oz.h"
 
 namespace {
@@ -23,16 +27,33 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int OZ_T = 64;             // output tile (rows and columns)
-constexpr int OZ_NT = 512;           // threads per product workgroup
+// Output tile per variant: each wave owns one 32 x 32 block of one output plane.  The real x
+// complex products of the sweep take 4 waves and two workgroups per CU (one computes while the
+// other waits for its stage), with the complex operand on the tile's short side so that a
+// stage fits the registers that prefetch it; complex x complex keeps 64 x 64 and 8 waves.
+template <int V> struct OzTile;
+template <> struct OzTile<0> { static constexpr int TM = 64, TN = 32, NT = 256, WPC = 2; };
+template <> struct OzTile<1> { static constexpr int TM = 32, TN = 64, NT = 256, WPC = 2; };
+template <> struct OzTile<2> { static constexpr int TM = 64, TN = 64, NT = 512, WPC = 1; };
+template <> struct OzTile<3> { static constexpr int TM = 64, TN = 64, NT = 512, WPC = 1; };
 
-// ---- split ---------------------------------------------------------------------------
-template <bool BY_ROW, bool CPLX>
+// ---- split ----------------------------------------------------------------------------
+// Digit d of a scaled entry x (|x| < 1): d = clamp(rint(128 x), -127, 127), x <- 128 x - d.
+// Rounding to nearest leaves remainders in [-1/2, 1/2] (in [1/2, 1) only after a saturated
+// digit: an entry within 2^-8 of 2^e saturates the first digit, one within about 2^-15 the
+// second too), so the digit sums are unbiased, and after OZ_S digits |x| < 1 as with
+// truncation.
+__device__ __forceinline__ int pack4(int a, int b, int c, int d) {
+  return (a & 255) | ((b & 255) << 8) | ((c & 255) << 16) | (d << 24);
+}
+
+template <bool BY_ROW, bool CPLX, bool NEG>
 __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ X, int ld, int nidx, int nk,
                                                        const cplx* __restrict__ kscale, long long xs,
                                                        int8_t* __restrict__ out, int* __restrict__ ex, int rows,
-                                                       int planes, long long kp, long long os) {
+                                                       long long kp, long long os) {
   constexpr int KL = 16;                   // contraction entries per lane and chunk: one 16-byte store
+  constexpr int CH = 16 * KL;              // contraction entries per chunk of the 16 lanes of an index
   const long long z = blockIdx.z;
   X = (const char*)X + z * xs;
   if (kscale) kscale = (const cplx*)((const char*)kscale + z * xs);
@@ -46,29 +67,33 @@ __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ 
   const bool live = idx < nidx;
   __shared__ double red[16][17];
 
-  auto load = [&](int k, double& re, double& im) {
-    re = 0.0;
-    im = 0.0;
-    if (!live || k >= nk) return;
-    const long long off = BY_ROW ? (idx + (long long)k * ld) : (k + (long long)idx * ld);
-    if (CPLX) {
-      cplx v = ((const cplx*)X)[off];
-      if (kscale) v = cmul(v, kscale[k]);
-      re = v.x;
-      im = v.y;
-    } else {
-      re = ((const double*)X)[off];
-    }
-  };
-
-  double m = 0.0;
-  for (int k0 = 0; k0 < nk; k0 += 16 * KL) {
+  double vr[KL], vi[KL];
+  auto load_chunk = [&](long long k0) {
 #pragma unroll
     for (int u = 0; u < KL; ++u) {
-      double re, im;
-      load(k0 + KL * kl + u, re, im);
-      m = fmax(m, fmax(fabs(re), fabs(im)));
+      const long long k = k0 + KL * kl + u;
+      vr[u] = 0.0;
+      vi[u] = 0.0;
+      if (!live || k >= nk) continue;
+      const long long off = BY_ROW ? (idx + k * ld) : (k + (long long)idx * ld);
+      if (CPLX) {
+        cplx v = ((const cplx*)X)[off];
+        if (kscale) v = cmul(v, kscale[k]);
+        vr[u] = v.x;
+        vi[u] = v.y;
+      } else {
+        vr[u] = ((const double*)X)[off];
+      }
     }
+  };
+  // a contraction length of one chunk is read once: its entries stay in registers between the
+  // range and the digits
+  const bool one = kp <= CH;
+  double m = 0.0;
+  for (long long k0 = 0; k0 < nk; k0 += CH) {
+    load_chunk(k0);
+#pragma unroll
+    for (int u = 0; u < KL; ++u) m = fmax(m, fmax(fabs(vr[u]), fabs(vi[u])));
   }
   red[il][kl] = m;
   __syncthreads();
@@ -84,33 +109,39 @@ __global__ __launch_bounds__(256) void oz_split_kernel(const void* __restrict__ 
   if (live && kl == 0) ex[idx] = e;
   if (!live) return;
 
-  for (long long k0 = 0; k0 < kp; k0 += 16 * KL) {
-    const int k = (int)k0 + KL * kl;
+  for (long long k0 = 0; k0 < kp; k0 += CH) {
+    const long long k = k0 + KL * kl;
     if (k >= kp) break;                    // kp is a multiple of 64 and of KL
-    double vr[KL], vi[KL];
-#pragma unroll
-    for (int u = 0; u < KL; ++u) {
-      load(k + u, vr[u], vi[u]);
-      vr[u] = ldexp(vr[u], -e);
-      vi[u] = ldexp(vi[u], -e);
-    }
+    if (!one) load_chunk(k0);
+    // digit s of entries k .. k + 15 of this row: bytes (k % 32) .. of the digit's 32-byte
+    // segment in k-block k / 32
+    const long long kb = (k >> 5) * (32 * OZ_S) + (k & 31);
 #pragma unroll
     for (int p = 0; p < (CPLX ? 2 : 1); ++p) {
       double* r = p ? vi : vr;
+      int8_t* o = out + (long long)(p * rows + idx) * kp * OZ_S + kb;
+      int8_t* on = out + (long long)(2 * rows + idx) * kp * OZ_S + kb;
+#pragma unroll
+      for (int u = 0; u < KL; ++u) r[u] = ldexp(r[u], -e);
 #pragma unroll
       for (int s = 0; s < OZ_S; ++s) {
-        v4i w = {0, 0, 0, 0}, wn = {0, 0, 0, 0};
+        int di[KL];
 #pragma unroll
         for (int u = 0; u < KL; ++u) {
-          r[u] *= 128.0;
-          const double d = trunc(r[u]);
-          r[u] -= d;
-          const int di = (int)d;            // |di| <= 127
-          w[u >> 2] |= (int)((unsigned)(di & 0xff) << (8 * (u & 3)));
-          wn[u >> 2] |= (int)((unsigned)((-di) & 0xff) << (8 * (u & 3)));
+          const double tv = r[u] * 128.0;
+          const double d = fmin(fmax(rint(tv), -127.0), 127.0);
+          r[u] = tv - d;
+          di[u] = (int)d;
         }
-        *(v4i*)(out + ((long long)(p * OZ_S + s) * rows + idx) * kp + k) = w;
-        if (p == 1 && planes == 3) *(v4i*)(out + ((long long)(2 * OZ_S + s) * rows + idx) * kp + k) = wn;
+        v4i w;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = pack4(di[4 * q], di[4 * q + 1], di[4 * q + 2], di[4 * q + 3]);
+        *(v4i*)(o + 32 * s) = w;
+        if (NEG && p == 1) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) w[q] = pack4(-di[4 * q], -di[4 * q + 1], -di[4 * q + 2], -di[4 * q + 3]);
+          *(v4i*)(on + 32 * s) = w;
+        }
       }
     }
   }
@@ -142,27 +173,29 @@ struct OzGemmArgs {
   cplx* C; int ldc; cplx alpha, beta; const cplx* rs; const cplx* cs; long long cst;
 };
 
-// byte offset of 16-byte segment sg of LDS row `row` (rows of KC bytes; the segment index
-// is swizzled with the row so that 32 lanes reading one segment of 32 rows spread over banks)
-template <int KC>
-__device__ __forceinline__ int lds_off(int tile, int row, int sg) {
-  return (tile * OZ_T + row) * KC + 16 * (sg ^ ((row >> 2) & (KC / 16 - 1)));
-}
+// LDS holds a stage as the global layout does: one 256-byte row (OZ_S digits x 32 bytes) per
+// operand row and plane, A's rows first.  16-byte segment j of row r sits at slot
+// j ^ (r % 16), so the 8 lanes of one LDS cycle hit 8 different slots both when a stage is
+// stored (consecutive lanes: consecutive segments of one row) and when a fragment is read
+// (consecutive lanes: one segment of consecutive rows).
+constexpr int OZ_ROW = OZ_S * 32;
+__device__ __forceinline__ int lds_seg(int row, int j) { return row * OZ_ROW + 16 * (j ^ (row & 15)); }
 
-template <int V, int Q, int KC>
-__device__ __forceinline__ void oz_kstep(const int8_t* lds, int bi, int bj, int lane, int kk, v16i (&acc)[OZ_S]) {
+template <int V, int Q>
+__device__ __forceinline__ void oz_kstep(const int8_t* lds, int bi, int bj, int lane, v16i (&acc)[OZ_S]) {
   typedef OzVar<V> T;
-  const int r = lane & 31, sg = 2 * kk + (lane >> 5);
+  typedef OzTile<V> W;
+  const int r = lane & 31, h = lane >> 5;
 #pragma unroll
   for (int p = 0; p < T::NP; ++p) {
     const int ap = pair_a<V>(Q, p), bp = pair_b<V>(Q, p);
+    const int arow = ap * W::TM + 32 * bi + r, brow = T::PA * W::TM + bp * W::TN + 32 * bj + r;
     v4i bf[OZ_S];
 #pragma unroll
-    for (int t = 0; t < OZ_S; ++t)
-      bf[t] = *(const v4i*)(lds + lds_off<KC>(T::PA * OZ_S + bp * OZ_S + t, 32 * bj + r, sg));
+    for (int t = 0; t < OZ_S; ++t) bf[t] = *(const v4i*)(lds + lds_seg(brow, 2 * t + h));
 #pragma unroll
     for (int s = 0; s < OZ_S; ++s) {
-      const v4i af = *(const v4i*)(lds + lds_off<KC>(ap * OZ_S + s, 32 * bi + r, sg));
+      const v4i af = *(const v4i*)(lds + lds_seg(arow, 2 * s + h));
 #pragma unroll
       for (int t = 0; t + s < OZ_S; ++t) acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[t], acc[s + t], 0, 0, 0);
     }
@@ -170,15 +203,15 @@ __device__ __forceinline__ void oz_kstep(const int8_t* lds, int bi, int bj, int 
 }
 
 template <int V>
-__global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
+__global__ __launch_bounds__(OzTile<V>::NT, OzTile<V>::WPC) void oz_gemm_kernel(OzGemmArgs g) {
   typedef OzVar<V> T;
-  constexpr int NTILE = (T::PA + T::PB) * OZ_S;
-  // contraction bytes per LDS stage: 32 (one k-step), so that the next stage's segments fit in
-  // registers beside the accumulators
+  typedef OzTile<V> W;
+  constexpr int OZ_TM = W::TM, OZ_TN = W::TN, OZ_NT = W::NT;
+  // contraction entries per LDS stage: 32 (one k-step), so that the next stage's segments fit
+  // in registers beside the accumulators
   constexpr int KC = 32;
-  constexpr int SEGS = KC / 16;
-  constexpr int STAGE = NTILE * OZ_T * KC;
-  constexpr int RES = 2 * OZ_T * (OZ_T + 1) * (int)sizeof(double);
+  constexpr int STAGE = (T::PA * OZ_TM + T::PB * OZ_TN) * OZ_ROW;
+  constexpr int RES = 2 * OZ_TM * (OZ_TN + 1) * (int)sizeof(double);
   __shared__ __align__(16) int8_t lds[STAGE > RES ? STAGE : RES];
 
   int bx, by, bzi;
@@ -193,8 +226,10 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
   const cplx* cs = g.cs ? (const cplx*)((const char*)g.cs + z * g.cst) : nullptr;
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int q = wave >> 2, bi = wave & 1, bj = (wave >> 1) & 1;
-  const int i0 = bx * OZ_T, j0 = by * OZ_T;
+  constexpr int NBI = OZ_TM / 32, NBJ = OZ_TN / 32;
+  static_assert(2 * NBI * NBJ * 64 == OZ_NT, "one wave per 32 x 32 block of each plane");
+  const int q = wave / (NBI * NBJ), bi = wave % NBI, bj = (wave / NBI) % NBJ;
+  const int i0 = bx * OZ_TM, j0 = by * OZ_TN;
 
   v16i acc[OZ_S];
 #pragma unroll
@@ -202,26 +237,25 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[s][r] = 0;
 
-  // a stage: every digit plane of A's 64 rows and B's 64 columns for k0 .. k0 + KC - 1, NPRE
-  // 16-byte segments per thread; the next stage's segments are loaded into registers while
-  // the current one is multiplied, and stored to LDS after the barrier that ends it
-  constexpr int NSEG = NTILE * OZ_T * SEGS, NPRE = (NSEG + OZ_NT - 1) / OZ_NT;
+  // a stage of one row of one plane is OZ_ROW contiguous bytes (every digit of its k-block);
+  // LDS row lr = (plane, row) of A, then of B; thread segment seg = 16 lr + j, so
+  // consecutive threads read consecutive 16-byte segments, whole cache lines
+  constexpr int RSEG = OZ_ROW / 16;
+  constexpr int NAR = T::PA * OZ_TM, NR = NAR + T::PB * OZ_TN;
+  constexpr int NSEG = NR * RSEG, NPRE = (NSEG + OZ_NT - 1) / OZ_NT;
   v4i pre[NPRE];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int p = 0; p < NPRE; ++p) {
       const int seg = tid + p * OZ_NT;
-      const int tile = seg / (OZ_T * SEGS), rem = seg % (OZ_T * SEGS), row = rem / SEGS, sg = rem % SEGS;
+      const int lr = seg / RSEG, j = seg % RSEG;
       v4i v = {0, 0, 0, 0};
-      if (seg < NSEG) {
-        if (tile < T::PA * OZ_S) {
-          const int gi = i0 + row;
-          if (gi < g.M) v = *(const v4i*)(A + ((long long)tile * g.arows + gi) * g.akp + k0 + 16 * sg);
-        } else {
-          const int gj = j0 + row;
-          if (gj < g.N)
-            v = *(const v4i*)(B + ((long long)(tile - T::PA * OZ_S) * g.brows + gj) * g.bkp + k0 + 16 * sg);
-        }
+      if (lr < NAR) {
+        const int pl = lr / OZ_TM, gi = i0 + lr % OZ_TM;
+        if (gi < g.M) v = *(const v4i*)(A + ((long long)pl * g.arows + gi) * g.akp * OZ_S + (long long)k0 * OZ_S + 16 * j);
+      } else if (lr < NR) {
+        const int pl = (lr - NAR) / OZ_TN, gj = j0 + (lr - NAR) % OZ_TN;
+        if (gj < g.N) v = *(const v4i*)(B + ((long long)pl * g.brows + gj) * g.bkp * OZ_S + (long long)k0 * OZ_S + 16 * j);
       }
       pre[p] = v;
     }
@@ -230,8 +264,7 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
 #pragma unroll
     for (int p = 0; p < NPRE; ++p) {
       const int seg = tid + p * OZ_NT;
-      const int tile = seg / (OZ_T * SEGS), rem = seg % (OZ_T * SEGS), row = rem / SEGS, sg = rem % SEGS;
-      if (seg < NSEG) *(v4i*)(lds + lds_off<KC>(tile, row, sg)) = pre[p];
+      if (seg < NSEG) *(v4i*)(lds + lds_seg(seg / RSEG, seg % RSEG)) = pre[p];
     }
   };
   if (g.K > 0) gload(0);
@@ -239,11 +272,8 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
     lstore();
     __syncthreads();
     if (k0 + KC < g.K) gload(k0 + KC);
-#pragma unroll
-    for (int kk = 0; kk < KC / 32; ++kk) {
-      if (q == 0) oz_kstep<V, 0, KC>(lds, bi, bj, lane, kk, acc);
-      else oz_kstep<V, 1, KC>(lds, bi, bj, lane, kk, acc);
-    }
+    if (q == 0) oz_kstep<V, 0>(lds, bi, bj, lane, acc);
+    else oz_kstep<V, 1>(lds, bi, bj, lane, acc);
     __syncthreads();
   }
 
@@ -256,15 +286,15 @@ __global__ __launch_bounds__(OZ_NT, 1) void oz_gemm_kernel(OzGemmArgs g) {
 #pragma unroll
     for (int gq = OZ_S - 1; gq >= 0; --gq) v += ldexp((double)acc[gq][r], -7 * (gq + 2));
     const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), j = lane & 31;
-    res[(q * OZ_T + 32 * bi + i) * (OZ_T + 1) + 32 * bj + j] = v;
+    res[(q * OZ_TM + 32 * bi + i) * (OZ_TN + 1) + 32 * bj + j] = v;
   }
   __syncthreads();
-  for (int o = tid; o < OZ_T * OZ_T; o += OZ_NT) {
-    const int i = o & (OZ_T - 1), j = o >> 6;
+  for (int o = tid; o < OZ_TM * OZ_TN; o += OZ_NT) {
+    const int i = o & (OZ_TM - 1), j = o / OZ_TM;
     const int gi = i0 + i, gj = j0 + j;
     if (gi >= g.M || gj >= g.N) continue;
     const int e = ea[gi] + eb[gj];
-    cplx v = make_double2(ldexp(res[i * (OZ_T + 1) + j], e), ldexp(res[(OZ_T + i) * (OZ_T + 1) + j], e));
+    cplx v = make_double2(ldexp(res[i * (OZ_TN + 1) + j], e), ldexp(res[(OZ_TM + i) * (OZ_TN + 1) + j], e));
     v = cmul(g.alpha, v);
     if (rs) v = cmul(rs[gi], v);
     if (cs) v = cmul(v, cs[gj]);
@@ -284,16 +314,19 @@ int oz_split(const void* X, bool x_cplx, int ld, int nidx, int nk, bool by_row, 
     return ST_GENERIC;
   }
   const dim3 grid((unsigned)((nidx + 15) / 16), 1, (unsigned)nchain);
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, X, ld, nidx, nk, kscale, xs, out.d, out.e, out.rows, out.kp,
+                       out.stride);
+  };
+  const bool neg = out.planes == 3;
   if (by_row) {
-    if (x_cplx) hipLaunchKernelGGL((oz_split_kernel<true, true>), grid, dim3(256), 0, s, X, ld, nidx, nk, kscale, xs,
-                                   out.d, out.e, out.rows, out.planes, out.kp, out.stride);
-    else hipLaunchKernelGGL((oz_split_kernel<true, false>), grid, dim3(256), 0, s, X, ld, nidx, nk, kscale, xs,
-                            out.d, out.e, out.rows, out.planes, out.kp, out.stride);
+    if (!x_cplx) launch(oz_split_kernel<true, false, false>);
+    else if (neg) launch(oz_split_kernel<true, true, true>);
+    else launch(oz_split_kernel<true, true, false>);
   } else {
-    if (x_cplx) hipLaunchKernelGGL((oz_split_kernel<false, true>), grid, dim3(256), 0, s, X, ld, nidx, nk, kscale, xs,
-                                   out.d, out.e, out.rows, out.planes, out.kp, out.stride);
-    else hipLaunchKernelGGL((oz_split_kernel<false, false>), grid, dim3(256), 0, s, X, ld, nidx, nk, kscale, xs,
-                            out.d, out.e, out.rows, out.planes, out.kp, out.stride);
+    if (!x_cplx) launch(oz_split_kernel<false, false, false>);
+    else if (neg) launch(oz_split_kernel<false, true, true>);
+    else launch(oz_split_kernel<false, true, false>);
   }
   HIP_OK(hipGetLastError());
   return ST_NONE;
@@ -308,11 +341,14 @@ int oz_gemm(int M, int N, int K, const OzOperand& A, const OzOperand& B, cplx* C
   }
   OzGemmArgs g{M, N, K, A.d, A.e, A.kp, A.rows, A.stride, B.d, B.e, B.kp, B.rows, B.stride,
                C, ldc, alpha, beta, rs, cs, cstride};
-  const dim3 grid((unsigned)((M + OZ_T - 1) / OZ_T), (unsigned)((N + OZ_T - 1) / OZ_T), (unsigned)nchain);
-  if (A.planes == 1 && B.planes == 2) hipLaunchKernelGGL(oz_gemm_kernel<0>, grid, dim3(OZ_NT), 0, s, g);
-  else if (A.planes == 2 && B.planes == 1) hipLaunchKernelGGL(oz_gemm_kernel<1>, grid, dim3(OZ_NT), 0, s, g);
-  else if (A.planes == 3 && B.planes == 2) hipLaunchKernelGGL(oz_gemm_kernel<2>, grid, dim3(OZ_NT), 0, s, g);
-  else if (A.planes == 2 && B.planes == 3) hipLaunchKernelGGL(oz_gemm_kernel<3>, grid, dim3(OZ_NT), 0, s, g);
+  auto launch = [&](auto kern, int tm, int tn, int nt) {
+    const dim3 grid((unsigned)((M + tm - 1) / tm), (unsigned)((N + tn - 1) / tn), (unsigned)nchain);
+    hipLaunchKernelGGL(kern, grid, dim3(nt), 0, s, g);
+  };
+  if (A.planes == 1 && B.planes == 2) launch(oz_gemm_kernel<0>, OzTile<0>::TM, OzTile<0>::TN, OzTile<0>::NT);
+  else if (A.planes == 2 && B.planes == 1) launch(oz_gemm_kernel<1>, OzTile<1>::TM, OzTile<1>::TN, OzTile<1>::NT);
+  else if (A.planes == 3 && B.planes == 2) launch(oz_gemm_kernel<2>, OzTile<2>::TM, OzTile<2>::TN, OzTile<2>::NT);
+  else if (A.planes == 2 && B.planes == 3) launch(oz_gemm_kernel<3>, OzTile<3>::TM, OzTile<3>::TN, OzTile<3>::NT);
   else {
     alfq_set_error("oz_gemm: unsupported operand planes");
     return ST_GENERIC;

@@ -617,7 +617,8 @@ def sweep_many(engines, nsweep=1):
 
 
 def mfma_rate(kind):
-    """Measured TFLOP/s ceiling: 0 FP64 MFMA, 1 FP32 MFMA, 2 vector FP64 FMA (alfq_mfma_rate)."""
+    """Measured ceiling: 0 FP64 MFMA, 1 FP32 MFMA, 2 vector FP64 FMA (TFLOP/s), 3 int8 MFMA
+    (TOP/s) (alfq_mfma_rate)."""
     lib = load_library()
     v = C.c_double()
     check(lib, lib.alfq_mfma_rate(int(kind), C.byref(v)))

@@ -102,6 +102,9 @@ def cpu_oracle_rate(ham, fields, sweeps=3):
     return 1.0 / float(np.median(times))
 
 
+BATCH_THERMALISATION = 15
+
+
 def batched_rate(chains, steps, prec=0, checkerboard=False, params=None):
     """Aggregate sweeps/s of `chains` independent chains of the workload (or of the model
     `params` names) in one batched context."""
@@ -115,11 +118,15 @@ def batched_rate(chains, steps, prec=0, checkerboard=False, params=None):
         b.chain(z).set_fields(ham.nsigma.f.copy())
         b.chain(z).set_philox(seed=z, stream=0)
     b.init()
-    b.sweep(3)            # the first sweep runs eagerly, the next two capture both graph parities
+    # thermalisation from the random start: the first sweep runs eagerly and the next two
+    # capture both graph parities; the acceptance, and with it the number of delayed-update
+    # rounds per slice (set by the busiest slice of any chain), settles within ~15 sweeps
+    b.sweep(BATCH_THERMALISATION)
     ms = b.sweep_timed(steps)
     ctl = [b.chain(z).control() for z in range(chains)]
     b.close()
-    return {"chains": chains, "steps": steps, "ms_per_batched_sweep": ms / steps,
+    return {"chains": chains, "steps": steps, "thermalisation_sweeps": BATCH_THERMALISATION,
+            "ms_per_batched_sweep": ms / steps,
             "aggregate_sweeps_per_s": chains * steps / (ms / 1e3),
             "precision_green_max": max(c["XMAXG"] for c in ctl),
             "precision_green_mean": float(np.mean([c["precision_green_mean"] for c in ctl])),
@@ -399,7 +406,7 @@ def main():
     if not args.no_mixed:
         note("mixed precision forms")
         mixed = {"form": "hopping wraps of G (Hop_mod products in WRAPGRUP/WRAPGRDO) as int8 split products: each "
-                         "operand row/column = 2^e x 8 truncated base-128 int8 digits, digit pairs with s+t <= 9 "
+                         "operand row/column = 2^e x 8 base-128 int8 digits rounded to nearest, digit pairs with s+t <= 9 "
                          "multiplied exactly on v_mfma_i32_32x32x32_i8 (int32 sums), combined in FP64; UDV chain, "
                          "QR, CGR, Metropolis ratio and delayed updates FP64",
                  "single": single_rate(ham, fields, args.delay, args.steps, args.warmup, PREC_INT8_SPLIT),
@@ -444,11 +451,12 @@ def main():
     if args.c4_chains > 0:
         note("uneven mix")
         result["uneven_batch_mix"] = c4_mix_rate(args.c4_chains, args.c4_sweeps)
-    # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s)
+    # attainable issue rates on this device (published dense peaks: FP64 78.6, FP32 157.3 TF/s,
+    # int8 5000 TOP/s)
     from alf_amd.runtime import mfma_rate
     if not args.only_headline:
         result["device_ceilings_tflops"] = {"fp64_mfma": mfma_rate(0), "fp32_mfma": mfma_rate(1),
-                                            "fp64_vector_fma": mfma_rate(2)}
+                                            "fp64_vector_fma": mfma_rate(2), "int8_mfma_tops": mfma_rate(3)}
     if cpu is not None:
         result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)

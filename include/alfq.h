@@ -122,7 +122,7 @@ int alfq_set_gemm_policy(alfq_ctx* ctx, int32_t policy);
 #define ALFQ_PREC_FP32_WRAPS 1
 /* ALFQ_PREC_INT8_SPLIT (no counterpart in the reference): the same products as exact
  * integer sums on the int8 matrix cores.  Each operand row (left) or column (right) is
- * 2^e times 8 truncated base-128 int8 digits (56 bits of the row's range); the digit
+ * 2^e times 8 base-128 int8 digits rounded to nearest (56 bits of the row's range); the digit
  * pairs with s + t <= 9 are multiplied exactly (int32 sums) and combined in FP64.  The
  * product's error is about 2^-56 of |row| |column|, against 2^-24 for FP32 wraps.  The
  * first call allocates the digit planes and splits exp(-dtau T) and exp(dtau T) once. */
@@ -229,7 +229,8 @@ int alfq_kernel_timing_get(alfq_ctx* ctx, double* total_ms, int64_t* launches);
 int alfq_sweep_graphs(alfq_ctx* ctx, int32_t enable);
 /* Measured issue-rate ceiling (no counterpart in the reference; measurement only) of
  * kind 0: FP64 MFMA v_mfma_f64_16x16x4f64, 1: FP32 MFMA v_mfma_f32_16x16x4f32,
- * 2: vector FP64 FMA, in TFLOP/s over all CUs of the current device. */
+ * 2: vector FP64 FMA, in TFLOP/s, 3: int8 MFMA v_mfma_i32_32x32x32_i8, in TOP/s, over all
+ * CUs of the current device. */
 int alfq_mfma_rate(int32_t kind, double* tflops);
 /* ---- equal-time measurements (Hubbard plugin, N_FL = 1) ------------------- */
 /* Replaces ham%Obser (Prog/Hamiltonians/Hamiltonian_Hubbard_smod.F90:695-801) called by the

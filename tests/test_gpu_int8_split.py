@@ -2,7 +2,7 @@
 
 * The product itself: for real x complex, complex x real and complex x complex operands of
   ragged and full shapes, every entry of A B lies within 16 K 2^-56 max_k |A_ik| max_k |B_kj| of
-  the exact product (the truncation bound of 8 base-128 digits per operand), and the result
+  the exact product (the bound of 8 base-128 digits per operand), and the result
   is not the FP64 MFMA one (the integer path ran).
 * Sweeps: with the wraps of G on the split products, the chain takes exactly the FP64
   drop-in's decisions on the same random tape (fields and counters bit-exact), G after
@@ -124,3 +124,35 @@ def test_split_wraps_batch_meets_the_bar(rt):
           f"{m64.max():.2e}, split {ms.max():.2e}", flush=True)
     assert np.all(xs <= np.maximum(1e-8, 2.0 * x64))
     assert ms.max() <= 1e-8
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_split_batch_chains_match_single_chains(rt, mixed):
+    """Batched split products equal single-chain ones: a batch whose chains share one
+    exp(-dtau T) (its digit planes split once and read by every chain) and a batch of two
+    hoppings (dtau 0.1 and 0.05 at the same Ltrot: planes per chain).  Every chain takes the
+    decisions it takes alone, with G within 1e-10."""
+    ha = hubbard(L1=8, L2=8, beta=4.0, U=4.0, nwrap=10)
+    hb = hubbard(L1=8, L2=8, beta=2.0, U=6.0, nwrap=10, dtau=0.05) if mixed else ha
+    assert ha.Ltrot == hb.Ltrot
+    hams = [ha, ha, hb, hb]
+    b = rt.BatchEngine(hams, delay=32)
+    b.set_precision(rt.PREC_INT8_SPLIT)
+    for z, h in enumerate(hams):
+        b.chain(z).set_fields(fields_for(h, 40 + z))
+        b.chain(z).set_philox(seed=40 + z, stream=0)
+    b.init()
+    b.sweep(2)
+    for z, h in enumerate(hams):
+        e = rt.Engine(h, delay=32)
+        e.set_precision(rt.PREC_INT8_SPLIT)
+        e.set_fields(fields_for(h, 40 + z))
+        e.set_philox(seed=40 + z, stream=0)
+        e.init()
+        e.sweep(2)
+        c = b.chain(z)
+        assert np.array_equal(c.get_fields(), e.get_fields())
+        assert rel(c.get_green(), e.get_green()) < 1e-10
+        assert abs(c.get_phase() - e.get_phase()) < 1e-10
+        e.close()
+    b.close()

@@ -571,10 +571,11 @@ def test_fp32_wraps_sweep_precision(rt):
 
 def test_arithmetic_rate_probe(rt):
     """The measured issue-rate ceilings are positive and below the published dense peaks."""
-    f64, f32, v64 = rt.mfma_rate(0), rt.mfma_rate(1), rt.mfma_rate(2)
+    f64, f32, v64, i8 = rt.mfma_rate(0), rt.mfma_rate(1), rt.mfma_rate(2), rt.mfma_rate(3)
     assert 5.0 < f64 < 78.6 * 1.05
     assert 10.0 < f32 < 157.3 * 1.05
     assert 5.0 < v64 < 78.6 * 1.05
+    assert 100.0 < i8 < 5000.0 * 1.05
 
 
 @pytest.mark.parametrize("lattice,L,U,beta,nwrap,n_phi", [("Honeycomb", 12, 3.0, 1.2, 8, 18), ("Square", 24, 4.0, 1.0, 10, 0)])
