@@ -290,6 +290,7 @@ struct alfq_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   KTimer chain_timer;
   int rounds = 1 << 30;                  // chain rounds per slice (adapted per sweep)
+  bool round_fusion = true;              // one launch for a round's X product and flush (few chains)
   int* stat_dev = nullptr;               // 3 words per chain read after every sweep
   int* stat_host = nullptr;              // their pinned host copy
   bool uniforms_ready = false;           // a whole sweep's uniforms are already on the device
@@ -429,11 +430,21 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   gx.nchain = c->B;
   gx.policy = c->gemm_policy;
   a.yhat_external = reg_kernel && gemm_direct_path(gx) ? 1 : 0;
-  if (a.yhat_external) gx.yhat = c->Y;
+  // few chains: X product and flush in one launch from G into the other G buffer (Gtmp is
+  // free during the rounds), then the two swap
+  const bool fused = a.yhat_external && c->round_fusion;
+  if (a.yhat_external && !fused) gx.yhat = c->Y;
   for (int r = 0; r < rounds; ++r) {
     a.first_round = (r == 0);
     a.final_round = (r == rounds - 1);
+    a.G = c->G;
     if ((st = launch_chain(a, c->s, &c->chain_timer))) return st;
+    if (fused) {
+      if ((st = round_flush_launch(c->G, c->Gtmp, c->Wk, c->amapk, &c->st->m, c->n, c->K, c->B, c->cs, c->s)))
+        return st;
+      std::swap(c->G, c->Gtmp);
+      continue;
+    }
     if (reg_kernel) {
       if ((st = gemm_c(c, gx))) return st;
     }
@@ -1439,6 +1450,8 @@ int create_impl(const alfq_model_desc* descs, int nchain, alfq_ctx** out) {
   // ALFQ_SWEEP_GRAPHS=0 launches every kernel from the host (kernel tracers that do not
   // follow graph replays need this)
   if (const char* e = std::getenv("ALFQ_SWEEP_GRAPHS")) c->graphs_enabled = std::atoi(e) != 0;
+  // ALFQ_ROUND_FUSION=0 keeps a round's X product and flush as two launches for few chains
+  if (const char* e = std::getenv("ALFQ_ROUND_FUSION")) c->round_fusion = std::atoi(e) != 0;
   *out = c;
   return ST_NONE;
 }

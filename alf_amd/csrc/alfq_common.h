@@ -184,3 +184,8 @@ bool gemm_direct_path(const GemmArgs& g);
 // clamped to kmax <= 32, for the nchain chains of a batch (pointers advance by cs bytes).
 int flush_launch(cplx* G, const cplx* X, const cplx* Y, const int* mdev, int n, int kmax, int nchain,
                  long long cs, hipStream_t s);
+// the round's X product and flush in one launch for a few chains: Go = Gi + (Gi(:, A) W) Yhat^T
+// with Yhat(c, j) = delta(c, amap[j]) - Gi(amap[j], c), rank m = *mdev <= kmax <= 32; W is
+// m x m with leading dimension 32; Gi and Go are different buffers.
+int round_flush_launch(const cplx* Gi, cplx* Go, const cplx* W, const int* amap, const int* mdev, int n, int kmax,
+                       int nchain, long long cs, hipStream_t s);

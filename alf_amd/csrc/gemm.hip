@@ -748,6 +748,113 @@ void launch_direct(const GemmArgs& g, hipStream_t s) {
 }
 }  // namespace
 
+// The end of a delayed-update round for a few chains, in one launch:
+//   Go = Gi + (Gi(:, A) W) Yhat^T,   Yhat(c, j) = delta(c, P_j) - Gi(P_j, c),
+// A = (P_0 .. P_{m-1}) the round's accepted sites, W = C D^T (m x m, ld FL_K), m = *mdev.
+// Gi is only read and Go only written (two buffers), so a workgroup can gather the columns A
+// and rows A of Gi that it needs while others write their tiles.  A workgroup of two waves
+// owns a 16 x 16 tile: wave w forms columns 16w .. 16w + 15 of its rows of X = Gi(:, A) W,
+// then the waves split the rank-m product over k and add their halves through LDS.  Every
+// global load of the tile is issued before the first MFMA.
+constexpr int RF_T = 16;
+__global__ __launch_bounds__(128) void round_flush_kernel(const cplx* __restrict__ Gi, cplx* __restrict__ Go,
+                                                          const cplx* __restrict__ W, const int* __restrict__ amap,
+                                                          const int* mdev, int n, int kmax, long long cs) {
+  typedef Mma<false> M;
+  __shared__ cplx Xs[RF_T][FL_K + 1];
+  __shared__ cplx red[RF_T * RF_T];
+  const int z = blockIdx.z;
+  Gi = chain_ptr_at(Gi, cs, z);
+  Go = chain_ptr_at(Go, cs, z);
+  W = chain_ptr_at(W, cs, z);
+  amap = chain_ptr_at(amap, cs, z);
+  mdev = chain_ptr_at(mdev, cs, z);
+  int m = *mdev;
+  m = m < kmax ? m : kmax;
+  m = m > 0 ? m : 0;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int i0 = blockIdx.x * RF_T, j0 = blockIdx.y * RF_T;
+  const int gi = i0 + lr, gj = j0 + lr;
+  const bool row_ok = gi < n, col_ok = gj < n;
+
+  // X(gi, 16 wave + lr') = sum_k Gi(gi, P_k) W(k, 16 wave + lr'): A operand lane = row + 16 k,
+  // B operand lane = column + 16 k, 8 k-steps of 4
+  cplx xa[FL_K / 4], xb[FL_K / 4];
+#pragma unroll
+  for (int u = 0; u < FL_K / 4; ++u) {
+    const int k = 4 * u + lk;
+    const int P = k < m ? amap[k] : 0;
+    xa[u] = (k < m && row_ok) ? Gi[gi + (long long)P * n] : make_double2(0.0, 0.0);
+    xb[u] = k < m ? W[k + (long long)(16 * wave + lr) * FL_K] : make_double2(0.0, 0.0);
+  }
+  // this wave's half of the rank-m product: Yhat(gj, k) for k in [16 wave, 16 wave + 16)
+  cplx yb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = 16 * wave + 4 * u + lk;
+    yb[u] = make_double2(0.0, 0.0);
+    if (k < m && col_ok) {
+      const int P = amap[k];
+      const cplx v = Gi[P + (long long)gj * n];
+      yb[u] = make_double2((gj == P ? 1.0 : 0.0) - v.x, -v.y);
+    }
+  }
+  // the tile of Gi in the MFMA result layout (wave 0 adds it)
+  cplx gt[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ri = i0 + M::row(r, lk);
+    gt[r] = (wave == 0 && ri < n && col_ok) ? Gi[ri + (long long)gj * n] : make_double2(0.0, 0.0);
+  }
+
+  v4d xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < FL_K / 4; ++u) {
+    xr = M::mac(xa[u].x, xb[u].x, xr);
+    xi = M::mac(xa[u].x, xb[u].y, xi);
+    xr = M::mac(-xa[u].y, xb[u].y, xr);
+    xi = M::mac(xa[u].y, xb[u].x, xi);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Xs[M::row(r, lk)][16 * wave + lr] = make_double2(xr[r], xi[r]);
+  __syncthreads();
+
+  v4d gr = {0, 0, 0, 0}, gim = {0, 0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const cplx a = Xs[lr][16 * wave + 4 * u + lk];
+    gr = M::mac(a.x, yb[u].x, gr);
+    gim = M::mac(a.x, yb[u].y, gim);
+    gr = M::mac(-a.y, yb[u].y, gr);
+    gim = M::mac(a.y, yb[u].x, gim);
+  }
+  if (wave == 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[M::row(r, lk) + RF_T * lr] = make_double2(gr[r], gim[r]);
+  }
+  __syncthreads();
+  if (wave == 1 || !col_ok) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ri = i0 + M::row(r, lk);
+    if (ri >= n) continue;
+    const cplx h = red[M::row(r, lk) + RF_T * lr];
+    Go[ri + (long long)gj * n] = make_double2(gt[r].x + (gr[r] + h.x), gt[r].y + (gim[r] + h.y));
+  }
+}
+
+int round_flush_launch(const cplx* Gi, cplx* Go, const cplx* W, const int* amap, const int* mdev, int n, int kmax,
+                       int nchain, long long cs, hipStream_t s) {
+  if (kmax < 1 || kmax > FL_K) { alfq_set_error("round flush: rank outside [1, 32]"); return ST_GENERIC; }
+  if (nchain < 1 || nchain > 65535 || n < 1) { alfq_set_error("round flush: bad batch"); return ST_GENERIC; }
+  const unsigned t = (unsigned)cdivl(n, RF_T);
+  hipLaunchKernelGGL(round_flush_kernel, dim3(t, t, (unsigned)nchain), dim3(128), 0, s, Gi, Go, W, amap, mdev, n,
+                     kmax, cs);
+  HIP_OK(hipGetLastError());
+  return ST_NONE;
+}
+
 int flush_launch(cplx* G, const cplx* X, const cplx* Y, const int* mdev, int n, int kmax, int nchain,
                  long long cs, hipStream_t s) {
   if (kmax < 1 || kmax > FL_K) { alfq_set_error("flush: rank outside [1, 32]"); return ST_GENERIC; }
