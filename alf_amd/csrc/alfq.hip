@@ -298,13 +298,13 @@ struct alfq_ctx {
   struct HostState {
     cplx* G; cplx* Gtmp; double* u;
     int var_r, var_l, rounds, K, rng_mode, prec, gemm_policy;
-    bool obs_on, cb_on, tau_on;
+    bool obs_on, cb_on, tau_on, round_fusion;
     std::vector<int> var_st, refs;
     bool operator==(const HostState& o) const {
       return G == o.G && Gtmp == o.Gtmp && u == o.u && var_r == o.var_r && var_l == o.var_l && rounds == o.rounds &&
              K == o.K && rng_mode == o.rng_mode && prec == o.prec && gemm_policy == o.gemm_policy &&
              obs_on == o.obs_on && cb_on == o.cb_on &&
-             tau_on == o.tau_on &&
+             tau_on == o.tau_on && round_fusion == o.round_fusion &&
              var_st == o.var_st && refs == o.refs;
     }
   };
@@ -431,8 +431,9 @@ int metropolis_slice(alfq_ctx* c, int slice, int dir) {
   gx.policy = c->gemm_policy;
   a.yhat_external = reg_kernel && gemm_direct_path(gx) ? 1 : 0;
   // few chains: X product and flush in one launch from G into the other G buffer (Gtmp is
-  // free during the rounds), then the two swap
-  const bool fused = a.yhat_external && c->round_fusion;
+  // free during the rounds), then the two swap.  Up to Ndim 256, where it was measured faster:
+  // at Ndim 576 each 16 x 16 tile's gather of 32 rows of G costs more than the launch it saves
+  const bool fused = a.yhat_external && c->round_fusion && c->n <= 256;
   if (a.yhat_external && !fused) gx.yhat = c->Y;
   for (int r = 0; r < rounds; ++r) {
     a.first_round = (r == 0);
@@ -1043,6 +1044,7 @@ alfq_ctx::HostState host_state(const alfq_ctx* c) {
   h.obs_on = c->obs_on;
   h.cb_on = c->cb_on;
   h.tau_on = c->tau_on;
+  h.round_fusion = c->round_fusion;
   h.var_st = c->var_st;
   h.refs.reserve(c->pool.size());
   for (const auto& sl : c->pool) h.refs.push_back(sl.refs);
@@ -1585,6 +1587,12 @@ int alfq_set_gemm_policy(alfq_ctx* c, int32_t policy) {
   c->gemm_policy = policy;
   c->qr.policy = policy;
   c->tau.w2.policy = policy;
+  return ST_NONE;
+}
+
+int alfq_set_round_fusion(alfq_ctx* c, int32_t on) {
+  if (!c) return ST_GENERIC;
+  c->round_fusion = on != 0;
   return ST_NONE;
 }
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "alfq_init", "alfq_sweep", "alfq_sweep_timed", "alfq_sweep_many",
     "alfq_sync", "alfq_control_get", "alfq_control_reset", "alfq_precision_log",
     "alfq_last_error", "alfq_version", "alfq_gemm", "alfq_kernel_timing", "alfq_kernel_timing_get",
-    "alfq_sweep_graphs", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
+    "alfq_sweep_graphs", "alfq_set_round_fusion", "alfq_ctx_create_batch", "alfq_set_chain", "alfq_batch_size",
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
     "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc", "alfq_chain_load",
@@ -144,6 +144,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_precision_log": [P, dp, C.c_int64, C.POINTER(C.c_int64)],
         "alfq_kernel_timing": [P, C.c_int32],
         "alfq_sweep_graphs": [P, C.c_int32],
+        "alfq_set_round_fusion": [P, C.c_int32],
         "alfq_ctx_create_batch": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(P)],
         "alfq_set_chain": [P, C.c_int32],
         "alfq_chain_load": [P, C.c_int32, P, C.c_int32],
@@ -511,6 +512,11 @@ class Engine:
     def sweep_graphs(self, enable=True):
         """Replay sweeps as captured HIP graphs (default) or launch every kernel from the host."""
         self._c(self.lib.alfq_sweep_graphs(self.ctx, int(enable)))
+
+    def round_fusion(self, on=True):
+        """One launch per delayed-update round for contexts of fewer than 32 chains (default)
+        or an X-product and a flush launch (alfq_set_round_fusion)."""
+        self._c(self.lib.alfq_set_round_fusion(self.ctx, int(on)))
 
     def kernel_timing(self, enable=True):
         self._c(self.lib.alfq_kernel_timing(self.ctx, int(enable)))

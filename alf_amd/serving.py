@@ -131,6 +131,9 @@ class _HubbardBackend:
         fill = [reqs[min(z, len(reqs) - 1)] for z in range(nslot)]
         hams = [self.model(r) for r in fill]
         eng = self.rt.BatchEngine(hams, delay=self.delay)
+        # two launches per delayed-update round in the lanes: a stream of 500-sweep requests
+        # over nine concurrent lanes stalled twice with the one-launch rounds (cause not found)
+        eng.round_fusion(False)
         for z, r in enumerate(fill):
             with self._lock:
                 hams[z].nsigma.set_uniform_four_state(r.seed)
@@ -179,10 +182,14 @@ class _Lane:
             eng = be.lane(first, self.nslot)
             t_adm = srv.clock() - srv.t0
             slots = {z: [r, r.nsweeps, t_adm] for z, r in enumerate(first)}
+            swept = 0
             try:
                 while True:
                     if slots:
                         be.sweep(eng)
+                        swept += 1
+                        if srv.lane_note and srv.note_every and swept % srv.note_every == 0:
+                            srv.lane_note(self, swept, sorted(s[0].rid for s in slots.values()))
                     done = []
                     for z, s in list(slots.items()):
                         s[1] -= 1
@@ -233,12 +240,14 @@ class Server:
     """Continuous-batching loop over a list of timed requests (see the module docstring)."""
 
     def __init__(self, memory_budget: int | None = None, max_batch: int = 64, backend=None, clock=time.perf_counter,
-                 min_slots: int = 2, progress=None):
+                 min_slots: int = 2, progress=None, lane_note=None, note_every: int = 0):
         self.backend = backend if backend is not None else _HubbardBackend()
         self.budget = memory_budget if memory_budget is not None else int(0.8 * device_free_bytes())
         self.max_batch = max_batch
         self.min_slots = min_slots
         self.progress = progress           # called with each list of results as it arrives
+        self.lane_note = lane_note         # called by a lane's thread every note_every sweeps
+        self.note_every = note_every
         self.clock = clock
         self.peak_batches = 0              # lanes running at once
         self.lane_peak = 0                 # occupied slots of one lane at a sweep boundary
@@ -446,6 +455,7 @@ def main(argv=None) -> None:
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--min-slots", type=int, default=2)
+    ap.add_argument("--note-every", type=int, default=50, help="progress line per lane every this many sweeps")
     args = ap.parse_args(argv)
     reqs = poisson_requests(args.requests, args.rate, seed=args.seed, nsweeps=args.sweeps)
     import sys
@@ -453,7 +463,10 @@ def main(argv=None) -> None:
     def progress(done):
         for r in done:
             print(f"[serving] request {r.rid} done after {r.latency:.1f} s", file=sys.stderr, flush=True)
-    srv = Server(min_slots=args.min_slots, progress=progress)
+    def lane_note(lane, swept, rids):
+        print(f"[serving] {srv.clock() - srv.t0:7.1f} s  lane L={lane.key[1]} Ltrot={lane.key[2]}: sweep {swept}, "
+              f"requests {rids}", file=sys.stderr, flush=True)
+    srv = Server(min_slots=args.min_slots, progress=progress, lane_note=lane_note, note_every=args.note_every)
     res = srv.run(reqs)
     out = latency_summary(res, srv.wall)
     out.update({"sweeps_per_request": args.sweeps, "rate_per_s": args.rate, "seed": args.seed,

@@ -227,6 +227,12 @@ int alfq_kernel_timing_get(alfq_ctx* ctx, double* total_ms, int64_t* launches);
  * sweep into a HIP graph the first time its starting state is seen and replay it with
  * one launch afterwards; enable = 0 launches every kernel from the host. */
 int alfq_sweep_graphs(alfq_ctx* ctx, int32_t enable);
+/* One launch per delayed-update round (no counterpart in the reference): with on != 0 (the
+ * default) a context of fewer than 32 chains and Ndim <= 256 forms a round's columns and
+ * applies them to G in one kernel that reads G and writes the second G buffer; on = 0 uses
+ * an X-product and a flush launch.  The serving loop's lanes use 0: a stream of 500-sweep
+ * requests over nine concurrent lanes stalled twice with fused rounds, cause not found. */
+int alfq_set_round_fusion(alfq_ctx* ctx, int32_t on);
 /* Measured issue-rate ceiling (no counterpart in the reference; measurement only) of
  * kind 0: FP64 MFMA v_mfma_f64_16x16x4f64, 1: FP32 MFMA v_mfma_f32_16x16x4f32,
  * 2: vector FP64 FMA, in TFLOP/s, 3: int8 MFMA v_mfma_i32_32x32x32_i8, in TOP/s, over all

@@ -239,6 +239,32 @@ def test_graph_replayed_sweeps_match_eager_sweeps(rt):
         assert pe == pg
 
 
+def test_round_fusion_switch_keeps_the_trajectory(rt):
+    """One launch per delayed-update round (the default for few chains) and the X-product +
+    flush launches give the same decisions and G to rounding, for one chain and for a batch
+    of three, with graph replay; switching in the middle of a run keeps the chain going."""
+    ham = hubbard(L1=8, L2=8, beta=4.0, nwrap=10)
+    runs = []
+    for fused in (True, False, None):
+        b = rt.BatchEngine([ham] * 3, delay=32)
+        b.round_fusion(bool(fused))
+        for z in range(3):
+            b.chain(z).set_fields(fields_for(ham, 70 + z))
+            b.chain(z).set_philox(seed=70 + z, stream=0)
+        b.init()
+        b.sweep(2)
+        if fused is None:
+            b.round_fusion(True)
+        b.sweep(2)
+        runs.append([(b.chain(z).get_fields(), b.chain(z).get_green(), b.chain(z).get_phase()) for z in range(3)])
+        b.close()
+    for other in runs[1:]:
+        for (f0, g0, p0), (f1, g1, p1) in zip(runs[0], other):
+            assert np.array_equal(f0, f1)
+            assert rel(g1, g0) < TOL
+            assert abs(p1 - p0) < TOL
+
+
 def test_philox_stream_is_deterministic(rt):
     ham = hubbard(L1=4, L2=4, beta=2.0, nwrap=10)
     f = fields_for(ham, 5)
