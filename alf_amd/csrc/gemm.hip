@@ -7,9 +7,8 @@
 // does not depend on scheduling).  Each wave holds a 16 x 16 sub-tile as two
 // (re, im) v_mfma_f64_16x16x4f64 accumulators.  K advances in stages of 16 KS
 // through LDS, where the operands are split into real and imaginary planes so each
-// MFMA consumes one plane; a complex product costs 4 MFMAs per k-step of 4 (3 in the
-// 64 x 64 batched kernel, which forms (Ar + Ai)(Br + Bi) from the staged planes), a
-// real x complex one 2.  The host picks the tile shape per launch: at Ndim = 256 a
+// MFMA consumes one plane; a complex x complex product costs 3 MFMAs per k-step of 4 (3M:
+// Ar Br, Ai Bi and (Ar + Ai)(Br + Bi), combined after the K loop), a real x complex one 2.  The host picks the tile shape per launch: at Ndim = 256 a
 // 32 x 32 tiling leaves most of the 256 CUs idle, a 16 x 16 tiling with KS = 4 fills
 // them with a quarter of the per-wave K.
 //
@@ -217,6 +216,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
 
   typename M::Acc acc_re = {0, 0, 0, 0};
   typename M::Acc acc_im = {0, 0, 0, 0};
+  typename M::Acc acc_t2 = {0, 0, 0, 0};   // complex x complex: Ai Bi (3M)
 
   Stage<TM, TN, BK, A_CPLX, B_CPLX> st;
 
@@ -233,19 +233,23 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
       E b_im = B_CPLX ? Bs_im[kk + lk][wc + lr] : (E)0;
       if (B_CPLX && g.bs) scale_b(bsl[kk + lk], b_re, b_im);
       acc_re = M::mac(a_re, b_re, acc_re);
-      if (B_CPLX) {
+      if (A_CPLX && B_CPLX) {
+        // 3M: acc_t2 = Ai Bi, acc_im = (Ar + Ai)(Br + Bi), combined after the K loop
+        const E a_im = As_im[kk + lk][wr + lr];
+        acc_t2 = M::mac(a_im, b_im, acc_t2);
+        acc_im = M::mac(a_re + a_im, b_re + b_im, acc_im);
+      } else if (B_CPLX) {
         acc_im = M::mac(a_re, b_im, acc_im);
-        if (A_CPLX) {
-          const E a_im = As_im[kk + lk][wr + lr];
-          acc_re = M::mac(-a_im, b_im, acc_re);
-          acc_im = M::mac(a_im, b_re, acc_im);
-        }
       } else if (A_CPLX) {
         const E a_im = As_im[kk + lk][wr + lr];
         acc_im = M::mac(a_im, b_re, acc_im);
       }
     }
     __syncthreads();
+  }
+  if (A_CPLX && B_CPLX) {
+    acc_im = acc_im - acc_re - acc_t2;
+    acc_re = acc_re - acc_t2;
   }
 
   // ---- epilogue: register r of a lane holds row wr + M::row(r, lane / 16) of column
@@ -557,7 +561,8 @@ template <bool F32, int BK>
 void launch_big_p(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)cdivl(g.M, 64), (unsigned)cdivl(g.N, 64), gemm_zdim(g));
   const bool bc = g.b_cplx || g.bs != nullptr;
-  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true, F32, BK>), grid, dim3(NT), 0, s, g);
+  // complex x complex always in K stages of 16 (32 would not fit two waves per SIMD)
+  if (g.a_cplx && bc) hipLaunchKernelGGL((gemm_big_kernel<true, true, F32, 16>), grid, dim3(NT), 0, s, g);
   else if (g.a_cplx) hipLaunchKernelGGL((gemm_big_kernel<true, false, F32, BK>), grid, dim3(NT), 0, s, g);
   else if (bc) hipLaunchKernelGGL((gemm_big_kernel<false, true, F32, BK>), grid, dim3(NT), 0, s, g);
   else hipLaunchKernelGGL((gemm_big_kernel<false, false, F32, BK>), grid, dim3(NT), 0, s, g);
@@ -677,7 +682,10 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     }
   };
 
-  typename M::Acc acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
+  // complex x complex in three real products (3M): acc_re = Ar Br, acc_t2 = Ai Bi,
+  // acc_im = (Ar + Ai)(Br + Bi); re = acc_re - acc_t2, im = acc_im - acc_re - acc_t2
+  constexpr bool M3 = A_CPLX && B_CPLX;
+  typename M::Acc acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0}, acc_t2 = {0, 0, 0, 0};
   cplx av[U], bv[U], sv[U];
   load_batch(kb, av, bv, sv);
   finish(av, bv, sv);
@@ -686,16 +694,23 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
     load_batch(k0 + 4 * U, an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      acc_re = M::mac((E)av[u].x, (E)bv[u].x, acc_re);
-      if (B_CPLX) acc_im = M::mac((E)av[u].x, (E)bv[u].y, acc_im);
-      if (A_CPLX) {
-        if (B_CPLX) acc_re = M::mac((E)-av[u].y, (E)bv[u].y, acc_re);
-        acc_im = M::mac((E)av[u].y, (E)bv[u].x, acc_im);
+      if (M3) {
+        acc_re = M::mac((E)av[u].x, (E)bv[u].x, acc_re);
+        acc_t2 = M::mac((E)av[u].y, (E)bv[u].y, acc_t2);
+        acc_im = M::mac((E)(av[u].x + av[u].y), (E)(bv[u].x + bv[u].y), acc_im);
+      } else {
+        acc_re = M::mac((E)av[u].x, (E)bv[u].x, acc_re);
+        if (B_CPLX) acc_im = M::mac((E)av[u].x, (E)bv[u].y, acc_im);
+        if (A_CPLX) acc_im = M::mac((E)av[u].y, (E)bv[u].x, acc_im);
       }
     }
     finish(an, bn, sn);
 #pragma unroll
     for (int u = 0; u < U; ++u) { av[u] = an[u]; bv[u] = bn[u]; }
+  }
+  if (M3) {
+    acc_im = acc_im - acc_re - acc_t2;
+    acc_re = acc_re - acc_t2;
   }
   // register r of a lane holds row M::row(r, lane / 16) of the tile
   if (NW > 1) {

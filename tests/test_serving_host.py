@@ -152,3 +152,15 @@ def test_poisson_stream_and_summary():
     assert s["requests"] == 6 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
     assert abs(s["aggregate_sweeps_per_s"] - 12 / srv.wall) < 1e-9
     assert all(r.latency >= 2 * SWEEP_S for r in res)
+
+
+def test_lane_progress_notes():
+    # a lane reports every note_every sweeps from its own thread, with the requests it holds
+    notes = []
+    be = FakeBackend({4: 10, 6: 20})
+    srv = Server(memory_budget=1000, backend=be, lane_note=lambda ln, n, rids: notes.append((ln.key[1], n, rids)),
+                 note_every=2)
+    srv.run([Request(0, 4, 1.0, 4.0, seed=0, nsweeps=5, arrival=0.0),
+             Request(1, 4, 1.0, 3.0, seed=1, nsweeps=3, arrival=0.0)])
+    assert [n for _, n, _ in notes] == [2, 4]
+    assert notes[0] == (4, 2, [0, 1]) and notes[1] == (4, 4, [0])
