@@ -1,7 +1,7 @@
 // Error-free split products on the gfx950 int8 matrix cores: the hopping wraps of G at
 // FP64-class accuracy (ALFQ_PREC_INT8_SPLIT).
 //
-// Split (oz_split_kernel): a workgroup takes 16 rows (or columns) of one chain's operand,
+// Split (oz_split_kernel): a workgroup takes 16 rows (or 8 columns) of one chain's operand,
 // finds each one's largest |re|, |im| over the contraction index, and writes OZ_S rounded
 // base-128 digits of every entry scaled by 2^-e into int8 planes laid out by k-block of 32
 // (every digit of a block contiguous; zero-padded to a multiple of 64), with the exponent e.
