@@ -258,6 +258,43 @@ def traffic_summary(fetch_csv: str, write_csv, copy_fetch_csv: str, copy_write_c
     return "\n".join(lines)
 
 
+def mfma_summary(db_path: str, clock_ghz: float = 2.4) -> str:
+    """Per-kernel matrix-core use from one rocprofv3 counter pass of SQ_VALU_MFMA_BUSY_CYCLES
+    (summed over the SIMDs) and SQ_INSTS_VALU_MFMA_MOPS_F64 (FP64 MFMA operations / 512), read
+    from the pass's database: the FP64 MFMA rate = 512 x MOPS over the dispatch's duration, and
+    the MFMA-busy fraction of the SIMDs the dispatch can occupy (4 per CU, one CU per workgroup
+    up to 256) at clock_ghz.  (GRBM_GUI_ACTIVE of a dispatch counts the whole collection window
+    of a serialised dispatch, so the duration is the denominator.)"""
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select dispatch_id, kernel_name, grid_size, workgroup_size, counter_name, value, duration "
+                       "from counters_collection").fetchall()
+    con.close()
+    disp = collections.defaultdict(dict)
+    for did, name, gs, ws, cname, val, dur in rows:
+        d = disp[did]
+        d["name"] = name.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0].strip()
+        d["wg"] = gs // max(ws, 1)
+        d["ns"] = dur
+        d[cname] = val
+    agg = collections.defaultdict(lambda: [0, 0.0, 0.0, 0.0, 0.0])
+    for d in disp.values():
+        a = agg[d["name"]]
+        a[0] += 1
+        a[1] += d.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
+        a[2] += 512.0 * d.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0)
+        a[3] += d["ns"]
+        a[4] += d["ns"] * min(d["wg"], 256) * 4 * clock_ghz
+    total = sum(a[3] for a in agg.values()) or 1.0
+    lines = ["| Kernel | Dispatches | Average us | FP64 MFMA TF/s | MFMA busy (occupied SIMDs) | Time share |",
+             "|---|---|---|---|---|---|"]
+    for name, a in sorted(agg.items(), key=lambda kv: -kv[1][3])[:16]:
+        lines.append(f"| `{name}` | {a[0]} | {a[3] / a[0] / 1e3:.2f} | {a[2] / a[3] / 1e3:.2f} | "
+                     f"{100 * a[1] / a[4]:.1f}% | {100 * a[3] / total:.1f}% |")
+    return "\n".join(lines)
+
+
 def shim_slices() -> None:
     """Per-slice time of alfq_wrapgrup with G on the device and with G crossing PCIe both
     ways around every call (PHASE is returned by every call in both forms)."""
@@ -310,6 +347,10 @@ if __name__ == "__main__":
         # traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv [--mib N]
         mib = int(sys.argv[sys.argv.index("--mib") + 1]) if "--mib" in sys.argv else 256
         print(traffic_summary(*sys.argv[2:6], copy_mib=mib))
+    elif len(sys.argv) >= 2 and sys.argv[1] == "mfma":
+        # mfma RESULTS.db [--clock-ghz F]
+        print(mfma_summary(sys.argv[2], float(sys.argv[sys.argv.index("--clock-ghz") + 1]) if "--clock-ghz" in sys.argv
+                           else 2.4))
     elif len(sys.argv) >= 2 and sys.argv[1] == "shim":
         shim_slices()
     else:
