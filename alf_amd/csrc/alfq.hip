@@ -173,7 +173,6 @@ struct alfq_ctx {
   char* oz_buf = nullptr;
   long long oz_cs = 0;
   OzOperand oz_e_row, oz_e_col, oz_ei_row, oz_ei_col, oz_x;
-  bool oz_shared = false;                // one copy of the hopping digits for the whole batch
   bool symm = false;                     // Symm = .T.: measurements on Hop_mod_Symm(G)
   // equal-time measurements
   ObsDev obs;
@@ -564,7 +563,6 @@ int oz_split_hopping(alfq_ctx* c) {
   const int nz = shared ? 1 : c->B;
   OzOperand* ops[4] = {&c->oz_e_row, &c->oz_e_col, &c->oz_ei_row, &c->oz_ei_col};
   for (OzOperand* o : ops) o->stride = shared ? 0 : c->oz_cs;
-  c->oz_shared = shared;
   int st;
   if ((st = oz_split(c->expT, !c->t_real, n, n, n, true, nullptr, c->cs, c->oz_e_row, nz, c->s)) ||
       (st = oz_split(c->expT, !c->t_real, n, n, n, false, nullptr, c->cs, c->oz_e_col, nz, c->s)) ||

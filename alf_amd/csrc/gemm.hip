@@ -5,12 +5,12 @@
 // the waves are arranged WM x WN x KS, so with KS > 1 the waves of one tile split
 // K and their partial tiles are summed through LDS in a fixed order (the result
 // does not depend on scheduling).  Each wave holds a 16 x 16 sub-tile as two
-// (re, im) v_mfma_f64_16x16x4f64 accumulators.  K advances in stages of 16 KS
+// (re, im) v_mfma_f64_16x16x4f64 accumulators (three for complex x complex).  K advances in stages of 16 KS
 // through LDS, where the operands are split into real and imaginary planes so each
 // MFMA consumes one plane; a complex x complex product costs 3 MFMAs per k-step of 4 (3M:
-// Ar Br, Ai Bi and (Ar + Ai)(Br + Bi), combined after the K loop), a real x complex one 2.  The host picks the tile shape per launch: at Ndim = 256 a
-// 32 x 32 tiling leaves most of the 256 CUs idle, a 16 x 16 tiling with KS = 4 fills
-// them with a quarter of the per-wave K.
+// Ar Br, Ai Bi and (Ar + Ai)(Br + Bi), combined after the K loop), a real x complex one 2.
+// The host picks the tile shape per launch: at Ndim = 256 a 32 x 32 tiling leaves most of
+// the 256 CUs idle, a 16 x 16 tiling with KS = 4 fills them with a quarter of the per-wave K.
 //
 // This one kernel carries every dense product of the sweep:
 //   * hopping wraps  G <- D exp(-dtau T) G exp(dtau T) D^-1       (real T: 2 MFMA/step)
