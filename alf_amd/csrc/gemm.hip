@@ -47,9 +47,11 @@ template <> struct Mma<true> {
   __device__ __forceinline__ static int row(int r, int lk) { return 4 * lk + r; }
 };
 
-// moves every operand pointer to the arena of the chain bz / batch
-__device__ __forceinline__ void chain_operands(GemmArgs& g, int bz) {
-  if (g.nchain <= 1) return;
+// moves every operand pointer to the arena of the chain bz / batch; true when the product is
+// the identity slice of a chain beyond its own Ltrot (the hopping operand is the identity) and
+// the workgroup can write its tile as a copy of the other operand (identity_tile)
+__device__ __forceinline__ bool chain_operands(GemmArgs& g, int bz) {
+  if (g.nchain <= 1) return false;
   const long long off = (long long)(bz / (g.batch > 1 ? g.batch : 1)) * g.cstride;
   auto mv = [off](auto p) { return p ? (decltype(p))((const char*)p + off) : p; };
   g.A = mv(g.A); g.B = mv(g.B); g.C = (void*)mv((const char*)g.C);
@@ -59,6 +61,40 @@ __device__ __forceinline__ void chain_operands(GemmArgs& g, int bz) {
   if (g.lz && g.slice >= g.lz[bz / (g.batch > 1 ? g.batch : 1)]) {
     if (g.hop_side == 1) g.A = g.ident;
     else if (g.hop_side == 2) g.B = g.ident;
+    // the identity is K x K: M = K when it is opA, N = K when it is opB
+    return ((g.hop_side == 1 && g.M == g.K) || (g.hop_side == 2 && g.N == g.K)) && g.batch <= 1 && !g.amap &&
+           !g.bmap && !g.kdev && !g.yhat;
+  }
+  return false;
+}
+
+// The tile [i0, i0 + tm) x [j0, j0 + tn) of C = alpha rs (I X) cs + beta C when the hopping
+// operand is the identity: X = diag(bs) opB for hop_side 1, opA diag(bs) for hop_side 2, each element
+// with the epilogue of the product.  Exactly the product's value, without its MFMA work.
+__device__ void identity_tile(const GemmArgs& g, int i0, int j0, int tm, int tn, int tid, int nt) {
+  cplx* C = (cplx*)g.C;
+  for (int e = tid; e < tm * tn; e += nt) {
+    const int i = i0 + e % tm, j = j0 + e / tm;
+    if (i >= g.M || j >= g.N) continue;
+    cplx v;
+    if (g.hop_side == 1) {
+      const long long off = (g.opB == OP_N) ? i + (long long)j * g.ldb : j + (long long)i * g.ldb;
+      v = g.b_cplx ? ((const cplx*)g.B)[off] : make_double2(((const double*)g.B)[off], 0.0);
+      if (g.opB == OP_C) v.y = -v.y;
+      if (g.bs) v = cmul(g.bs[i], v);
+    } else {
+      const long long off = (g.opA == OP_N) ? i + (long long)j * g.lda : j + (long long)i * g.lda;
+      v = g.a_cplx ? ((const cplx*)g.A)[off] : make_double2(((const double*)g.A)[off], 0.0);
+      if (g.opA == OP_C) v.y = -v.y;
+      if (g.bs) v = cmul(v, g.bs[j]);
+    }
+    v = cmul(g.alpha, v);
+    if (g.rs) v = cmul(g.rs[i], v);
+    if (g.rs_real) v = cscale(v, g.rs_real[i]);
+    if (g.cs) v = cmul(v, g.cs[j]);
+    const long long o = i + (long long)j * g.ldc;
+    if (g.beta.x != 0.0 || g.beta.y != 0.0) v = cadd(v, cmul(g.beta, C[o]));
+    C[o] = v;
   }
 }
 
@@ -195,7 +231,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TM;
   const int j0 = blockIdx.y * TN;
-  chain_operands(g, blockIdx.z);
+  if (chain_operands(g, blockIdx.z)) { identity_tile(g, i0, j0, TM, TN, tid, blockDim.x); return; }
   const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
@@ -312,7 +348,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_big_kernel(GemmArgs g) {
   xcd_order(bx, by, bz);
   const int i0 = bx * TM;
   const int j0 = by * TN;
-  chain_operands(g, bz);
+  if (chain_operands(g, bz)) { identity_tile(g, i0, j0, TM, TN, tid, blockDim.x); return; }
   const int z = bz % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
@@ -592,7 +628,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_direct_kernel(GemmArgs g) {
   const int wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-  chain_operands(g, blockIdx.z);
+  if (chain_operands(g, blockIdx.z)) { identity_tile(g, i0, j0, 16, 16, tid, blockDim.x); return; }
   const int z = blockIdx.z % (g.batch > 1 ? g.batch : 1);
   int K = g.K;
   if (g.kdev) {
