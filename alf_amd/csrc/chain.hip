@@ -20,6 +20,7 @@
 #include <hip/hip_ext.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "alfq_chain.h"
 
@@ -368,24 +369,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     have_records = true;
   };
   // W = C D^T into gs[1] (zero outside m x m): W(l, j) = sum_i C(l, i) D(j, i)
+  // Column j of C and D is zero for j >= m, so of a thread's columns gq + 8 q only q < live_q(m)
+  // can be nonzero in a product with them (the same bound for every lane of the wave).  The
+  // products below are instantiated per bound (with_count), so their loops stay unrolled with
+  // every load of a step issued ahead of its FMAs.
+  auto live_q = [&](int mm) {
+    const int q = (mm - 2 * (tid >> 6) + 7) >> 3;
+    return __builtin_amdgcn_readfirstlane(q < 0 ? 0 : (q > 4 ? 4 : q));
+  };
+  auto with_count = [](int c, auto&& f) {
+    switch (c) {
+      case 1: f(std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      default: break;
+    }
+  };
   auto form_w = [&]() {
     cplx acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
-    for (int i0 = 0; i0 < m; i0 += 4) {
-      cplx cli[4], dv[4][4];
+    with_count(live_q(m), [&](auto QC) {
+      constexpr int QN = decltype(QC)::value;
+      for (int i0 = 0; i0 < m; i0 += 4) {
+        cplx cli[4], dv[4][QN];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        cli[t] = Cm[i0 + t][ga];
+        for (int t = 0; t < 4; ++t) {
+          cli[t] = Cm[i0 + t][ga];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) dv[t][q] = Dm[i0 + t][gq + 8 * q];
+          for (int q = 0; q < QN; ++q) dv[t][q] = Dm[i0 + t][gq + 8 * q];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int q = 0; q < QN; ++q) acc[q] = cfma(cli[t], dv[t][q], acc[q]);
+        }
       }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = cfma(cli[t], dv[t][q], acc[q]);
-      }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < 4; ++q) gs[1][ga + SB_K * (gq + 8 * q)] = acc[q];
   };
@@ -485,27 +506,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       cplx ax[4], ay[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) { ax[q] = make_double2(0.0, 0.0); ay[q] = make_double2(0.0, 0.0); }
-      for (int l0 = 0; l0 < m; l0 += 4) {
-        cplx gl[4], yl[4], cv[4][4], dv[4][4];
+      with_count(live_q(m), [&](auto QC) {
+        constexpr int QN = decltype(QC)::value;
+        for (int l0 = 0; l0 < m; l0 += 4) {
+          cplx gl[4], yl[4], cv[4][QN], dv[4][QN];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          gl[t] = gs[1][ga + SB_V * (l0 + t)];
-          yl[t] = UW[ga + SB_V * (l0 + t)];
+          for (int t = 0; t < 4; ++t) {
+            gl[t] = gs[1][ga + SB_V * (l0 + t)];
+            yl[t] = UW[ga + SB_V * (l0 + t)];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            cv[t][q] = Cm[gq + 8 * q][l0 + t];
-            dv[t][q] = Dm[gq + 8 * q][l0 + t];
+            for (int q = 0; q < QN; ++q) {
+              cv[t][q] = Cm[gq + 8 * q][l0 + t];
+              dv[t][q] = Dm[gq + 8 * q][l0 + t];
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+#pragma unroll
+            for (int q = 0; q < QN; ++q) {
+              ax[q] = cfma(gl[t], cv[t][q], ax[q]);
+              ay[q] = cfma(yl[t], dv[t][q], ay[q]);
+            }
           }
         }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            ax[q] = cfma(gl[t], cv[t][q], ax[q]);
-            ay[q] = cfma(yl[t], dv[t][q], ay[q]);
-          }
-        }
-      }
+      });
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         Xb[gq + 8 * q][ga] = ax[q];
@@ -618,59 +642,74 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const int mat = tid >> 7, l = (tid >> 2) & (SB_K - 1), c4 = tid & 3;
         cplx(*Mm)[SB_K] = mat ? Dm : Cm;
         cplx(*Lold)[SB_V] = mat ? Xb : Yb;
-        cplx acc[4];
-        int pj[4];
+        // columns jj < 4 TN, TN = ceil(mb / 4), cover the block's flips
+        with_count(__builtin_amdgcn_readfirstlane((mb + 3) >> 2), [&](auto TC) {
+          constexpr int TN = decltype(TC)::value;
+          cplx acc[TN];
+          int pj[TN];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          acc[t] = make_double2(0.0, 0.0);
-          const int jj = c4 + 4 * t;
-          pj[t] = s_aloc[jj < mb ? jj : 0];
-        }
-        for (int i0 = 0; i0 < m; i0 += 4) {
-          cplx mli[4], lv[4][4];
+          for (int t = 0; t < TN; ++t) {
+            acc[t] = make_double2(0.0, 0.0);
+            const int jj = c4 + 4 * t;
+            pj[t] = s_aloc[jj < mb ? jj : 0];
+          }
+          for (int i0 = 0; i0 < m; i0 += 4) {
+            cplx mli[4], lv[4][TN];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            mli[u] = Mm[i0 + u][l];
+            for (int u = 0; u < 4; ++u) {
+              mli[u] = Mm[i0 + u][l];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) lv[u][t] = Lold[i0 + u][pj[t]];
+              for (int t = 0; t < TN; ++t) lv[u][t] = Lold[i0 + u][pj[t]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+              for (int t = 0; t < TN; ++t) acc[t] = cfma(mli[u], lv[u][t], acc[t]);
+            }
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = cfma(mli[u], lv[u][t], acc[t]);
-          }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) Pre[(mat * SB_A + c4 + 4 * t) * SB_K + l] = acc[t];
+          for (int t = 0; t < TN; ++t) Pre[(mat * SB_A + c4 + 4 * t) * SB_K + l] = acc[t];
+        });
       }
       __syncthreads();
-      // (2) the in-block recurrence, one thread per (matrix, row), new entries in registers
+      // (2) the in-block recurrence, one thread per (matrix, row), new entries in registers.
+      //     Right-looking: once entry jj is final its terms go into the sums of every later
+      //     column, and each table entry of row jj, once used, is replaced by the same column's
+      //     entry of row jj + 1, so the next row's reads are in flight while this row's terms
+      //     are added.  Each sum still adds its terms in the order ii = 0, 1, ...  Columns up to
+      //     4 ceil(mb / 4) are carried without branches; those at or beyond mb are not stored.
       if (tid < 2 * SB_K) {
         const int mat = tid >> 5, l = tid & (SB_K - 1);
         const cplx* Lnew = mat ? Us : Ws;   // X(P_j, i) = u_i(p_j) / Y(P_j, i) = w_i(p_j)
-        cplx nm[SB_A];
-#pragma unroll
-        for (int jj = 0; jj < SB_A; ++jj) {
-          nm[jj] = make_double2(0.0, 0.0);
-          if (jj < mb) {
-            const int j = m + jj, pjj = s_aloc[jj];
-            cplx acc = Pre[(mat * SB_A + jj) * SB_K + l];
-#pragma unroll
-            for (int ii = 0; ii < jj; ++ii) acc = cfma(nm[ii], Lnew[ii * SB_V + pjj], acc);
-            if (l <= j) {
-              if (mat) {
-                nm[jj] = make_double2((l == j ? 1.0 : 0.0) - acc.x, -acc.y);
-              } else {
-                const cplx xj = s_x[j];
-                nm[jj] = cmul(make_double2(-xj.x, -xj.y), make_double2((l == j ? 1.0 : 0.0) + acc.x, acc.y));
-              }
-            }
-          }
-        }
         cplx(*Mm)[SB_K] = mat ? Dm : Cm;
+        with_count(__builtin_amdgcn_readfirstlane((mb + 3) >> 2), [&](auto TC) {
+          constexpr int JN = 4 * decltype(TC)::value;
+          cplx acc[JN], lr[JN];
+          int pj[JN];
 #pragma unroll
-        for (int jj = 0; jj < SB_A; ++jj)
-          if (jj < mb) Mm[m + jj][l] = nm[jj];
+          for (int jj = 0; jj < JN; ++jj) {
+            pj[jj] = s_aloc[jj < mb ? jj : 0];
+            acc[jj] = Pre[(mat * SB_A + jj) * SB_K + l];
+          }
+#pragma unroll
+          for (int kk = 1; kk < JN; ++kk) lr[kk] = Lnew[pj[kk]];
+#pragma unroll
+          for (int jj = 0; jj < JN; ++jj) {
+            const int j = m + jj;
+            const cplx xj = s_x[j < SB_K ? j : SB_K - 1];
+            const double dl = l == j ? 1.0 : 0.0;
+            const cplx vd = make_double2(dl - acc[jj].x, -acc[jj].y);
+            const cplx vc = cmul(make_double2(-xj.x, -xj.y), make_double2(dl + acc[jj].x, acc[jj].y));
+            const cplx v = (jj < mb && l <= j) ? (mat ? vd : vc) : make_double2(0.0, 0.0);
+            if (jj < mb) Mm[j][l] = v;
+#pragma unroll
+            for (int kk = jj + 1; kk < JN; ++kk) {
+              acc[kk] = cfma(v, lr[kk], acc[kk]);
+              if (kk > jj + 1) lr[kk] = Lnew[(jj + 1) * SB_V + pj[kk]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        });
       }
       m += mb;
     }
