@@ -322,6 +322,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--serve-sweeps", type=int, default=20)
     args = ap.parse_args()
+    # several batches and serving lanes run on their own streams: give them hardware queues
+    # of their own (before the first GPU call; the single chain uses one stream either way)
+    from alf_amd.serving import use_hw_queues
+    use_hw_queues()
     if args.only_headline:
         args.batch, args.no_mixed, args.no_checkerboard, args.no_configs = 0, True, True, True
         args.serve_requests, args.no_cpu_baseline, args.c4_chains = 0, True, 0
