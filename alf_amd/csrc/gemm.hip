@@ -70,7 +70,9 @@ __device__ __forceinline__ bool chain_operands(GemmArgs& g, int bz) {
 
 // The tile [i0, i0 + tm) x [j0, j0 + tn) of C = alpha rs (I X) cs + beta C when the hopping
 // operand is the identity: X = diag(bs) opB for hop_side 1, opA diag(bs) for hop_side 2, each element
-// with the epilogue of the product.  Exactly the product's value, without its MFMA work.
+// with the epilogue of the product.  The product's value without its MFMA work: in FP64 the
+// MFMA product with an identity adds only zeros, and with FP32 operands (prec 1) the copy
+// keeps the double values the product would have rounded.
 __device__ void identity_tile(const GemmArgs& g, int i0, int j0, int tm, int tn, int tid, int nt) {
   cplx* C = (cplx*)g.C;
   for (int e = tid; e < tm * tn; e += nt) {
