@@ -102,11 +102,31 @@ __global__ __launch_bounds__(1024) void stabilise_kernel(const cplx* __restrict_
   double mx = 0.0, sm = 0.0;
   int nan = 0;
   if (check) {
-    const long long nn = (long long)n * n;
-    for (long long i = tid; i < nn; i += blockDim.x) {
+    // |a - b| as sqrt(dx^2 + dy^2): entries of G are O(1), and an overflow gives inf, which
+    // fails the XMAX check as the value itself would; four entries per step keep loads ahead
+    const int nn = n * n;
+    int i = tid;
+    for (; i + 3 * (int)blockDim.x < nn; i += 4 * blockDim.x) {
+      cplx a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = Gf[i + u * blockDim.x];
+        b[u] = Gw[i + u * blockDim.x];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (a[u].x != a[u].x || a[u].y != a[u].y || b[u].x != b[u].x || b[u].y != b[u].y) nan = 1;
+        const double dx = a[u].x - b[u].x, dy = a[u].y - b[u].y;
+        const double d = sqrt(dx * dx + dy * dy);
+        mx = fmax(mx, d);
+        sm += d;
+      }
+    }
+    for (; i < nn; i += blockDim.x) {
       const cplx a = Gf[i], b = Gw[i];
       if (a.x != a.x || a.y != a.y || b.x != b.x || b.y != b.y) nan = 1;
-      const double d = hypot(a.x - b.x, a.y - b.y);
+      const double dx = a.x - b.x, dy = a.y - b.y;
+      const double d = sqrt(dx * dx + dy * dy);
       mx = fmax(mx, d);
       sm += d;
     }

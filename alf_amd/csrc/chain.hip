@@ -767,18 +767,28 @@ __global__ void dvec_kernel(const int8_t* __restrict__ f, int n_op, int s0, int 
   dconj[idx] = cconj(p);
 }
 
-// Op_phase: prod over (op, slice) of exp(i Im(g alpha phi(f))) as a block product.
+// Op_phase: prod over (op, slice) of exp(i Im(g alpha phi(f))) as a block product.  A thread
+// takes operators op = tid, tid + 256, ... over all count / n_op slices, as four partial
+// products over the slices (so the factor loads run ahead of the multiplications), combined
+// in a fixed order.
 __global__ void opphase_kernel(const int8_t* __restrict__ f, long long count, int n_op,
                                const cplx* __restrict__ phasefac, cplx* __restrict__ out, long long cs) {
   __shared__ cplx sh[256];
   f = chain_ptr(f, cs);
   phasefac = chain_ptr(phasefac, cs);
   out = chain_ptr(out, cs);
+  const long long nsl = count / n_op;
   cplx p = make_double2(1.0, 0.0);
-  for (long long i = threadIdx.x; i < count; i += blockDim.x) {
-    const int op = (int)(i % n_op);
-    const int sp = f[i];
-    p = cmul(p, phasefac[op * 5 + (sp + 2)]);
+  for (int op = threadIdx.x; op < n_op; op += blockDim.x) {
+    const cplx* pf = phasefac + op * 5 + 2;
+    cplx q[4] = {make_double2(1.0, 0.0), make_double2(1.0, 0.0), make_double2(1.0, 0.0), make_double2(1.0, 0.0)};
+    long long sl = 0;
+    for (; sl + 4 <= nsl; sl += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = cmul(q[u], pf[f[(sl + u) * n_op + op]]);
+    }
+    for (; sl < nsl; ++sl) q[0] = cmul(q[0], pf[f[sl * n_op + op]]);
+    p = cmul(p, cmul(cmul(q[0], q[1]), cmul(q[2], q[3])));
   }
   sh[threadIdx.x] = p;
   __syncthreads();

@@ -761,22 +761,42 @@ __global__ __launch_bounds__(PT) void qr_finish_kernel(const cplx* __restrict__ 
   cplx qd = make_double2(1.0, 0.0);
   for (int b = tid; b < nblk; b += blockDim.x) qd = cmul(qd, pdet[b]);
   qd = block_cprod(qd, sh);
+  // permutation parity: the parity of its inversions, counted by all threads from a copy
+  // of perm in LDS (n <= 4 PT; larger n walk the cycles in thread 0)
+  __shared__ int s_par;
+  int* sp = (int*)sh;
+  const bool lds_perm = n <= 4 * PT;
+  if (tid == 0) s_par = 0;
+  if (lds_perm)
+    for (int i = tid; i < n; i += blockDim.x) sp[i] = perm[i];
+  __syncthreads();
+  if (lds_perm) {
+    int odd = 0;
+    for (int i = tid; i < n; i += blockDim.x) {
+      const int pi = sp[i];
+      for (int j = i + 1; j < n; ++j) odd ^= (pi > sp[j]) ? 1 : 0;
+    }
+    if (odd) atomicXor(&s_par, 1);
+    __syncthreads();
+  }
   if (tid == 0) {
-    // permutation parity from the cycle decomposition
-    int sgn = 1;
-    // visited marks in LDS would cost n bytes; reuse sh as a bitmap for n <= 16384
-    unsigned* vis = (unsigned*)sh;
-    const int words = (n + 31) / 32;
-    for (int w = 0; w < words; ++w) vis[w] = 0u;
-    for (int i = 0; i < n; ++i) {
-      if (vis[i >> 5] & (1u << (i & 31))) continue;
-      int L = 0, nx = i;
-      while (!(vis[nx >> 5] & (1u << (nx & 31)))) {
-        vis[nx >> 5] |= 1u << (nx & 31);
-        ++L;
-        nx = perm[nx];
+    int sgn = s_par ? -1 : 1;
+    if (!lds_perm) {
+      // cycle decomposition; sh as a visited bitmap (n <= 16384)
+      sgn = 1;
+      unsigned* vis = (unsigned*)sh;
+      const int words = (n + 31) / 32;
+      for (int w = 0; w < words; ++w) vis[w] = 0u;
+      for (int i = 0; i < n; ++i) {
+        if (vis[i >> 5] & (1u << (i & 31))) continue;
+        int L = 0, nx = i;
+        while (!(vis[nx >> 5] & (1u << (nx & 31)))) {
+          vis[nx >> 5] |= 1u << (nx & 31);
+          ++L;
+          nx = perm[nx];
+        }
+        if ((L & 1) == 0) sgn = -sgn;
       }
-      if ((L & 1) == 0) sgn = -sgn;
     }
     cplx ph;
     if (mode <= 1) {
