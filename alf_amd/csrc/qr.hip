@@ -117,15 +117,25 @@ __global__ void sortperm_kernel(const double* __restrict__ norms, int n, int* __
   norms = chain_ptr(norms, cs);
   perm = chain_ptr(perm, cs);
   pinv = chain_ptr(pinv, cs);
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  // four lanes per column j, each counting every fourth i; the lanes' counts are summed with
+  // two exchanges inside the wave, so every lane runs the same number of rounds
+  const int q = threadIdx.x & 3, g = threadIdx.x >> 2, G = blockDim.x >> 2;
+  const int rounds = (n + G - 1) / G;
+  for (int it = 0; it < rounds; ++it) {
+    const int j0 = it * G + g;
+    const int j = j0 < n ? j0 : n - 1;
     const double x = norms[j];
     int r = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = q; i < n; i += 4) {
       const double y = norms[i];
       r += (y > x) || (y == x && i < j);
     }
-    perm[r] = j;
-    pinv[j] = r;
+    r += __shfl_xor(r, 1, 64);
+    r += __shfl_xor(r, 2, 64);
+    if (q == 0 && j0 < n) {
+      perm[r] = j;
+      pinv[j] = r;
+    }
   }
 }
 
@@ -858,12 +868,24 @@ __global__ __launch_bounds__(PT) void trinv_diag_kernel(const cplx* __restrict__
     x[i + NBMAX * j] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  // column j of the inverse by back substitution, one thread per column
-  for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    for (int i = j; i >= 0; --i) {
-      cplx acc = make_double2(i == j ? 1.0 : 0.0, 0.0);
-      for (int l = i + 1; l <= j; ++l) acc = csub(acc, cmul(s[i + NBMAX * l], x[l + NBMAX * j]));
-      x[i + NBMAX * j] = cdiv(acc, s[i + NBMAX * i]);
+  // column j of the inverse by back substitution, eight lanes per column (blockDim.x = 8
+  // NBMAX): lane q adds the terms l = i + 1 + q, i + 9 + q, ... of row i, the eight partial
+  // sums meet by three exchanges inside the lane group, and lane 0 of the group stores x(i, j),
+  // which the group reads back in later rows (LDS operations of one wave complete in order)
+  {
+    static_assert(PT == 8 * NBMAX, "the kernel is launched with eight lanes per column");
+    const int j = threadIdx.x >> 3, q = threadIdx.x & 7;
+    if (j < m) {
+      for (int i = j; i >= 0; --i) {
+        cplx acc = make_double2(0.0, 0.0);
+        for (int l = i + 1 + q; l <= j; l += 8) acc = cfma(s[i + NBMAX * l], x[l + NBMAX * j], acc);
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+          acc.x += __shfl_xor(acc.x, o, 8);
+          acc.y += __shfl_xor(acc.y, o, 8);
+        }
+        if (q == 0) x[i + NBMAX * j] = cdiv(make_double2((i == j ? 1.0 : 0.0) - acc.x, -acc.y), s[i + NBMAX * i]);
+      }
     }
   }
   __syncthreads();
@@ -931,7 +953,7 @@ static int qr_factor(QRWork& w, const cplx* S, int lds, const cplx* D, int ctran
   int st;
   hipLaunchKernelGGL(colnorm_kernel, dim3(nblocks(n, 4), 1, w.nchain), dim3(256), 0, s, S, n, lds, D, ctrans,
                      w.norms, w.cs);
-  hipLaunchKernelGGL(sortperm_kernel, dim3(1, 1, w.nchain), dim3(256), 0, s, w.norms, n, w.perm, w.pinv, w.cs);
+  hipLaunchKernelGGL(sortperm_kernel, dim3(1, 1, w.nchain), dim3(1024), 0, s, w.norms, n, w.perm, w.pinv, w.cs);
   hipLaunchKernelGGL(gather_kernel, dim3(nblocks((long long)n * n, 256), 1, w.nchain), dim3(256), 0, s, S, n, lds, D,
                      ctrans, w.perm, w.A, w.cs);
   HIP_OK(hipGetLastError());
