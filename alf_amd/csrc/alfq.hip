@@ -1608,6 +1608,29 @@ int alfq_set_gemm_policy(alfq_ctx* c, int32_t policy) {
   return ST_NONE;
 }
 
+int alfq_check_wrap_args(alfq_ctx* c, int32_t propose_s0, int32_t nt_sequential_start, int32_t nt_sequential_end,
+                         int32_t n_global_tau) {
+  if (!c) return ST_GENERIC;
+  const int no = (int)c->h_op_type.size();
+  if (nt_sequential_start != 1 || nt_sequential_end != no) {
+    alfq_set_error("wrap arguments: the engine updates every operator of a slice in sequence "
+                   "(Nt_sequential_start = 1, Nt_sequential_end = size(Op_V, 1))");
+    return ST_HAMILTONIAN;
+  }
+  if (n_global_tau != 0) {
+    alfq_set_error("wrap arguments: global moves in the slice (N_Global_tau > 0) are not supported");
+    return ST_HAMILTONIAN;
+  }
+  if (propose_s0)
+    for (int t : c->h_op_type)
+      if (t == 1) {
+        alfq_set_error("wrap arguments: Propose_S0 with Ising (type 1) operators changes the proposal "
+                       "probability; the engine proposes every visit");
+        return ST_HAMILTONIAN;
+      }
+  return ST_NONE;
+}
+
 int alfq_set_round_fusion(alfq_ctx* c, int32_t on) {
   if (!c) return ST_GENERIC;
   c->round_fusion = on != 0;

@@ -1,8 +1,9 @@
 !  ISO_C_BINDING interface of the gfx950 DQMC sweep engine (include/alfq.h) and
 !  drop-in shims with the argument lists of the reference routines they replace:
 !
-!    alfq_WRAPGRUP(GR, NTAU, PHASE)       <->  WRAPGRUP  (Prog/Wrapgr_mod.F90:81)
-!    alfq_WRAPGRDO(GR, NTAU, PHASE)       <->  WRAPGRDO  (Prog/Wrapgr_mod.F90:160)
+!    alfq_WRAPGRUP(GR, NTAU, PHASE[, Propose_S0, Nt_sequential_start, Nt_sequential_end,
+!                  N_Global_tau])         <->  WRAPGRUP  (Prog/Wrapgr_mod.F90:81)
+!    alfq_WRAPGRDO(GR, NTAU, PHASE[, ...]) <->  WRAPGRDO  (Prog/Wrapgr_mod.F90:160)
 !    alfq_WRAPUR(NTAU, NTAU1)             <->  WRAPUR    (Prog/wrapur_mod.F90:37)
 !    alfq_WRAPUL(NTAU1, NTAU)             <->  WRAPUL    (Prog/wrapul_mod.F90:36)
 !    alfq_CGR(Z, NVAR, GRUP)              <->  CGR       (Prog/cgr1_mod.F90:36)
@@ -67,7 +68,7 @@ module alfq_mod
             alfq_udv_assign, alfq_rng_set_tape, alfq_rng_set_philox, alfq_wrapgrup, alfq_wrapgrdo, &
             alfq_wrapur, alfq_wrapul, alfq_cgr, alfq_init, alfq_sweep, alfq_sync, alfq_control_get, &
             alfq_last_error_string, alfq_obs_setup, alfq_obs_enable, alfq_obs_reset, alfq_obs_get, &
-            alfq_set_checkerboard, alfq_tau_enable, alfq_tau_m, alfq_tau_get
+            alfq_set_checkerboard, alfq_tau_enable, alfq_tau_m, alfq_tau_get, alfq_check_wrap_args
   public :: alfq_WRAPGRUP_shim, alfq_WRAPGRDO_shim, alfq_WRAPUR_shim, alfq_WRAPUL_shim, alfq_CGR_shim
 
   interface
@@ -199,6 +200,12 @@ module alfq_mod
       integer(c_int32_t), value :: ntau
       complex(c_double_complex), intent(inout) :: phase
     end function
+    integer(c_int) function alfq_check_wrap_args(ctx, propose_s0, nt_start, nt_end, n_global_tau) &
+        bind(C, name="alfq_check_wrap_args")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: propose_s0, nt_start, nt_end, n_global_tau
+    end function
     integer(c_int) function alfq_wrapur(ctx, ntau, ntau1) bind(C, name="alfq_wrapur")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
@@ -275,12 +282,40 @@ contains
     enddo
   end function
 
+  ! The optional trailing arguments of WRAPGRUP/WRAPGRDO (Propose_S0, Nt_sequential_start,
+  ! Nt_sequential_end, N_Global_tau, Prog/Wrapgr_mod.F90:81, 160), when the caller passes
+  ! them, checked against what the engine does (alfq_check_wrap_args); all four or none
+  integer function wrap_args_status(ctx, Propose_S0, Nt_sequential_start, Nt_sequential_end, &
+                                    N_Global_tau) result(st)
+    type(c_ptr), intent(in) :: ctx
+    logical, intent(in), optional :: Propose_S0
+    integer, intent(in), optional :: Nt_sequential_start, Nt_sequential_end, N_Global_tau
+    integer(c_int32_t) :: ps0
+    st = 0
+    if (.not. (present(Propose_S0) .or. present(Nt_sequential_start) .or. present(Nt_sequential_end) &
+               .or. present(N_Global_tau))) return
+    if (.not. (present(Propose_S0) .and. present(Nt_sequential_start) .and. present(Nt_sequential_end) &
+               .and. present(N_Global_tau))) then
+      st = 1
+      return
+    endif
+    ps0 = 0
+    if (Propose_S0) ps0 = 1
+    st = alfq_check_wrap_args(ctx, ps0, int(Nt_sequential_start, c_int32_t), int(Nt_sequential_end, c_int32_t), &
+                              int(N_Global_tau, c_int32_t))
+  end function
+
   ! WRAPGRUP(GR, NTAU, PHASE): G(NTAU) -> G(NTAU+1) with the updates of slice NTAU+1
-  integer function alfq_WRAPGRUP_shim(ctx, GR, NTAU, PHASE) result(st)
+  integer function alfq_WRAPGRUP_shim(ctx, GR, NTAU, PHASE, Propose_S0, Nt_sequential_start, &
+                                      Nt_sequential_end, N_Global_tau) result(st)
     type(c_ptr), intent(in) :: ctx
     complex(kind=8), intent(inout) :: GR(:,:)
     integer, intent(in) :: NTAU
     complex(kind=8), intent(inout) :: PHASE
+    logical, intent(in), optional :: Propose_S0
+    integer, intent(in), optional :: Nt_sequential_start, Nt_sequential_end, N_Global_tau
+    st = wrap_args_status(ctx, Propose_S0, Nt_sequential_start, Nt_sequential_end, N_Global_tau)
+    if (st /= 0) return
     st = alfq_green_set(ctx, GR, int(size(GR, 1), c_int32_t))
     if (st /= 0) return
     st = alfq_wrapgrup(ctx, int(NTAU, c_int32_t), PHASE)
@@ -289,11 +324,16 @@ contains
   end function
 
   ! WRAPGRDO(GR, NTAU, PHASE): updates of slice NTAU, then G(NTAU) -> G(NTAU-1)
-  integer function alfq_WRAPGRDO_shim(ctx, GR, NTAU, PHASE) result(st)
+  integer function alfq_WRAPGRDO_shim(ctx, GR, NTAU, PHASE, Propose_S0, Nt_sequential_start, &
+                                      Nt_sequential_end, N_Global_tau) result(st)
     type(c_ptr), intent(in) :: ctx
     complex(kind=8), intent(inout) :: GR(:,:)
     integer, intent(in) :: NTAU
     complex(kind=8), intent(inout) :: PHASE
+    logical, intent(in), optional :: Propose_S0
+    integer, intent(in), optional :: Nt_sequential_start, Nt_sequential_end, N_Global_tau
+    st = wrap_args_status(ctx, Propose_S0, Nt_sequential_start, Nt_sequential_end, N_Global_tau)
+    if (st /= 0) return
     st = alfq_green_set(ctx, GR, int(size(GR, 1), c_int32_t))
     if (st /= 0) return
     st = alfq_wrapgrdo(ctx, int(NTAU, c_int32_t), PHASE)

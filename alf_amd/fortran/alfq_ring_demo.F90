@@ -89,8 +89,12 @@ program alfq_ring_demo
   call check(alfq_phase_get(ctx, phase))
   ! the first slices the way main.F90 calls WRAPGRUP (G on the host between calls)
   call check(alfq_udv_reset(ctx, ALFQ_UDVR, 'R'))
+  ! main.F90:729 passes Propose_S0, Nt_sequential_start/end and N_Global_tau; the engine takes
+  ! the full sequential pass without global moves and refuses anything else before any work
+  if (alfq_WRAPGRUP_shim(ctx, GR, 0, phase, .false., 1, n_op, 1) /= 6) stop 'N_Global_tau > 0 not refused'
+  if (alfq_WRAPGRUP_shim(ctx, GR, 0, phase, .false., 2, n_op, 0) /= 6) stop 'partial sequential pass not refused'
   do nt = 0, nwrap - 1
-    call check(alfq_WRAPGRUP_shim(ctx, GR, nt, phase))
+    call check(alfq_WRAPGRUP_shim(ctx, GR, nt, phase, .false., 1, n_op, 0))
   enddo
   ! the rest of this sweep is not run: reset to tau = 0 from a fresh init, then whole sweeps
   call check(alfq_ctx_destroy(ctx))

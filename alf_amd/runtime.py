@@ -37,7 +37,7 @@ EXPORTS = [
     "alfq_set_precision", "alfq_mfma_rate", "alfq_chain_bytes",
     "alfq_obs_setup", "alfq_obs_enable", "alfq_obs_reset", "alfq_obs_get", "alfq_rng_set_position",
     "alfq_set_checkerboard", "alfq_tau_enable", "alfq_tau_m", "alfq_tau_get", "alfq_gemm_noalloc", "alfq_chain_load",
-    "alfq_set_gemm_policy",
+    "alfq_set_gemm_policy", "alfq_check_wrap_args",
 ]
 
 
@@ -145,6 +145,7 @@ def load_library(path: str = LIB_PATH):
         "alfq_kernel_timing": [P, C.c_int32],
         "alfq_sweep_graphs": [P, C.c_int32],
         "alfq_set_round_fusion": [P, C.c_int32],
+        "alfq_check_wrap_args": [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
         "alfq_ctx_create_batch": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(P)],
         "alfq_set_chain": [P, C.c_int32],
         "alfq_chain_load": [P, C.c_int32, P, C.c_int32],
@@ -512,6 +513,14 @@ class Engine:
     def sweep_graphs(self, enable=True):
         """Replay sweeps as captured HIP graphs (default) or launch every kernel from the host."""
         self._c(self.lib.alfq_sweep_graphs(self.ctx, int(enable)))
+
+    def check_wrap_args(self, propose_s0=False, nt_sequential_start=1, nt_sequential_end=None, n_global_tau=0):
+        """The trailing WRAPGRUP/WRAPGRDO arguments of the reference (Propose_S0,
+        Nt_sequential_start/end, N_Global_tau) checked against what the engine does
+        (alfq_check_wrap_args): raises AlfqError (ERROR_HAMILTONIAN) for anything else."""
+        end = len(self.model.Op_V) if nt_sequential_end is None else nt_sequential_end
+        self._c(self.lib.alfq_check_wrap_args(self.ctx, int(bool(propose_s0)), int(nt_sequential_start), int(end),
+                                              int(n_global_tau)))
 
     def round_fusion(self, on=True):
         """One launch per delayed-update round for contexts of fewer than 32 chains (default)

@@ -164,6 +164,16 @@ int alfq_rng_set_position(alfq_ctx* ctx, int64_t pos);
 int alfq_wrapgrup(alfq_ctx* ctx, int32_t ntau, double* phase);
 /* WRAPGRDO(GR, NTAU, PHASE, ...) (Prog/Wrapgr_mod.F90:160-243), NTAU in [Ltrot, 1]. */
 int alfq_wrapgrdo(alfq_ctx* ctx, int32_t ntau, double* phase);
+/* The remaining arguments of WRAPGRUP/WRAPGRDO (Prog/Wrapgr_mod.F90:81, 160:
+ * Propose_S0, Nt_sequential_start, Nt_sequential_end, N_Global_tau; main.F90:729, 788 passes
+ * get_Propose_S0() etc.), checked against what the engine does: every operator of the slice
+ * in sequence (start 1, end size(Op_V, 1), lines 115 and 198), no global moves in the slice
+ * (N_Global_tau = 0, lines 149 and 190), and no S0-weighted proposals for Ising operators
+ * (Propose_S0 with type-1 operators, lines 127-132 and 212; with type-2 operators it changes
+ * nothing).  ERROR_HAMILTONIAN with a message when the caller asks for more.  The engine
+ * takes S0 = 1 (the Hubbard plugins'). */
+int alfq_check_wrap_args(alfq_ctx* ctx, int32_t propose_s0, int32_t nt_sequential_start,
+                         int32_t nt_sequential_end, int32_t n_global_tau);
 /* WRAPUR(NTAU, NTAU1, UDVR) (Prog/wrapur_mod.F90:37-128) on udvr. */
 int alfq_wrapur(alfq_ctx* ctx, int32_t ntau, int32_t ntau1);
 /* WRAPUL(NTAU1, NTAU, UDVL) (Prog/wrapul_mod.F90:36-133) on udvl. */

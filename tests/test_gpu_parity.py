@@ -265,6 +265,24 @@ def test_round_fusion_switch_keeps_the_trajectory(rt):
             assert abs(p1 - p0) < TOL
 
 
+def test_reference_wrap_arguments_are_checked(rt):
+    """The trailing arguments of WRAPGRUP/WRAPGRDO (Prog/Wrapgr_mod.F90:81, 160, as main.F90:729
+    passes them): the full sequential pass without global moves is what the engine does and is
+    accepted (Propose_S0 changes nothing for the discrete HS fields of type 2); a partial pass
+    and N_Global_tau > 0 are refused with ERROR_HAMILTONIAN, before any work."""
+    ham = hubbard(L1=4, L2=4, beta=1.0, nwrap=5)
+    e = rt.Engine(ham)
+    nop = len(ham.Op_V)
+    e.check_wrap_args(False, 1, nop, 0)
+    e.check_wrap_args(True, 1, nop, 0)
+    with pytest.raises(rt.AlfqError, match="in sequence") as ex:
+        e.check_wrap_args(False, 1, nop - 1, 0)
+    assert ex.value.code == 6
+    with pytest.raises(rt.AlfqError, match="global moves"):
+        e.check_wrap_args(False, 1, nop, 2)
+    e.close()
+
+
 def test_philox_stream_is_deterministic(rt):
     ham = hubbard(L1=4, L2=4, beta=2.0, nwrap=10)
     f = fields_for(ham, 5)
