@@ -822,14 +822,21 @@ __global__ __launch_bounds__(128) void round_flush_kernel(const cplx* __restrict
   W = chain_ptr_at(W, cs, z);
   amap = chain_ptr_at(amap, cs, z);
   mdev = chain_ptr_at(mdev, cs, z);
-  int m = *mdev;
-  m = m < kmax ? m : kmax;
-  m = m > 0 ? m : 0;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int i0 = blockIdx.x * RF_T, j0 = blockIdx.y * RF_T;
   const int gi = i0 + lr, gj = j0 + lr;
   const bool row_ok = gi < n, col_ok = gj < n;
+  // the rank m and the sites are loaded together (the sites' buffer holds at least kmax
+  // entries), so the gathers below wait for one load instead of two
+  int pa[FL_K / 4], py[4];
+#pragma unroll
+  for (int u = 0; u < FL_K / 4; ++u) pa[u] = 4 * u + lk < kmax ? amap[4 * u + lk] : 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) py[u] = 16 * wave + 4 * u + lk < kmax ? amap[16 * wave + 4 * u + lk] : 0;
+  int m = *mdev;
+  m = m < kmax ? m : kmax;
+  m = m > 0 ? m : 0;
 
   // X(gi, 16 wave + lr') = sum_k Gi(gi, P_k) W(k, 16 wave + lr'): A operand lane = row + 16 k,
   // B operand lane = column + 16 k, 8 k-steps of 4
@@ -837,7 +844,7 @@ __global__ __launch_bounds__(128) void round_flush_kernel(const cplx* __restrict
 #pragma unroll
   for (int u = 0; u < FL_K / 4; ++u) {
     const int k = 4 * u + lk;
-    const int P = k < m ? amap[k] : 0;
+    const int P = pa[u];
     xa[u] = (k < m && row_ok) ? Gi[gi + (long long)P * n] : make_double2(0.0, 0.0);
     xb[u] = k < m ? W[k + (long long)(16 * wave + lr) * FL_K] : make_double2(0.0, 0.0);
   }
@@ -848,7 +855,7 @@ __global__ __launch_bounds__(128) void round_flush_kernel(const cplx* __restrict
     const int k = 16 * wave + 4 * u + lk;
     yb[u] = make_double2(0.0, 0.0);
     if (k < m && col_ok) {
-      const int P = amap[k];
+      const int P = py[u];
       const cplx v = Gi[P + (long long)gj * n];
       yb[u] = make_double2((gj == P ? 1.0 : 0.0) - v.x, -v.y);
     }
